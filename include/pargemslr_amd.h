@@ -1,0 +1,244 @@
+/*
+ * pargemslr_amd: C ABI of the MI355X GeMSLR-preconditioned FGMRES solve path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status:
+ * 0 on success, the reference's PARGEMSLR_ERROR_* numbering for the shared
+ * conditions (ParGeMSLR/INC/pargemslr.hpp:9-21) and PGS_ERR_* (>= 200) for
+ * conditions of this library.  Nothing throws across the ABI.
+ *
+ * Numbering: values are double (real operators) or interleaved (re, im)
+ * double pairs (complex operators); device pointers are hipMalloc'd memory on
+ * the device the operator lives on; `stream` is a hipStream_t (NULL = default
+ * stream).  Multi-RHS vectors are row-interleaved: entry (row i, column c) of
+ * an n x nrhs block is at [i * nrhs + c].
+ *
+ * Ownership: setup and upload copy the caller's arrays; x, b, v, z are caller
+ * owned.  A device operator may be shared by several workspaces; calls that
+ * use distinct workspaces may run concurrently on distinct streams.
+ */
+#ifndef PARGEMSLR_AMD_H
+#define PARGEMSLR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (reference values: ParGeMSLR/INC/pargemslr.hpp:9-21) */
+#define PGS_SUCCESS 0
+#define PGS_ERROR_INVALID_OPTION 100
+#define PGS_ERROR_INVALID_PARAM 101
+#define PGS_ERROR_IO 102
+#define PGS_ERROR_ILU_EMPTY_ROW 103
+#define PGS_ERROR_FUNCTION_CALL 106
+#define PGS_ERROR_MEMORY_LOCATION 107
+#define PGS_ERR_HIP 200
+#define PGS_ERR_OOM 201
+#define PGS_ERR_NO_DEVICE 202
+#define PGS_ERR_BREAKDOWN 203
+#define PGS_ERR_NOT_CONVERGED 204
+#define PGS_ERR_UNSUPPORTED 205
+
+/* parameter array layout: the reference's PARGEMSLR_IO_* indices
+ * (ParGeMSLR/SRC/utils/utils.hpp:77-297); only the entries listed here are read */
+#define PGS_IO_SIZE 180
+#define PGS_IO_SOLVER_KDIM 11
+#define PGS_IO_SOLVER_MAXITS 12
+#define PGS_IO_SOLVER_TOL 13
+#define PGS_IO_SOLVER_ATOL 14
+#define PGS_IO_PREPOSS_NLEV_GLOBAL 20
+#define PGS_IO_PREPOSS_NCOMP_GLOBAL 22
+#define PGS_IO_PREPOSS_PARTITION_GLOBAL 24
+#define PGS_IO_PREPOSS_MINSEP 26
+#define PGS_IO_PREPOSS_KMIN_GLOBAL 27
+#define PGS_IO_PREPOSS_KFACTOR_GLOBAL 29
+#define PGS_IO_PREPOSS_VTXSEP_GLOBAL 31
+#define PGS_IO_ILU_DROPTOL_B_GLOBAL 40
+#define PGS_IO_ILU_DROPTOL_C_GLOBAL 44
+#define PGS_IO_ILU_ROWNNZ_B_GLOBAL 48
+#define PGS_IO_ILU_ROWNNZ_C_GLOBAL 50
+#define PGS_IO_ILU_PERM_OPTION_GLOBAL 58
+#define PGS_IO_LR_ARNOLDI_OPTION1_GLOBAL 70
+#define PGS_IO_LR_ARNOLDI_OPTION2_GLOBAL 71
+#define PGS_IO_LR_RANK1_GLOBAL 74
+#define PGS_IO_LR_RANK2_GLOBAL 75
+#define PGS_IO_LR_RANK_FACTOR1_GLOBAL 78
+#define PGS_IO_LR_RANK_FACTOR2_GLOBAL 79
+#define PGS_IO_LR_ARNOLDI_FACTOR1_GLOBAL 82
+#define PGS_IO_LR_ARNOLDI_FACTOR2_GLOBAL 83
+#define PGS_IO_LR_TOL_EIG1_GLOBAL 86
+#define PGS_IO_LR_TOL_EIG2_GLOBAL 87
+#define PGS_IO_LR_MAXITS1_GLOBAL 90
+#define PGS_IO_LR_MAXITS2_GLOBAL 91
+#define PGS_IO_LR_TR_FACTOR 94
+#define PGS_IO_LR_TOL_ORTH 95
+#define PGS_IO_LR_RAND_INIT_GUESS 97
+#define PGS_IO_SCHUR_ENABLE 130
+#define PGS_IO_SCHUR_ITER_TOL 131
+#define PGS_IO_SCHUR_MAXITS 132
+
+/* precision of the preconditioner apply */
+#define PGS_PREC_FP64 1 /* apply in the operator's own precision (double / complex128) */
+#define PGS_PREC_FP32 2 /* apply in float / complex64 with float accumulation */
+
+/* Krylov orthogonalization */
+#define PGS_ORTH_MGS 0  /* modified Gram-Schmidt, as FlexGmresClass (fgmres.hpp:424-426) */
+#define PGS_ORTH_CGS2 1 /* classical Gram-Schmidt with one re-orthogonalization pass */
+
+typedef struct pgs_gemslr_s* pgs_gemslr_t; /* host-resident factored operator (setup output) */
+typedef struct pgs_dev_op_s* pgs_dev_op_t; /* operator resident in device memory */
+typedef struct pgs_ws_s* pgs_ws_t;         /* per-stream solve workspace */
+
+/* ---------------- host setup (GemslrClass::Setup) ---------------- */
+
+/* Fills params[PGS_IO_SIZE] with the reference defaults
+ * (PargemslrSetDefaultParameterArray, ParGeMSLR/SRC/utils/utils.cpp:31-127). */
+int pgs_params_default(double* params);
+
+/* GemslrClass::SetMatrix + SetWithParameterArray + Setup
+ * (ParGeMSLR/SRC/preconditioners/gemslr.hpp:2069-2161, gemslr.cpp:1602-1711).
+ * CSR (0-based, int32), `a` holds n values per row entry: double (is_complex=0)
+ * or (re, im) pairs (is_complex=1).  seed selects the deterministic random
+ * streams of the setup (partition tie-breaking, Arnoldi start vector). */
+int pgs_gemslr_setup(int n, const int* ia, const int* ja, const double* a, int is_complex, const double* params,
+                     uint64_t seed, pgs_gemslr_t* out);
+int pgs_gemslr_destroy(pgs_gemslr_t op);
+
+typedef struct {
+  int n;
+  int nlev;
+  int is_complex;
+  int64_t nnz_a;
+  int64_t nnz_ilu;    /* sum over blocks of n_b + nnz(L_b) + nnz(U_b) */
+  int64_t nnz_lr;     /* sum over levels of 2 * m_l * k_l */
+  double setup_seconds[4]; /* partition+ordering, ILUT, low-rank, total */
+} pgs_gemslr_info_t;
+int pgs_gemslr_info(pgs_gemslr_t op, pgs_gemslr_info_t* info);
+
+/* One level of the factored operator, level-local numbering.  The factors of
+ * all ncomp diagonal blocks are concatenated into one block-diagonal CSR
+ * (L strict lower, U strict upper, Dinv = inverse pivots).  E is nrem x nloc,
+ * F is nloc x nrem, W and WH are nrem x rank column-major.  Pointers stay
+ * valid until pgs_gemslr_destroy. */
+typedef struct {
+  int lev_start, nloc, nrem, ncomp, rank;
+  const int* dom_ptr; /* ncomp+1, level-local */
+  const int *L_ia, *L_ja, *U_ia, *U_ja;
+  const double *L_a, *U_a, *Dinv;
+  const int *E_ia, *E_ja, *F_ia, *F_ja;
+  const double *E_a, *F_a;
+  const double *W, *WH;
+} pgs_level_view_t;
+int pgs_gemslr_level_view(pgs_gemslr_t op, int level, pgs_level_view_t* view);
+/* perm[i]: (P b)[i] = b[perm[i]] */
+int pgs_gemslr_perm(pgs_gemslr_t op, const int** perm);
+
+/* Host-resident apply z = M^{-1} v in FP64 (GemslrClass::Solve on kMemoryHost,
+ * gemslr.cpp:3431-3467). */
+int pgs_gemslr_host_apply(pgs_gemslr_t op, const double* v, double* z);
+
+/* ---------------- device ---------------- */
+
+int pgs_device_count(int* count);
+
+/* Device memory and streams of the library's own HIP runtime, for callers
+ * that do not hold one (tests, benchmarks, the request server).  No
+ * counterpart in the reference, whose memory layer is
+ * ParGeMSLR/SRC/utils/memory.cpp:93-503. */
+int pgs_dev_malloc(int device, int64_t bytes, void** ptr);
+int pgs_dev_free(int device, void* ptr);
+int pgs_memcpy_htod(int device, void* dst, const void* src, int64_t bytes, void* stream);
+int pgs_memcpy_dtoh(int device, void* dst, const void* src, int64_t bytes, void* stream);
+int pgs_memset_zero(int device, void* ptr, int64_t bytes, void* stream);
+int pgs_stream_create(int device, void** stream);
+int pgs_stream_destroy(int device, void* stream);
+int pgs_stream_synchronize(int device, void* stream);
+int pgs_device_synchronize(int device);
+
+/* Uploads the operator once (A, permutation, level factors in level-set
+ * order, E, F, W, WH) for the precisions in prec_mask (PGS_PREC_* bits). */
+int pgs_dev_op_create(pgs_gemslr_t op, int device, int prec_mask, pgs_dev_op_t* out);
+int pgs_dev_op_destroy(pgs_dev_op_t dop);
+/* device bytes held by the operator */
+int pgs_dev_op_bytes(pgs_dev_op_t dop, int64_t* bytes);
+
+/* Workspace for up to max_nrhs right-hand sides and Krylov dimension kdim. */
+int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out);
+int pgs_ws_destroy(pgs_ws_t ws);
+
+/* y = alpha * A * x + beta * y  (CsrMatrixClass::MatVec('N', ...),
+ * ParGeMSLR/SRC/matrices/csr_matrix.cpp:1340-1365), FP64 / complex128. */
+int pgs_spmv(pgs_dev_op_t dop, pgs_ws_t ws, const double* alpha, const double* x, const double* beta, double* y,
+             int nrhs, void* stream);
+
+/* z = M^{-1} v on the device (GemslrClass::Solve, gemslr.cpp:3431-3467).
+ * v and z are in the operator's precision (double / complex128); `prec`
+ * selects the arithmetic of the apply (PGS_PREC_FP64 or PGS_PREC_FP32). */
+int pgs_apply(pgs_dev_op_t dop, pgs_ws_t ws, const double* v, double* z, int nrhs, int prec, void* stream);
+
+typedef struct {
+  int kdim;         /* restart length m */
+  int maxits;       /* total iteration cap */
+  double tol;       /* stop when ||r|| <= tol * ||b|| (or <= tol if absolute) */
+  int absolute_tol;
+  int prec;         /* PGS_PREC_* of the preconditioner apply */
+  int orth;         /* PGS_ORTH_* */
+  int use_precond;  /* 0: no preconditioner (z = v) */
+  int check_every;  /* host convergence poll interval in iterations (0 = 1) */
+} pgs_fgmres_opts_t;
+
+typedef struct {
+  int status;       /* 0 converged, PGS_ERR_NOT_CONVERGED, PGS_ERR_BREAKDOWN */
+  int iters;
+  double rel_res;   /* Givens estimate ||r||/||b|| at exit (FlexGmresClass::_rel_res) */
+} pgs_fgmres_result_t;
+
+/* Flexible GMRES(m), right preconditioned (FlexGmresClass::Solve,
+ * ParGeMSLR/SRC/solvers/fgmres.hpp:263-558).  x holds the initial guess on
+ * entry.  For nrhs > 1 every column runs its own FGMRES (own Hessenberg,
+ * rotations, stopping test); kernels process all columns per launch.
+ * res must hold nrhs entries. */
+int pgs_fgmres(pgs_dev_op_t dop, pgs_ws_t ws, const double* b, double* x, int nrhs, const pgs_fgmres_opts_t* opts,
+               pgs_fgmres_result_t* res, void* stream);
+
+/* Kernel timing for the roofline report (no counterpart in the reference).
+ * When enabled, the workspace records a HIP event pair around every launch of
+ * the level-0 block triangular solve it issues (on the launch stream) and
+ * accumulates the algorithmic bytes of each launch:
+ *   (nnz(L_0) + nnz(U_0)) * (s + 4) + 2 * (n_0 + nblocks) * 4 + n_0 * s + 2 * n_0 * nrhs * s
+ * (s = value bytes).  Reading synchronizes the recorded events. */
+typedef struct {
+  int64_t launches;
+  double total_ms;
+  double total_bytes;
+} pgs_kernel_profile_t;
+int pgs_ws_profile(pgs_ws_t ws, int enable);
+int pgs_ws_profile_read(pgs_ws_t ws, pgs_kernel_profile_t* out);
+
+const char* pgs_strerror(int code);
+/* message of the last HIP error seen by this thread */
+const char* pgs_last_hip_error(void);
+
+/* ---------------- problem generators (test and benchmark inputs) ---------------- */
+
+/* CooMatrixClass::Laplacian (ParGeMSLR/SRC/matrices/coo_matrix.cpp:519-640):
+ * 5-point (one of nx, ny, nz == 1) or 7-point stencil, Dirichlet boundary,
+ * diagonal (2D ? 4 : 6) - shift, +direction neighbour -1 - alpha, -direction
+ * neighbour -1 + alpha.  is_complex selects complex values (shift_im used).
+ * Arrays are malloc'd; release with pgs_free. */
+int pgs_gen_laplacian(int nx, int ny, int nz, double shift_re, double shift_im, double ax, double ay, double az,
+                      int is_complex, int* n, int64_t* nnz, int** ia, int** ja, double** a);
+/* 3D first-order upwind convection-diffusion: diagonal 6 + 3c, upstream
+ * (-x, -y, -z) neighbours -1 - c, downstream neighbours -1. */
+int pgs_gen_convdiff(int nx, int ny, int nz, double c, int* n, int64_t* nnz, int** ia, int** ja, double** a);
+void pgs_free(void* p);
+/* counter-based splitmix64 streams: uniform [lo, hi) and standard normal */
+int pgs_rng_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);
+int pgs_rng_normal(uint64_t seed, int64_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARGEMSLR_AMD_H */

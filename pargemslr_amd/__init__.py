@@ -1,0 +1,12 @@
+"""pargemslr_amd: MI355X GeMSLR-preconditioned FGMRES solve path.
+
+The compute path is the in-tree shared library libpargemslr_amd.so (host
+setup in C++, preconditioner apply and FGMRES in HIP kernels for gfx950)
+behind the C ABI of include/pargemslr_amd.h; this package binds it with
+ctypes.  Importing does not touch the GPU.
+"""
+from .capi import (  # noqa: F401
+    PGS_ORTH_CGS2, PGS_ORTH_MGS, PGS_PREC_FP32, PGS_PREC_FP64, CsrHost, DeviceArray, DevOp, Gemslr, PgsError,
+    Stream, Workspace, config_params, convdiff, default_params, device_count, laplacian, lib, rng_normal,
+    rng_uniform, synchronize,
+)

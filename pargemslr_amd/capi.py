@@ -1,0 +1,505 @@
+"""ctypes bindings of the pargemslr_amd C ABI (include/pargemslr_amd.h).
+
+Host arrays are numpy; device arrays are torch tensors on the operator's
+device (torch only provides allocation and streams here).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpargemslr_amd.so")
+
+PGS_IO_SIZE = 180
+PGS_PREC_FP64 = 1
+PGS_PREC_FP32 = 2
+PGS_ORTH_MGS = 0
+PGS_ORTH_CGS2 = 1
+PGS_ERR_BREAKDOWN = 203
+PGS_ERR_NOT_CONVERGED = 204
+
+# parameter indices (reference PARGEMSLR_IO_* numbering)
+IO = dict(
+    SOLVER_KDIM=11, SOLVER_MAXITS=12, SOLVER_TOL=13, SOLVER_ATOL=14,
+    NLEV=20, NCOMP=22, PARTITION=24, MINSEP=26, KMIN=27, KFACTOR=29, VTXSEP=31,
+    DROPTOL_B=40, DROPTOL_C=44, ROWNNZ_B=48, ROWNNZ_C=50, PERM_OPTION=58,
+    LR_OPTION1=70, LR_OPTION2=71, LR_RANK1=74, LR_RANK2=75, LR_RANK_FACTOR1=78, LR_RANK_FACTOR2=79,
+    LR_ARNOLDI_FACTOR1=82, LR_ARNOLDI_FACTOR2=83, LR_TOL_EIG1=86, LR_TOL_EIG2=87,
+    LR_MAXITS1=90, LR_MAXITS2=91, LR_TR_FACTOR=94, LR_TOL_ORTH=95, LR_RAND_INIT=97,
+    SCHUR_ENABLE=130, SCHUR_ITER_TOL=131, SCHUR_MAXITS=132,
+)
+
+
+class PgsError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__(f"{what}: {strerror(code)} ({code})")
+
+
+class GemslrInfo(C.Structure):
+    _fields_ = [("n", C.c_int), ("nlev", C.c_int), ("is_complex", C.c_int), ("nnz_a", C.c_int64),
+                ("nnz_ilu", C.c_int64), ("nnz_lr", C.c_int64), ("setup_seconds", C.c_double * 4)]
+
+
+class LevelView(C.Structure):
+    _fields_ = [("lev_start", C.c_int), ("nloc", C.c_int), ("nrem", C.c_int), ("ncomp", C.c_int), ("rank", C.c_int),
+                ("dom_ptr", C.POINTER(C.c_int)),
+                ("L_ia", C.POINTER(C.c_int)), ("L_ja", C.POINTER(C.c_int)),
+                ("U_ia", C.POINTER(C.c_int)), ("U_ja", C.POINTER(C.c_int)),
+                ("L_a", C.POINTER(C.c_double)), ("U_a", C.POINTER(C.c_double)), ("Dinv", C.POINTER(C.c_double)),
+                ("E_ia", C.POINTER(C.c_int)), ("E_ja", C.POINTER(C.c_int)),
+                ("F_ia", C.POINTER(C.c_int)), ("F_ja", C.POINTER(C.c_int)),
+                ("E_a", C.POINTER(C.c_double)), ("F_a", C.POINTER(C.c_double)),
+                ("W", C.POINTER(C.c_double)), ("WH", C.POINTER(C.c_double))]
+
+
+class FgmresOpts(C.Structure):
+    _fields_ = [("kdim", C.c_int), ("maxits", C.c_int), ("tol", C.c_double), ("absolute_tol", C.c_int),
+                ("prec", C.c_int), ("orth", C.c_int), ("use_precond", C.c_int), ("check_every", C.c_int)]
+
+
+class FgmresResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_int), ("rel_res", C.c_double)]
+
+
+class KernelProfile(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("total_bytes", C.c_double)]
+
+
+# every extern "C" entry point of include/pargemslr_amd.h
+EXPORTED = [
+    "pgs_params_default", "pgs_gemslr_setup", "pgs_gemslr_destroy", "pgs_gemslr_info", "pgs_gemslr_level_view",
+    "pgs_gemslr_perm", "pgs_gemslr_host_apply", "pgs_device_count", "pgs_dev_op_create", "pgs_dev_op_destroy",
+    "pgs_dev_op_bytes", "pgs_ws_create", "pgs_ws_destroy", "pgs_spmv", "pgs_apply", "pgs_fgmres", "pgs_strerror",
+    "pgs_last_hip_error", "pgs_gen_laplacian", "pgs_gen_convdiff", "pgs_free", "pgs_rng_uniform", "pgs_rng_normal",
+    "pgs_ws_profile", "pgs_ws_profile_read", "pgs_dev_malloc", "pgs_dev_free", "pgs_memcpy_htod", "pgs_memcpy_dtoh",
+    "pgs_memset_zero", "pgs_stream_create", "pgs_stream_destroy", "pgs_stream_synchronize", "pgs_device_synchronize",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads the in-tree shared library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pargemslr_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    L.pgs_params_default.argtypes = [dp]
+    L.pgs_gemslr_setup.argtypes = [C.c_int, ip, ip, dp, C.c_int, dp, C.c_uint64, C.POINTER(vp)]
+    L.pgs_gemslr_destroy.argtypes = [vp]
+    L.pgs_gemslr_info.argtypes = [vp, C.POINTER(GemslrInfo)]
+    L.pgs_gemslr_level_view.argtypes = [vp, C.c_int, C.POINTER(LevelView)]
+    L.pgs_gemslr_perm.argtypes = [vp, C.POINTER(ip)]
+    L.pgs_gemslr_host_apply.argtypes = [vp, dp, dp]
+    L.pgs_device_count.argtypes = [ip]
+    L.pgs_dev_op_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
+    L.pgs_dev_op_destroy.argtypes = [vp]
+    L.pgs_dev_op_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.pgs_ws_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
+    L.pgs_ws_destroy.argtypes = [vp]
+    L.pgs_spmv.argtypes = [vp, vp, dp, vp, dp, vp, C.c_int, vp]
+    L.pgs_apply.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, vp]
+    L.pgs_fgmres.argtypes = [vp, vp, vp, vp, C.c_int, C.POINTER(FgmresOpts), C.POINTER(FgmresResult), vp]
+    L.pgs_strerror.argtypes = [C.c_int]
+    L.pgs_strerror.restype = C.c_char_p
+    L.pgs_last_hip_error.restype = C.c_char_p
+    L.pgs_gen_laplacian.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_int, ip, C.POINTER(C.c_int64), C.POINTER(ip), C.POINTER(ip),
+                                    C.POINTER(dp)]
+    L.pgs_gen_convdiff.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, ip, C.POINTER(C.c_int64), C.POINTER(ip),
+                                   C.POINTER(ip), C.POINTER(dp)]
+    L.pgs_free.argtypes = [vp]
+    L.pgs_rng_uniform.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, dp]
+    L.pgs_rng_normal.argtypes = [C.c_uint64, C.c_int64, dp]
+    L.pgs_ws_profile.argtypes = [vp, C.c_int]
+    L.pgs_ws_profile_read.argtypes = [vp, C.POINTER(KernelProfile)]
+    L.pgs_dev_malloc.argtypes = [C.c_int, C.c_int64, C.POINTER(vp)]
+    L.pgs_dev_free.argtypes = [C.c_int, vp]
+    L.pgs_memcpy_htod.argtypes = [C.c_int, vp, vp, C.c_int64, vp]
+    L.pgs_memcpy_dtoh.argtypes = [C.c_int, vp, vp, C.c_int64, vp]
+    L.pgs_memset_zero.argtypes = [C.c_int, vp, C.c_int64, vp]
+    L.pgs_stream_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.pgs_stream_destroy.argtypes = [C.c_int, vp]
+    L.pgs_stream_synchronize.argtypes = [C.c_int, vp]
+    L.pgs_device_synchronize.argtypes = [C.c_int]
+    _lib = L
+    return L
+
+
+def strerror(code):
+    try:
+        return lib().pgs_strerror(code).decode()
+    except Exception:
+        return "unknown"
+
+
+def _chk(code, what):
+    if code != 0:
+        raise PgsError(code, what)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def default_params():
+    p = np.zeros(PGS_IO_SIZE, dtype=np.float64)
+    _chk(lib().pgs_params_default(_dptr(p)), "pgs_params_default")
+    return p
+
+
+# ---------------------------------------------------------------- generators
+class CsrHost:
+    """0-based CSR on the host (numpy)."""
+
+    def __init__(self, n, ia, ja, a):
+        self.n = int(n)
+        self.ia = np.ascontiguousarray(ia, dtype=np.int32)
+        self.ja = np.ascontiguousarray(ja, dtype=np.int32)
+        self.a = np.ascontiguousarray(a)
+        self.is_complex = np.iscomplexobj(self.a)
+
+    @property
+    def nnz(self):
+        return int(self.ia[-1])
+
+    def matvec(self, x):
+        """y = A x (numpy, for residual checks)."""
+        rows = np.repeat(np.arange(self.n), np.diff(self.ia))
+        y = np.zeros(x.shape, dtype=np.result_type(self.a, x))
+        prod = self.a[:, None] * x[self.ja] if x.ndim == 2 else self.a * x[self.ja]
+        np.add.at(y, rows, prod)
+        return y
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.a, self.ja, self.ia), shape=(self.n, self.n))
+
+
+def _take_csr(n, nnz, ia, ja, a, is_complex):
+    L = lib()
+    n = n.value
+    nnz = nnz.value
+    ia_np = np.ctypeslib.as_array(ia, shape=(n + 1,)).copy()
+    ja_np = np.ctypeslib.as_array(ja, shape=(max(nnz, 1),)).copy()[:nnz]
+    if is_complex:
+        a_np = np.ctypeslib.as_array(a, shape=(max(2 * nnz, 2),)).copy()[: 2 * nnz].view(np.complex128)
+    else:
+        a_np = np.ctypeslib.as_array(a, shape=(max(nnz, 1),)).copy()[:nnz]
+    for p in (ia, ja, a):
+        L.pgs_free(C.cast(p, C.c_void_p))
+    return CsrHost(n, ia_np, ja_np, a_np)
+
+
+def laplacian(nx, ny, nz, shift=0.0, alphas=(0.0, 0.0, 0.0), complex_=False):
+    n = C.c_int()
+    nnz = C.c_int64()
+    ia = C.POINTER(C.c_int)()
+    ja = C.POINTER(C.c_int)()
+    a = C.POINTER(C.c_double)()
+    sh = complex(shift)
+    _chk(lib().pgs_gen_laplacian(nx, ny, nz, sh.real, sh.imag, alphas[0], alphas[1], alphas[2], int(complex_),
+                                 C.byref(n), C.byref(nnz), C.byref(ia), C.byref(ja), C.byref(a)), "pgs_gen_laplacian")
+    return _take_csr(n, nnz, ia, ja, a, complex_)
+
+
+def convdiff(nx, ny, nz, c):
+    n = C.c_int()
+    nnz = C.c_int64()
+    ia = C.POINTER(C.c_int)()
+    ja = C.POINTER(C.c_int)()
+    a = C.POINTER(C.c_double)()
+    _chk(lib().pgs_gen_convdiff(nx, ny, nz, c, C.byref(n), C.byref(nnz), C.byref(ia), C.byref(ja), C.byref(a)),
+         "pgs_gen_convdiff")
+    return _take_csr(n, nnz, ia, ja, a, False)
+
+
+def rng_uniform(seed, n, lo=-1.0, hi=1.0):
+    out = np.empty(n, dtype=np.float64)
+    _chk(lib().pgs_rng_uniform(seed, n, lo, hi, _dptr(out)), "pgs_rng_uniform")
+    return out
+
+
+def rng_normal(seed, n):
+    out = np.empty(n, dtype=np.float64)
+    _chk(lib().pgs_rng_normal(seed, n, _dptr(out)), "pgs_rng_normal")
+    return out
+
+
+# ---------------------------------------------------------------- host setup
+class Gemslr:
+    """Host-resident GeMSLR operator (GemslrClass after Setup)."""
+
+    def __init__(self, A, params, seed=0):
+        self.A = A
+        self._h = C.c_void_p()
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        a = A.a.view(np.float64) if A.is_complex else A.a
+        a = np.ascontiguousarray(a)
+        _chk(lib().pgs_gemslr_setup(A.n, _iptr(A.ia), _iptr(A.ja), _dptr(a), int(A.is_complex), _dptr(params),
+                                    C.c_uint64(seed), C.byref(self._h)), "pgs_gemslr_setup")
+
+    def close(self):
+        if self._h:
+            lib().pgs_gemslr_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        inf = GemslrInfo()
+        _chk(lib().pgs_gemslr_info(self._h, C.byref(inf)), "pgs_gemslr_info")
+        return inf
+
+    def level_view(self, l):
+        v = LevelView()
+        _chk(lib().pgs_gemslr_level_view(self._h, l, C.byref(v)), "pgs_gemslr_level_view")
+        return v
+
+    def perm(self):
+        p = C.POINTER(C.c_int)()
+        _chk(lib().pgs_gemslr_perm(self._h, C.byref(p)), "pgs_gemslr_perm")
+        return np.ctypeslib.as_array(p, shape=(self.A.n,)).copy()
+
+    def levels(self):
+        """Level structure as numpy arrays (copies)."""
+        out = []
+        nlev = self.info().nlev
+        for l in range(nlev):
+            v = self.level_view(l)
+            d = dict(lev_start=v.lev_start, nloc=v.nloc, nrem=v.nrem, ncomp=v.ncomp, rank=v.rank)
+            d["dom_ptr"] = np.ctypeslib.as_array(v.dom_ptr, shape=(v.ncomp + 1,)).copy()
+            for nm, nr in (("L", v.nloc), ("U", v.nloc), ("E", v.nrem), ("F", v.nloc)):
+                ia = getattr(v, nm + "_ia")
+                if nm in ("E", "F") and l == nlev - 1:
+                    d[nm] = None
+                    continue
+                ia_np = np.ctypeslib.as_array(ia, shape=(nr + 1,)).copy()
+                nnz = int(ia_np[-1])
+                ja_np = np.ctypeslib.as_array(getattr(v, nm + "_ja"), shape=(max(nnz, 1),)).copy()[:nnz] if nnz else np.zeros(0, np.int32)
+                a_np = np.ctypeslib.as_array(getattr(v, nm + "_a"), shape=(max(nnz, 1),)).copy()[:nnz] if nnz else np.zeros(0)
+                d[nm] = (ia_np, ja_np, a_np)
+            d["Dinv"] = np.ctypeslib.as_array(v.Dinv, shape=(v.nloc,)).copy()
+            if v.rank > 0:
+                d["W"] = np.ctypeslib.as_array(v.W, shape=(v.rank, v.nrem)).T.copy()
+                d["WH"] = np.ctypeslib.as_array(v.WH, shape=(v.rank, v.nrem)).T.copy()
+            out.append(d)
+        return out
+
+    def host_apply(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        z = np.empty_like(v)
+        _chk(lib().pgs_gemslr_host_apply(self._h, _dptr(v), _dptr(z)), "pgs_gemslr_host_apply")
+        return z
+
+
+# ---------------------------------------------------------------- device
+def device_count():
+    c = C.c_int()
+    rc = lib().pgs_device_count(C.byref(c))
+    return c.value if rc == 0 else 0
+
+
+def synchronize(device=0):
+    _chk(lib().pgs_device_synchronize(device), "pgs_device_synchronize")
+
+
+class DeviceArray:
+    """A dense float64/float32 array in device memory (row-major, n x nrhs
+    blocks are row-interleaved as the C ABI expects)."""
+
+    def __init__(self, shape, dtype=np.float64, device=0, zero=True):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.device = device
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _chk(lib().pgs_dev_malloc(device, self.nbytes, C.byref(p)), "pgs_dev_malloc")
+        self._p = p
+        if zero:
+            self.zero()
+
+    @classmethod
+    def from_numpy(cls, a, device=0):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype, device, zero=False)
+        d.upload(a)
+        return d
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        _chk(lib().pgs_memcpy_htod(self.device, self._p, C.c_void_p(a.ctypes.data), self.nbytes, stream),
+             "pgs_memcpy_htod")
+
+    def numpy(self, stream=None):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _chk(lib().pgs_memcpy_dtoh(self.device, C.c_void_p(out.ctypes.data), self._p, self.nbytes, stream),
+             "pgs_memcpy_dtoh")
+        return out
+
+    def zero(self, stream=None):
+        _chk(lib().pgs_memset_zero(self.device, self._p, self.nbytes, stream), "pgs_memset_zero")
+
+    def close(self):
+        if self._p:
+            lib().pgs_dev_free(self.device, self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self, device=0):
+        self.device = device
+        s = C.c_void_p()
+        _chk(lib().pgs_stream_create(device, C.byref(s)), "pgs_stream_create")
+        self.handle = s
+
+    def synchronize(self):
+        _chk(lib().pgs_stream_synchronize(self.device, self.handle), "pgs_stream_synchronize")
+
+    def close(self):
+        if self.handle:
+            lib().pgs_stream_destroy(self.device, self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _tptr(t):
+    """Device pointer of a DeviceArray (or of any object exposing data_ptr())."""
+    if isinstance(t, DeviceArray):
+        return t.ptr
+    return C.c_void_p(t.data_ptr())
+
+
+def _sh(stream):
+    if stream is None:
+        return None
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+class DevOp:
+    """Operator resident in device memory."""
+
+    def __init__(self, gem, device=0, prec_mask=PGS_PREC_FP64 | PGS_PREC_FP32):
+        self.gem = gem
+        self.n = gem.A.n
+        self.device = device
+        self._h = C.c_void_p()
+        _chk(lib().pgs_dev_op_create(gem.handle, device, prec_mask, C.byref(self._h)), "pgs_dev_op_create")
+
+    def bytes(self):
+        b = C.c_int64()
+        _chk(lib().pgs_dev_op_bytes(self._h, C.byref(b)), "pgs_dev_op_bytes")
+        return b.value
+
+    def close(self):
+        if self._h:
+            lib().pgs_dev_op_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Workspace:
+    def __init__(self, dop, max_nrhs=1, kdim=50):
+        self.dop = dop
+        self._h = C.c_void_p()
+        _chk(lib().pgs_ws_create(dop._h, max_nrhs, kdim, C.byref(self._h)), "pgs_ws_create")
+
+    def close(self):
+        if self._h:
+            lib().pgs_ws_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def profile(self, enable=True):
+        _chk(lib().pgs_ws_profile(self._h, int(enable)), "pgs_ws_profile")
+
+    def profile_read(self):
+        p = KernelProfile()
+        _chk(lib().pgs_ws_profile_read(self._h, C.byref(p)), "pgs_ws_profile_read")
+        return dict(launches=p.launches, total_ms=p.total_ms, total_bytes=p.total_bytes)
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0, nrhs=1, stream=None):
+        a = (C.c_double * 2)(alpha, 0.0)
+        b = (C.c_double * 2)(beta, 0.0)
+        _chk(lib().pgs_spmv(self.dop._h, self._h, a, _tptr(x), b, _tptr(y), nrhs, _sh(stream)), "pgs_spmv")
+
+    def apply(self, v, z, nrhs=1, prec=PGS_PREC_FP64, stream=None):
+        _chk(lib().pgs_apply(self.dop._h, self._h, _tptr(v), _tptr(z), nrhs, prec, _sh(stream)), "pgs_apply")
+
+    def fgmres(self, b, x, nrhs=1, kdim=50, maxits=1000, tol=1e-8, prec=PGS_PREC_FP64, orth=PGS_ORTH_CGS2,
+               use_precond=True, absolute=False, check_every=1, stream=None):
+        o = FgmresOpts(kdim, maxits, tol, int(absolute), prec, orth, int(use_precond), check_every)
+        res = (FgmresResult * nrhs)()
+        _chk(lib().pgs_fgmres(self.dop._h, self._h, _tptr(b), _tptr(x), nrhs, C.byref(o), res, _sh(stream)),
+             "pgs_fgmres")
+        return [(r.status, r.iters, r.rel_res) for r in res]
+
+
+def config_params(nlev, ncomp, droptol, rownnz, rank, kdim=50, maxits=1000, tol=1e-8, lr_tol_eig=1e-3,
+                  lr_maxits=10, droptol_c=None, rownnz_c=None, perm=1):
+    """Reference parameter array for a GeMSLR configuration (inner Schur iteration off)."""
+    p = default_params()
+    p[IO["NLEV"]] = nlev
+    p[IO["NCOMP"]] = ncomp
+    p[IO["PARTITION"]] = 1
+    p[IO["DROPTOL_B"]] = droptol
+    p[IO["DROPTOL_C"]] = droptol if droptol_c is None else droptol_c
+    p[IO["ROWNNZ_B"]] = rownnz
+    p[IO["ROWNNZ_C"]] = rownnz if rownnz_c is None else rownnz_c
+    p[IO["PERM_OPTION"]] = perm
+    p[IO["LR_RANK1"]] = rank
+    p[IO["LR_RANK2"]] = rank
+    p[IO["LR_TOL_EIG1"]] = lr_tol_eig
+    p[IO["LR_TOL_EIG2"]] = lr_tol_eig
+    p[IO["LR_MAXITS1"]] = lr_maxits
+    p[IO["LR_MAXITS2"]] = lr_maxits
+    p[IO["SCHUR_ENABLE"]] = 0
+    p[IO["SOLVER_KDIM"]] = kdim
+    p[IO["SOLVER_MAXITS"]] = maxits
+    p[IO["SOLVER_TOL"]] = tol
+    return p

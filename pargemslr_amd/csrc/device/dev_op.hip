@@ -1,0 +1,782 @@
+// Device-resident GeMSLR operator, workspaces, the preconditioner apply and
+// the FGMRES driver, exported through the C ABI of pargemslr_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/pargemslr_amd.h"
+#include "../host/gemslr_host.hpp"
+#include "../host/trsv_pack.hpp"
+#include "capi_internal.hpp"
+#include "kernels.hpp"
+
+using namespace pgs;
+using namespace pgs::dev;
+
+namespace {
+
+struct HipError {
+  hipError_t e;
+};
+
+#define HIPCHK(call)                                   \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) throw HipError{_e};          \
+  } while (0)
+
+thread_local std::string g_last_hip_error;
+
+inline unsigned nblocks_for(int64_t work, int nt) { return (unsigned)((work + nt - 1) / nt); }
+
+struct DevMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    bytes = b;
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) { p = nullptr; throw HipError{e}; }
+  }
+  template <typename T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+template <typename T>
+void upload(DevMem& m, const std::vector<T>& v, int64_t* counter) {
+  m.alloc(v.size() * sizeof(T));
+  if (!v.empty()) HIPCHK(hipMemcpy(m.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (counter) *counter += (int64_t)(v.size() * sizeof(T));
+}
+
+template <typename To, typename From>
+std::vector<To> convert_vec(const std::vector<From>& v) {
+  std::vector<To> o(v.size());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = (To)v[i];
+  return o;
+}
+
+// W (column-major m x k) -> row-major
+template <typename To>
+std::vector<To> to_row_major(const std::vector<double>& W, int m, int k) {
+  std::vector<To> o((size_t)m * k);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < m; ++i) o[(size_t)i * k + j] = (To)W[(size_t)j * m + i];
+  return o;
+}
+
+struct DevCsr {
+  int nrows = 0, ncols = 0;
+  int64_t nnz = 0;
+  int lpr = 1;
+  DevMem ia, ja, a;
+};
+
+template <typename V>
+void upload_csr(DevCsr& d, const Csr<double>& A, int64_t* counter) {
+  d.nrows = A.nrows;
+  d.ncols = A.ncols;
+  d.nnz = A.nnz();
+  upload(d.ia, A.ia, counter);
+  upload(d.ja, A.ja, counter);
+  upload(d.a, convert_vec<V>(A.a), counter);
+  double avg = A.nrows > 0 ? (double)A.nnz() / A.nrows : 0.0;
+  d.lpr = avg <= 2 ? 2 : avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
+}
+
+struct DevTrsv {
+  int nblocks = 0, nloc = 0, max_rows = 0, max_sets = 0;
+  int64_t nnz_padded = 0, nnz_real = 0, nrows_total = 0;
+  DevMem row0, setL, setU, desc, rows, col, val, dinv;
+};
+
+template <typename V>
+void upload_trsv(DevTrsv& d, const TrsvPack<double>& P, int64_t* counter) {
+  d.nblocks = P.nblocks;
+  d.nloc = P.nloc;
+  d.max_rows = P.max_block_rows;
+  d.max_sets = P.max_sets;
+  d.nnz_padded = (int64_t)P.col.size();
+  d.nnz_real = P.nnz_real;
+  d.nrows_total = (int64_t)P.rows.size();
+  upload(d.row0, P.blk_row0, counter);
+  upload(d.setL, P.blk_setL, counter);
+  upload(d.setU, P.blk_setU, counter);
+  upload(d.desc, P.desc, counter);
+  upload(d.rows, P.rows, counter);
+  upload(d.col, P.col, counter);
+  upload(d.val, convert_vec<V>(P.val), counter);
+  upload(d.dinv, convert_vec<V>(P.dinv), counter);
+}
+
+struct DevLevel {
+  int lev_start = 0, nloc = 0, nrem = 0, rank = 0;
+  DevTrsv B;
+  DevCsr E, F;
+  DevMem W, WH;  // row-major nrem x rank
+};
+
+template <typename V>
+struct DevPrec {
+  bool present = false;
+  std::vector<std::unique_ptr<DevLevel>> levels;
+};
+
+}  // namespace
+
+struct pgs_dev_op_s {
+  int device = 0;
+  int n = 0;
+  int nlev = 0;
+  int is_complex = 0;
+  int lds_max = 65536;
+  int64_t bytes = 0;
+  DevCsr A;  // FP64
+  DevMem perm;
+  std::vector<int> lev_ptr;
+  DevPrec<double> p64;
+  DevPrec<float> p32;
+};
+
+struct pgs_ws_s {
+  pgs_dev_op_s* op = nullptr;
+  int max_nrhs = 0, kdim = 0;
+  // apply buffers (per precision; float buffers sized as double for reuse)
+  DevMem r0, x0, lrpart, lrt;
+  std::vector<std::unique_ptr<DevMem>> zlev;
+  // Krylov
+  DevMem V, Z, w, part, hbuf, h2buf, t2, H, cs, sn, rs, coef, upd, st, zflag, nb2, nr2, flag;
+  int* flag_host = nullptr;
+  int part_blocks = 0;
+  int64_t v_bytes = 0;
+  // kernel timing of the level-0 triangular solve
+  bool prof = false;
+  std::vector<hipEvent_t> ev;         // pool, used in start/stop pairs
+  size_t ev_used = 0;
+  double prof_bytes = 0.0, prof_ms = 0.0;
+  int64_t prof_launches = 0;
+  ~pgs_ws_s() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+constexpr int kNT = 256;
+constexpr int kTrsvNT = 256;
+constexpr int kPMAX = 128;  // Krylov dimension + 1 limit of the shared-memory coefficient tables
+
+template <typename V>
+DevPrec<V>& prec_of(pgs_dev_op_s* op);
+template <>
+DevPrec<double>& prec_of<double>(pgs_dev_op_s* op) { return op->p64; }
+template <>
+DevPrec<float>& prec_of<float>(pgs_dev_op_s* op) { return op->p32; }
+
+template <typename VA, typename VX, typename VY, typename VACC>
+void launch_spmv(const DevCsr& M, const VX* x, VY* y, const VY* r, VACC alpha, VACC beta, int mode, int nrhs,
+                 hipStream_t s) {
+  if (M.nrows == 0) return;
+  if (nrhs == 1) {
+    const int64_t threads = (int64_t)M.nrows * M.lpr;
+    const unsigned g = nblocks_for(threads, kNT);
+#define PGS_SPMV_CASE(L) \
+  case L: k_spmv1<VA, VX, VY, VACC, L><<<g, kNT, 0, s>>>(M.nrows, M.ia.as<int>(), M.ja.as<int>(), M.a.as<VA>(), x, y, r, alpha, beta, mode); break;
+    switch (M.lpr) {
+      PGS_SPMV_CASE(2)
+      PGS_SPMV_CASE(4)
+      PGS_SPMV_CASE(8)
+      PGS_SPMV_CASE(16)
+      PGS_SPMV_CASE(32)
+      default: PGS_SPMV_CASE(64)
+    }
+#undef PGS_SPMV_CASE
+  } else {
+    const int64_t threads = (int64_t)M.nrows * nrhs;
+    k_spmvN<VA, VX, VY, VACC><<<nblocks_for(threads, kNT), kNT, 0, s>>>(M.nrows, nrhs, M.ia.as<int>(), M.ja.as<int>(),
+                                                                        M.a.as<VA>(), x, y, r, alpha, beta, mode);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// Folds the recorded event pairs into the totals (synchronizes them).
+void prof_flush(pgs_ws_s* ws) {
+  for (size_t i = 0; i + 1 < ws->ev_used; i += 2) {
+    HIPCHK(hipEventSynchronize(ws->ev[i + 1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ws->ev[i], ws->ev[i + 1]));
+    ws->prof_ms += ms;
+  }
+  ws->ev_used = 0;
+}
+
+hipEvent_t prof_event(pgs_ws_s* ws) {
+  if (ws->ev_used == ws->ev.size()) {
+    if (ws->ev.size() >= 4096) {
+      prof_flush(ws);
+    } else {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      ws->ev.push_back(e);
+    }
+  }
+  return ws->ev[ws->ev_used++];
+}
+
+template <typename V>
+void launch_trsv_impl(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s);
+
+// Level-set triangular solves of one GeMSLR level; the level-0 launches are
+// timed when the workspace profiler is on.
+template <typename V>
+void launch_trsv(pgs_dev_op_s* op, pgs_ws_s* ws, int level, const DevTrsv& T, const V* rhs, V* x, int nrhs,
+                 hipStream_t s) {
+  const bool timed = ws && ws->prof && level == 0;
+  if (timed) HIPCHK(hipEventRecord(prof_event(ws), s));
+  launch_trsv_impl<V>(op, T, rhs, x, nrhs, s);
+  if (timed) {
+    HIPCHK(hipEventRecord(prof_event(ws), s));
+    const double sv = sizeof(V);
+    ws->prof_bytes += (double)T.nnz_real * (sv + 4.0) + 2.0 * (T.nloc + 1) * 4.0 + T.nloc * sv +
+                      2.0 * (double)T.nloc * nrhs * sv;
+    ws->prof_launches += 1;
+  }
+}
+
+template <typename V>
+void launch_trsv_impl(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
+  if (T.nblocks == 0) return;
+  const size_t lds = (size_t)T.max_rows * sizeof(V);
+  if (nrhs == 1 && lds <= (size_t)op->lds_max) {
+    auto kern = k_trsv_lds<V, kTrsvNT, 4>;
+    if (lds > 65536) HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    kern<<<T.nblocks, kTrsvNT, lds, s>>>(T.row0.as<int>(), T.setL.as<int>(), T.setU.as<int>(), T.desc.as<int4>(),
+                                         T.rows.as<int>(), T.col.as<int>(), T.val.as<V>(), T.dinv.as<V>(), rhs, x);
+  } else {
+    k_trsv_glb<V, kTrsvNT><<<T.nblocks, kTrsvNT, 0, s>>>(T.row0.as<int>(), T.setL.as<int>(), T.setU.as<int>(),
+                                                         T.desc.as<int4>(), T.rows.as<int>(), T.col.as<int>(),
+                                                         T.val.as<V>(), T.dinv.as<V>(), rhs, x, nrhs);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+template <typename V>
+void launch_lowrank(pgs_ws_s* ws, const DevLevel& L, V* zl, int nrhs, hipStream_t s) {
+  const int m = L.nrem, k = L.rank;
+  if (k <= 0 || m <= 0) return;
+  const int rows_per_blk = 4096;
+  const int nblk = (m + rows_per_blk - 1) / rows_per_blk;
+  V* part = ws->lrpart.as<V>();
+  V* t = ws->lrt.as<V>();
+  k_lr_dot<V, kNT><<<nblk, kNT, 0, s>>>(m, k, nrhs, L.W.as<V>(), zl, part, rows_per_blk);
+  k_lr_final<V><<<nblocks_for((int64_t)k * nrhs, kNT), kNT, 0, s>>>(nblk, k, nrhs, part, t);
+  k_lr_update<V><<<nblocks_for((int64_t)m * nrhs, kNT), kNT, 0, s>>>(m, k, nrhs, L.WH.as<V>(), t, zl);
+  HIPCHK(hipGetLastError());
+}
+
+// SolveLevel (GemslrClass::SolveLevel, gemslr.cpp:3474-3625) on the device.
+template <typename V>
+void solve_level(pgs_dev_op_s* op, pgs_ws_s* ws, DevPrec<V>& P, int l, const V* rhs, V* x, int nrhs, hipStream_t s) {
+  DevLevel& L = *P.levels[l];
+  if (l == op->nlev - 1) {
+    launch_trsv<V>(op, ws, l, L.B, rhs, x, nrhs, s);
+    return;
+  }
+  const int nl = L.nloc;
+  V* z = ws->zlev[l]->as<V>();
+  V* zl = z + (int64_t)nl * nrhs;
+  // z_u = B^{-1} rhs_u
+  launch_trsv<V>(op, ws, l, L.B, rhs, z, nrhs, s);
+  // z_l = rhs_l - E z_u
+  launch_spmv<V, V, V, V>(L.E, z, zl, rhs + (int64_t)nl * nrhs, (V)1, (V)0, 1, nrhs, s);
+  // z_l += W H W^H z_l
+  launch_lowrank<V>(ws, L, zl, nrhs, s);
+  // x_l = SolveLevel(l+1, z_l)
+  solve_level<V>(op, ws, P, l + 1, zl, x + (int64_t)nl * nrhs, nrhs, s);
+  // z_u = rhs_u - F x_l
+  launch_spmv<V, V, V, V>(L.F, x + (int64_t)nl * nrhs, z, rhs, (V)1, (V)0, 1, nrhs, s);
+  // x_u = B^{-1} z_u
+  launch_trsv<V>(op, ws, l, L.B, z, x, nrhs, s);
+}
+
+// z = M^{-1} v with v in double, z in VZ, apply arithmetic V.
+template <typename V, typename VZ>
+void apply_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* v, VZ* z, int nrhs, hipStream_t s) {
+  DevPrec<V>& P = prec_of<V>(op);
+  if (!P.present) throw std::runtime_error("operator was uploaded without this precision");
+  const int64_t n = op->n;
+  V* r0 = ws->r0.as<V>();
+  V* x0 = ws->x0.as<V>();
+  k_gather<double, V><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
+  HIPCHK(hipGetLastError());
+  solve_level<V>(op, ws, P, 0, r0, x0, nrhs, s);
+  k_scatter<V, VZ><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), x0, z);
+  HIPCHK(hipGetLastError());
+}
+
+int set_err(const HipError& e) {
+  g_last_hip_error = hipGetErrorString(e.e);
+  if (e.e == hipErrorOutOfMemory) return PGS_ERR_OOM;
+  return PGS_ERR_HIP;
+}
+
+bool pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int pgs_device_count(int* count) {
+  if (!count) return PGS_ERROR_INVALID_PARAM;
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) { *count = 0; return PGS_ERR_NO_DEVICE; }
+  *count = c;
+  return 0;
+}
+
+int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t* out) {
+  if (!hop || !out) return PGS_ERROR_INVALID_PARAM;
+  *out = nullptr;
+  pgs_gemslr_s* h = reinterpret_cast<pgs_gemslr_s*>(hop);
+  if (h->is_complex) return PGS_ERR_UNSUPPORTED;
+  if ((prec_mask & (PGS_PREC_FP64 | PGS_PREC_FP32)) == 0) return PGS_ERROR_INVALID_PARAM;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PGS_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return PGS_ERROR_INVALID_PARAM;
+  std::unique_ptr<pgs_dev_op_s> op(new pgs_dev_op_s());
+  try {
+    HIPCHK(hipSetDevice(device));
+    op->device = device;
+    int lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    op->lds_max = lds;
+    const HostOp<double>& H = *h->op_d;
+    op->n = H.n;
+    op->nlev = H.nlev;
+    op->lev_ptr = H.lev_ptr;
+    upload_csr<double>(op->A, H.A, &op->bytes);
+    upload(op->perm, H.perm, &op->bytes);
+    auto build = [&](auto& P, auto tag) {
+      using V = decltype(tag);
+      P.present = true;
+      P.levels.clear();
+      for (int l = 0; l < H.nlev; ++l) {
+        const HostLevel<double>& HL = H.levels[l];
+        std::unique_ptr<DevLevel> L(new DevLevel());
+        L->lev_start = H.lev_ptr[l];
+        L->nloc = H.nloc(l);
+        L->nrem = H.nrem(l);
+        TrsvPack<double> pk = trsv_pack(HL, H.lev_ptr[l], H.nloc(l));
+        upload_trsv<V>(L->B, pk, &op->bytes);
+        if (l + 1 < H.nlev) {
+          upload_csr<V>(L->E, HL.E, &op->bytes);
+          upload_csr<V>(L->F, HL.F, &op->bytes);
+          L->rank = HL.rank;
+          if (HL.rank > 0) {
+            upload(L->W, to_row_major<V>(HL.W, L->nrem, HL.rank), &op->bytes);
+            upload(L->WH, to_row_major<V>(HL.WH, L->nrem, HL.rank), &op->bytes);
+          }
+        }
+        P.levels.push_back(std::move(L));
+      }
+    };
+    if (prec_mask & PGS_PREC_FP64) build(op->p64, double(0));
+    if (prec_mask & PGS_PREC_FP32) build(op->p32, float(0));
+    HIPCHK(hipDeviceSynchronize());
+  } catch (const HipError& e) {
+    return set_err(e);
+  } catch (const std::bad_alloc&) {
+    return PGS_ERR_OOM;
+  }
+  *out = reinterpret_cast<pgs_dev_op_t>(op.release());
+  return 0;
+}
+
+int pgs_dev_op_destroy(pgs_dev_op_t dop) {
+  if (!dop) return 0;
+  pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
+  (void)hipSetDevice(op->device);
+  delete op;
+  return 0;
+}
+
+int pgs_dev_op_bytes(pgs_dev_op_t dop, int64_t* bytes) {
+  if (!dop || !bytes) return PGS_ERROR_INVALID_PARAM;
+  *bytes = reinterpret_cast<pgs_dev_op_s*>(dop)->bytes;
+  return 0;
+}
+
+int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
+  if (!dop || !out || max_nrhs < 1 || max_nrhs > 64 || kdim < 1 || kdim + 1 > kPMAX) return PGS_ERROR_INVALID_PARAM;
+  *out = nullptr;
+  pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
+  int nr = 1;
+  while (nr < max_nrhs) nr <<= 1;
+  std::unique_ptr<pgs_ws_s> ws(new pgs_ws_s());
+  ws->op = op;
+  ws->max_nrhs = nr;
+  ws->kdim = kdim;
+  try {
+    HIPCHK(hipSetDevice(op->device));
+    const int64_t n = op->n;
+    const size_t vb = (size_t)n * nr * sizeof(double);
+    ws->r0.alloc(vb);
+    ws->x0.alloc(vb);
+    ws->zlev.clear();
+    for (int l = 0; l < op->nlev; ++l) {
+      std::unique_ptr<DevMem> m(new DevMem());
+      m->alloc((size_t)(n - op->lev_ptr[l]) * nr * sizeof(double));
+      ws->zlev.push_back(std::move(m));
+    }
+    int maxrank = 0, maxm = 0;
+    for (int l = 0; l + 1 < op->nlev; ++l) {
+      const DevPrec<double>& P64 = op->p64;
+      const DevPrec<float>& P32 = op->p32;
+      int rk = P64.present ? P64.levels[l]->rank : P32.levels[l]->rank;
+      maxrank = std::max(maxrank, rk);
+      maxm = std::max(maxm, n > 0 ? (int)(n - op->lev_ptr[l + 1]) : 0);
+    }
+    const int lr_blocks = (maxm + 4095) / 4096 + 1;
+    ws->lrpart.alloc((size_t)std::max(1, lr_blocks * maxrank * nr) * sizeof(double));
+    ws->lrt.alloc((size_t)std::max(1, maxrank * nr) * sizeof(double));
+    // Krylov
+    ws->V.alloc((size_t)(kdim + 1) * vb);
+    ws->Z.alloc((size_t)kdim * vb);
+    ws->v_bytes = (int64_t)vb;
+    const int64_t rows_per_blk = std::max<int64_t>(2048, (n + 1023) / 1024);
+    ws->part_blocks = (int)((n + rows_per_blk - 1) / rows_per_blk);
+    ws->part.alloc((size_t)ws->part_blocks * (kdim + 1) * nr * sizeof(double));
+    ws->hbuf.alloc((size_t)(kdim + 1) * nr * sizeof(double));
+    ws->h2buf.alloc((size_t)(kdim + 1) * nr * sizeof(double));
+    ws->t2.alloc((size_t)nr * sizeof(double));
+    ws->H.alloc((size_t)nr * (kdim + 1) * kdim * sizeof(double));
+    ws->cs.alloc((size_t)nr * kdim * sizeof(double));
+    ws->sn.alloc((size_t)nr * kdim * sizeof(double));
+    ws->rs.alloc((size_t)nr * (kdim + 1) * sizeof(double));
+    ws->coef.alloc((size_t)(kdim + 1) * nr * sizeof(double));
+    ws->upd.alloc((size_t)nr * sizeof(int));
+    ws->st.alloc((size_t)nr * sizeof(KState));
+    ws->zflag.alloc((size_t)nr * sizeof(double));
+    ws->nb2.alloc((size_t)nr * sizeof(double));
+    ws->nr2.alloc((size_t)nr * sizeof(double));
+    ws->flag.alloc(16);
+    HIPCHK(hipHostMalloc((void**)&ws->flag_host, 16, hipHostMallocDefault));
+  } catch (const HipError& e) {
+    return set_err(e);
+  }
+  *out = reinterpret_cast<pgs_ws_t>(ws.release());
+  return 0;
+}
+
+int pgs_ws_destroy(pgs_ws_t w) {
+  if (!w) return 0;
+  pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
+  (void)hipSetDevice(ws->op->device);
+  if (ws->flag_host) (void)hipHostFree(ws->flag_host);
+  delete ws;
+  return 0;
+}
+
+int pgs_spmv(pgs_dev_op_t dop, pgs_ws_t w, const double* alpha, const double* x, const double* beta, double* y,
+             int nrhs, void* stream) {
+  if (!dop || !x || !y || !alpha || !beta || !pow2(nrhs) || nrhs > 64) return PGS_ERROR_INVALID_PARAM;
+  pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
+  (void)w;
+  try {
+    HIPCHK(hipSetDevice(op->device));
+    launch_spmv<double, double, double, double>(op->A, x, y, (const double*)nullptr, alpha[0], beta[0], 0, nrhs,
+                                                (hipStream_t)stream);
+  } catch (const HipError& e) {
+    return set_err(e);
+  }
+  return 0;
+}
+
+int pgs_apply(pgs_dev_op_t dop, pgs_ws_t w, const double* v, double* z, int nrhs, int prec, void* stream) {
+  if (!dop || !w || !v || !z) return PGS_ERROR_INVALID_PARAM;
+  pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
+  pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
+  if (ws->op != op || !pow2(nrhs) || nrhs > ws->max_nrhs) return PGS_ERROR_INVALID_PARAM;
+  try {
+    HIPCHK(hipSetDevice(op->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == PGS_PREC_FP64) {
+      if (!op->p64.present) return PGS_ERROR_INVALID_OPTION;
+      apply_impl<double, double>(op, ws, v, z, nrhs, s);
+    } else if (prec == PGS_PREC_FP32) {
+      if (!op->p32.present) return PGS_ERROR_INVALID_OPTION;
+      apply_impl<float, double>(op, ws, v, z, nrhs, s);
+    } else {
+      return PGS_ERROR_INVALID_OPTION;
+    }
+  } catch (const HipError& e) {
+    return set_err(e);
+  } catch (const std::exception&) {
+    return PGS_ERROR_FUNCTION_CALL;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// multi-dot over the first p vectors of V against w: out[j*nrhs+c]
+template <typename VV, typename VW>
+void multidot(pgs_ws_s* ws, int64_t n, int nrhs, int p, const VV* V, int64_t ldv, const VW* w, double* out,
+              int accumulate, hipStream_t s) {
+  const int64_t rows_per_blk = (n + ws->part_blocks - 1) / ws->part_blocks;
+  const int nblk = (int)((n + rows_per_blk - 1) / rows_per_blk);
+  k_multidot<VV, VW, kNT><<<nblk, kNT, 0, s>>>(n, nrhs, p, V, ldv, w, ws->part.as<double>(), rows_per_blk);
+  k_dot_final<<<nblocks_for((int64_t)p * nrhs, kNT), kNT, 0, s>>>(nblk, p, nrhs, ws->part.as<double>(), out, accumulate);
+  HIPCHK(hipGetLastError());
+}
+
+template <typename VZ>
+int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int nrhs, const pgs_fgmres_opts_t& o,
+                pgs_fgmres_result_t* res, hipStream_t s) {
+  const int64_t n = op->n;
+  const int m = o.kdim;
+  const int64_t ldv = n * nrhs;
+  double* V = ws->V.as<double>();
+  VZ* Z = ws->Z.as<VZ>();
+  KState* st = ws->st.as<KState>();
+  double* H = ws->H.as<double>();
+  double* rs = ws->rs.as<double>();
+  const int nb = nblocks_for(n * nrhs, kNT);
+  const int check = o.check_every > 0 ? o.check_every : 1;
+  const int maxits = o.maxits;
+  // ||b||, v0 = b - A x, ||v0||
+  multidot<double, double>(ws, n, nrhs, 1, b, ldv, b, ws->nb2.as<double>(), 0, s);
+  launch_spmv<double, double, double, double>(op->A, x, V, b, 1.0, 0.0, 1, nrhs, s);
+  multidot<double, double>(ws, n, nrhs, 1, V, ldv, V, ws->nr2.as<double>(), 0, s);
+  k_fgmres_init<<<1, 64, 0, s>>>(nrhs, ws->nb2.as<double>(), ws->nr2.as<double>(), o.tol, o.absolute_tol, st,
+                                 ws->zflag.as<double>());
+  k_zero_cols<<<nb, kNT, 0, s>>>(n, nrhs, ws->zflag.as<double>(), x);
+  HIPCHK(hipGetLastError());
+  int* hflag = ws->flag_host;
+  // number of running (restartable) or active columns, read back through pinned memory
+  auto read_flag = [&](bool running) -> int {
+    if (running)
+      k_count_running<<<1, 1, 0, s>>>(nrhs, st, maxits, ws->flag.as<int>());
+    else
+      k_count_active<<<1, 1, 0, s>>>(nrhs, st, ws->flag.as<int>());
+    HIPCHK(hipMemcpyAsync(hflag, ws->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return hflag[0];
+  };
+  int iters_host = 0;
+  while (true) {
+    if (read_flag(true) == 0) break;
+    k_cycle_start<<<1, 64, 0, s>>>(nrhs, m, st, rs);
+    k_scale_cols<<<nb, kNT, 0, s>>>(n, nrhs, st, V);
+    int i = 0;
+    while (i < m && iters_host < maxits) {
+      ++i;
+      ++iters_host;
+      double* vprev = V + (int64_t)(i - 1) * ldv;
+      double* w = V + (int64_t)i * ldv;
+      VZ* zc = Z + (int64_t)(i - 1) * ldv;
+      if (o.use_precond) {
+        if (o.prec == PGS_PREC_FP64)
+          apply_impl<double, VZ>(op, ws, vprev, zc, nrhs, s);
+        else
+          apply_impl<float, VZ>(op, ws, vprev, zc, nrhs, s);
+      } else {
+        k_copy_conv<VZ><<<nb, kNT, 0, s>>>(n * nrhs, vprev, zc);
+      }
+      launch_spmv<double, VZ, double, double>(op->A, zc, w, (const double*)nullptr, 1.0, 0.0, 0, nrhs, s);
+      double* h = ws->hbuf.as<double>();
+      double* h2 = ws->h2buf.as<double>();
+      if (o.orth == PGS_ORTH_CGS2) {
+        multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h, 0, s);
+        k_multi_axpy<<<nb, kNT, sizeof(double) * i * nrhs, s>>>(n, nrhs, i, V, ldv, h, w);
+        multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h2, 0, s);
+        k_multi_axpy<<<nb, kNT, sizeof(double) * i * nrhs, s>>>(n, nrhs, i, V, ldv, h2, w);
+        k_dot_final<<<nblocks_for((int64_t)i * nrhs, kNT), kNT, 0, s>>>(1, i, nrhs, h2, h, 1);
+      } else {
+        for (int j = 0; j < i; ++j) {
+          multidot<double, double>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, w, h + (int64_t)j * nrhs, 0, s);
+          k_multi_axpy<<<nb, kNT, sizeof(double) * nrhs, s>>>(n, nrhs, 1, V + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w);
+        }
+      }
+      multidot<double, double>(ws, n, nrhs, 1, w, ldv, w, ws->t2.as<double>(), 0, s);
+      k_givens<<<1, 64, 0, s>>>(nrhs, m, i, maxits, h, ws->t2.as<double>(), H, ws->cs.as<double>(),
+                                ws->sn.as<double>(), rs, st);
+      k_scale_cols<<<nb, kNT, 0, s>>>(n, nrhs, st, w);
+      HIPCHK(hipGetLastError());
+      if (i % check == 0 || i == m) {
+        if (read_flag(false) == 0) break;
+      }
+    }
+    // x += Z y
+    k_backsolve<<<1, 64, 0, s>>>(nrhs, m, H, rs, st, ws->coef.as<double>(), ws->upd.as<int>());
+    k_update_x<VZ><<<nb, kNT, sizeof(double) * std::max(1, i) * nrhs, s>>>(n, nrhs, i, Z, ldv, ws->coef.as<double>(), ws->upd.as<int>(), x);
+    HIPCHK(hipGetLastError());
+    if (iters_host >= maxits) break;
+    if (read_flag(true) == 0) break;
+    // restart: v0 = b - A x (fgmres.hpp:500-514); normr keeps the Givens estimate
+    launch_spmv<double, double, double, double>(op->A, x, V, b, 1.0, 0.0, 1, nrhs, s);
+  }
+  std::vector<KState> hs(nrhs);
+  HIPCHK(hipMemcpyAsync(hs.data(), st, sizeof(KState) * nrhs, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int c = 0; c < nrhs; ++c) {
+    res[c].iters = hs[c].iters;
+    res[c].rel_res = hs[c].normb > 0 ? hs[c].normr / hs[c].normb : 0.0;
+    if (hs[c].status == 1 || hs[c].status == 4) res[c].status = 0;
+    else if (hs[c].status == 2) res[c].status = PGS_ERR_BREAKDOWN;
+    else res[c].status = PGS_ERR_NOT_CONVERGED;
+    if (hs[c].status == 4) res[c].rel_res = 0.0;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int pgs_fgmres(pgs_dev_op_t dop, pgs_ws_t w, const double* b, double* x, int nrhs,
+                          const pgs_fgmres_opts_t* opts, pgs_fgmres_result_t* res, void* stream) {
+  if (!dop || !w || !b || !x || !opts || !res) return PGS_ERROR_INVALID_PARAM;
+  pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
+  pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
+  if (ws->op != op || !pow2(nrhs) || nrhs > ws->max_nrhs) return PGS_ERROR_INVALID_PARAM;
+  if (opts->kdim < 1 || opts->kdim > ws->kdim || opts->maxits < 0) return PGS_ERROR_INVALID_PARAM;
+  if (opts->orth != PGS_ORTH_MGS && opts->orth != PGS_ORTH_CGS2) return PGS_ERROR_INVALID_OPTION;
+  if (opts->use_precond && opts->prec != PGS_PREC_FP64 && opts->prec != PGS_PREC_FP32) return PGS_ERROR_INVALID_OPTION;
+  if (opts->use_precond && opts->prec == PGS_PREC_FP64 && !op->p64.present) return PGS_ERROR_INVALID_OPTION;
+  if (opts->use_precond && opts->prec == PGS_PREC_FP32 && !op->p32.present) return PGS_ERROR_INVALID_OPTION;
+  try {
+    HIPCHK(hipSetDevice(op->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (opts->use_precond && opts->prec == PGS_PREC_FP32)
+      return fgmres_impl<float>(op, ws, b, x, nrhs, *opts, res, s);
+    return fgmres_impl<double>(op, ws, b, x, nrhs, *opts, res, s);
+  } catch (const HipError& e) {
+    return set_err(e);
+  } catch (const std::exception&) {
+    return PGS_ERROR_FUNCTION_CALL;
+  }
+}
+
+extern "C" const char* pgs_last_hip_error(void) { return g_last_hip_error.c_str(); }
+
+namespace {
+int hip_rc(hipError_t e) {
+  if (e == hipSuccess) return 0;
+  return set_err(HipError{e});
+}
+}  // namespace
+
+extern "C" int pgs_dev_malloc(int device, int64_t bytes, void** ptr) {
+  if (!ptr || bytes < 0) return PGS_ERROR_INVALID_PARAM;
+  *ptr = nullptr;
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc) return rc;
+  return hip_rc(hipMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1)));
+}
+
+extern "C" int pgs_dev_free(int device, void* ptr) {
+  if (!ptr) return 0;
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc) return rc;
+  return hip_rc(hipFree(ptr));
+}
+
+extern "C" int pgs_memcpy_htod(int device, void* dst, const void* src, int64_t bytes, void* stream) {
+  if (bytes < 0 || (bytes > 0 && (!dst || !src))) return PGS_ERROR_INVALID_PARAM;
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc || bytes == 0) return rc;
+  rc = hip_rc(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  if (rc) return rc;
+  return hip_rc(hipStreamSynchronize((hipStream_t)stream));
+}
+
+extern "C" int pgs_memcpy_dtoh(int device, void* dst, const void* src, int64_t bytes, void* stream) {
+  if (bytes < 0 || (bytes > 0 && (!dst || !src))) return PGS_ERROR_INVALID_PARAM;
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc || bytes == 0) return rc;
+  rc = hip_rc(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  if (rc) return rc;
+  return hip_rc(hipStreamSynchronize((hipStream_t)stream));
+}
+
+extern "C" int pgs_memset_zero(int device, void* ptr, int64_t bytes, void* stream) {
+  if (bytes < 0 || (bytes > 0 && !ptr)) return PGS_ERROR_INVALID_PARAM;
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc || bytes == 0) return rc;
+  return hip_rc(hipMemsetAsync(ptr, 0, (size_t)bytes, (hipStream_t)stream));
+}
+
+extern "C" int pgs_stream_create(int device, void** stream) {
+  if (!stream) return PGS_ERROR_INVALID_PARAM;
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc) return rc;
+  hipStream_t s = nullptr;
+  rc = hip_rc(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void*)s;
+  return rc;
+}
+
+extern "C" int pgs_stream_destroy(int device, void* stream) {
+  if (!stream) return 0;
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc) return rc;
+  return hip_rc(hipStreamDestroy((hipStream_t)stream));
+}
+
+extern "C" int pgs_stream_synchronize(int device, void* stream) {
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc) return rc;
+  return hip_rc(hipStreamSynchronize((hipStream_t)stream));
+}
+
+extern "C" int pgs_device_synchronize(int device) {
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc) return rc;
+  return hip_rc(hipDeviceSynchronize());
+}
+
+extern "C" int pgs_ws_profile(pgs_ws_t w, int enable) {
+  if (!w) return PGS_ERROR_INVALID_PARAM;
+  pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
+  try {
+    HIPCHK(hipSetDevice(ws->op->device));
+    prof_flush(ws);
+  } catch (const HipError& e) {
+    return set_err(e);
+  }
+  ws->prof = enable != 0;
+  ws->prof_ms = 0.0;
+  ws->prof_bytes = 0.0;
+  ws->prof_launches = 0;
+  return 0;
+}
+
+extern "C" int pgs_ws_profile_read(pgs_ws_t w, pgs_kernel_profile_t* out) {
+  if (!w || !out) return PGS_ERROR_INVALID_PARAM;
+  pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
+  try {
+    HIPCHK(hipSetDevice(ws->op->device));
+    prof_flush(ws);
+  } catch (const HipError& e) {
+    return set_err(e);
+  }
+  out->launches = ws->prof_launches;
+  out->total_ms = ws->prof_ms;
+  out->total_bytes = ws->prof_bytes;
+  return 0;
+}

@@ -1,0 +1,487 @@
+// HIP kernels of the GeMSLR-FGMRES solve path (gfx950, wave64).
+//
+// Vector blocks are row-interleaved: entry (row i, column c) of an n x nrhs
+// block is at [i * nrhs + c]; nrhs is a power of two <= 64 so that a wave
+// holds whole rows of columns.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pgs {
+namespace dev {
+
+constexpr int kWave = 64;
+
+template <typename To, typename From>
+__device__ __forceinline__ To cvt(From v) { return (To)v; }
+
+// ---------------------------------------------------------------------------
+// permutations (IntVectorClass::GatherPerm / ScatterRperm,
+// ParGeMSLR/SRC/vectors/int_vector.cpp:629-696) fused with a precision change
+// ---------------------------------------------------------------------------
+template <typename VI, typename VO>
+__global__ void k_gather(int64_t n, int nrhs, const int* __restrict__ perm, const VI* __restrict__ in, VO* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nrhs) return;
+  int64_t i = t / nrhs;
+  int c = (int)(t - i * nrhs);
+  out[t] = cvt<VO>(in[(int64_t)perm[i] * nrhs + c]);
+}
+
+template <typename VI, typename VO>
+__global__ void k_scatter(int64_t n, int nrhs, const int* __restrict__ perm, const VI* __restrict__ in, VO* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nrhs) return;
+  int64_t i = t / nrhs;
+  int c = (int)(t - i * nrhs);
+  out[(int64_t)perm[i] * nrhs + c] = cvt<VO>(in[t]);
+}
+
+template <typename VI, typename VO>
+__global__ void k_convert(int64_t m, const VI* __restrict__ in, VO* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) out[t] = cvt<VO>(in[t]);
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV (CsrMatrixPMatVecHostTemplate, ParGeMSLR/SRC/matrices/
+// matrixops.cpp:5139-5330).  mode 0: y = alpha*A*x + beta*y (beta == 0 does
+// not read y); mode 1: y = r - A*x.  Accumulation in VACC.
+// Single RHS: LPR lanes per row, entries strided over the lanes, DPP/shuffle
+// reduction.  Multi RHS: one thread per (row, column) over the row's entries.
+// ---------------------------------------------------------------------------
+template <typename VA, typename VX, typename VY, typename VACC, int LPR>
+__global__ void k_spmv1(int nrows, const int* __restrict__ ia, const int* __restrict__ ja, const VA* __restrict__ a,
+                        const VX* __restrict__ x, VY* __restrict__ y, const VY* __restrict__ r, VACC alpha, VACC beta,
+                        int mode) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int row = (int)(g / LPR);
+  int q = (int)(g % LPR);
+  VACC acc = 0;
+  if (row < nrows) {
+    int k1 = ia[row + 1];
+    for (int k = ia[row] + q; k < k1; k += LPR) acc += (VACC)a[k] * (VACC)x[ja[k]];
+  }
+#pragma unroll
+  for (int m = LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, LPR);
+  if (row < nrows && q == 0) {
+    if (mode == 1) y[row] = cvt<VY>((VACC)r[row] - acc);
+    else if (beta == (VACC)0) y[row] = cvt<VY>(alpha * acc);
+    else y[row] = cvt<VY>(alpha * acc + beta * (VACC)y[row]);
+  }
+}
+
+template <typename VA, typename VX, typename VY, typename VACC>
+__global__ void k_spmvN(int nrows, int nrhs, const int* __restrict__ ia, const int* __restrict__ ja,
+                        const VA* __restrict__ a, const VX* __restrict__ x, VY* __restrict__ y,
+                        const VY* __restrict__ r, VACC alpha, VACC beta, int mode) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nrows * nrhs) return;
+  int row = (int)(t / nrhs);
+  int c = (int)(t - (int64_t)row * nrhs);
+  VACC acc = 0;
+  int k1 = ia[row + 1];
+  for (int k = ia[row]; k < k1; ++k) acc += (VACC)a[k] * (VACC)x[(int64_t)ja[k] * nrhs + c];
+  if (mode == 1) y[t] = cvt<VY>((VACC)r[t] - acc);
+  else if (beta == (VACC)0) y[t] = cvt<VY>(alpha * acc);
+  else y[t] = cvt<VY>(alpha * acc + beta * (VACC)y[t]);
+}
+
+// ---------------------------------------------------------------------------
+// Block-diagonal ILU solve x = U^{-1} L^{-1} rhs (IluClass::SolveHost,
+// ParGeMSLR/SRC/preconditioners/ilu.cpp:3752-3783): one workgroup per diagonal
+// block, level sets in sequence separated by workgroup barriers.  x is first
+// loaded with rhs, then both sweeps run in place.
+//   desc[4*s] = {row_off, ent_off, nrows, width}
+// LDS variant: single RHS, block vector held in LDS; LPR lanes per row.
+// ---------------------------------------------------------------------------
+template <typename V, int NT, int LPR>
+__global__ __launch_bounds__(NT) void k_trsv_lds(const int* __restrict__ blk_row0, const int* __restrict__ setL,
+                                                 const int* __restrict__ setU, const int4* __restrict__ desc,
+                                                 const int* __restrict__ rows, const int* __restrict__ col,
+                                                 const V* __restrict__ val, const V* __restrict__ dinv,
+                                                 const V* __restrict__ rhs, V* __restrict__ x) {
+  extern __shared__ unsigned char smem_raw[];
+  V* xs = reinterpret_cast<V*>(smem_raw);
+  const int b = blockIdx.x;
+  const int r0 = blk_row0[b], nb = blk_row0[b + 1] - r0;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nb; i += NT) xs[i] = rhs[r0 + i];
+  __syncthreads();
+  const int q = tid % LPR;
+  const int t0 = tid / LPR;
+  constexpr int RPP = NT / LPR;  // rows per pass
+  // L sweep
+  const int sL0 = setL[b], sL1 = setU[b], sU1 = setL[b + 1];
+  for (int s = sL0; s < sL1; ++s) {
+    const int4 d = desc[s];
+    for (int t = t0; t < d.z; t += RPP) {
+      V acc = 0;
+      for (int j = q; j < d.w; j += LPR) {
+        const int e = d.y + j * d.z + t;
+        acc += val[e] * xs[col[e]];
+      }
+#pragma unroll
+      for (int m = LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, LPR);
+      if (q == 0) {
+        const int row = rows[d.x + t];
+        xs[row] = xs[row] - acc;
+      }
+    }
+    __syncthreads();
+  }
+  // U sweep
+  for (int s = sL1; s < sU1; ++s) {
+    const int4 d = desc[s];
+    for (int t = t0; t < d.z; t += RPP) {
+      V acc = 0;
+      for (int j = q; j < d.w; j += LPR) {
+        const int e = d.y + j * d.z + t;
+        acc += val[e] * xs[col[e]];
+      }
+#pragma unroll
+      for (int m = LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, LPR);
+      if (q == 0) {
+        const int row = rows[d.x + t];
+        xs[row] = (xs[row] - acc) * dinv[d.x + t];
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < nb; i += NT) x[r0 + i] = xs[i];
+}
+
+// Global-memory variant: any block size, nrhs columns (row-interleaved); one
+// thread per (row of the set, column).
+template <typename V, int NT>
+__global__ __launch_bounds__(NT) void k_trsv_glb(const int* __restrict__ blk_row0, const int* __restrict__ setL,
+                                                 const int* __restrict__ setU, const int4* __restrict__ desc,
+                                                 const int* __restrict__ rows, const int* __restrict__ col,
+                                                 const V* __restrict__ val, const V* __restrict__ dinv,
+                                                 const V* __restrict__ rhs, V* __restrict__ x, int nrhs) {
+  const int b = blockIdx.x;
+  const int r0 = blk_row0[b], nb = blk_row0[b + 1] - r0;
+  const int tid = threadIdx.x;
+  const V* rb = rhs + (int64_t)r0 * nrhs;
+  V* xb = x + (int64_t)r0 * nrhs;
+  for (int i = tid; i < nb * nrhs; i += NT) xb[i] = rb[i];
+  __syncthreads();
+  const int sL0 = setL[b], sL1 = setU[b], sU1 = setL[b + 1];
+  for (int s = sL0; s < sU1; ++s) {
+    const int4 d = desc[s];
+    const bool upper = (s >= sL1);
+    const int items = d.z * nrhs;
+    for (int it = tid; it < items; it += NT) {
+      const int t = it / nrhs;
+      const int c = it - t * nrhs;
+      const int row = rows[d.x + t];
+      V acc = xb[(int64_t)row * nrhs + c];
+      for (int j = 0; j < d.w; ++j) {
+        const int e = d.y + j * d.z + t;
+        acc -= val[e] * xb[(int64_t)col[e] * nrhs + c];
+      }
+      if (upper) acc *= dinv[d.x + t];
+      xb[(int64_t)row * nrhs + c] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Low-rank correction z += WH (W^H z) (GemslrClass::SolveApplyLowRankLevel,
+// ParGeMSLR/SRC/preconditioners/gemslr.cpp:3681-3719).  W, WH row-major m x k.
+// Stage 1: per-workgroup partial sums of W^H z for every (j, column);
+// stage 2: sum of the partials (fixed order -> deterministic);
+// stage 3: z[i, c] += sum_j WH[i, j] t[j, c].
+// ---------------------------------------------------------------------------
+template <typename V, int NT>
+__global__ __launch_bounds__(NT) void k_lr_dot(int m, int k, int nrhs, const V* __restrict__ W,
+                                               const V* __restrict__ z, V* __restrict__ part, int rows_per_blk) {
+  // thread layout: c = tid % nrhs, row lane = tid / nrhs
+  __shared__ V sh[NT];
+  const int tid = threadIdx.x;
+  const int c = tid % nrhs;
+  const int lanes = NT / nrhs;
+  const int rl = tid / nrhs;
+  const int i0 = blockIdx.x * rows_per_blk;
+  const int i1 = min(m, i0 + rows_per_blk);
+  for (int j = 0; j < k; ++j) {
+    V acc = 0;
+    for (int i = i0 + rl; i < i1; i += lanes) acc += W[(int64_t)i * k + j] * z[(int64_t)i * nrhs + c];
+    sh[tid] = acc;
+    __syncthreads();
+    for (int s = NT / 2; s >= nrhs; s >>= 1) {
+      if (tid < s) sh[tid] += sh[tid + s];
+      __syncthreads();
+    }
+    if (tid < nrhs) part[((int64_t)blockIdx.x * k + j) * nrhs + tid] = sh[tid];
+    __syncthreads();
+  }
+}
+
+template <typename V>
+__global__ void k_lr_final(int nblk, int k, int nrhs, const V* __restrict__ part, V* __restrict__ t) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= k * nrhs) return;
+  V s = 0;
+  for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * k * nrhs + g];
+  t[g] = s;
+}
+
+template <typename V>
+__global__ void k_lr_update(int m, int k, int nrhs, const V* __restrict__ WH, const V* __restrict__ t, V* __restrict__ z) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)m * nrhs) return;
+  int64_t i = g / nrhs;
+  int c = (int)(g - i * nrhs);
+  V acc = 0;
+  for (int j = 0; j < k; ++j) acc += WH[i * k + j] * t[j * nrhs + c];
+  z[g] += acc;
+}
+
+// ---------------------------------------------------------------------------
+// Krylov kernels (FP64).  A block of p vectors V_j (each n x nrhs) at
+// V + j*ldv.  State per column in KState.
+// ---------------------------------------------------------------------------
+struct KState {
+  double normb, tolr, normr, scale;
+  int active;   // running in the current cycle
+  int status;   // 0 running, 1 converged, 2 breakdown, 3 max iterations, 4 zero rhs/residual
+  int iters;    // total inner iterations (FlexGmresClass::_iter)
+  int icur;     // inner steps of the current cycle
+};
+
+// partial dot products d[j, c] = sum_rows V_j[r, c] * w[r, c] for j < p
+// (w may be one of the V_j).  One partial per workgroup.
+template <typename VV, typename VW, int NT>
+__global__ __launch_bounds__(NT) void k_multidot(int64_t n, int nrhs, int p, const VV* __restrict__ V, int64_t ldv,
+                                                 const VW* __restrict__ w, double* __restrict__ part,
+                                                 int64_t rows_per_blk) {
+  __shared__ double sh[NT];
+  const int tid = threadIdx.x;
+  const int c = tid % nrhs;
+  const int lanes = NT / nrhs;
+  const int rl = tid / nrhs;
+  const int64_t i0 = (int64_t)blockIdx.x * rows_per_blk;
+  const int64_t i1 = min(n, i0 + rows_per_blk);
+  for (int j = 0; j < p; ++j) {
+    const VV* vj = V + j * ldv;
+    double acc = 0;
+    for (int64_t i = i0 + rl; i < i1; i += lanes) acc += (double)vj[i * nrhs + c] * (double)w[i * nrhs + c];
+    // reduce across lanes that share column c
+    for (int m = kWave / 2; m >= nrhs; m >>= 1) acc += __shfl_xor(acc, m);
+    const int wid = tid / kWave, lane = tid % kWave;
+    if (lane < nrhs) sh[wid * nrhs + lane] = acc;
+    __syncthreads();
+    if (tid < nrhs) {
+      double s = 0;
+      for (int q = 0; q < NT / kWave; ++q) s += sh[q * nrhs + tid];
+      part[((int64_t)blockIdx.x * p + j) * nrhs + tid] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// out[j*nrhs + c] = sum over blocks of the partials (+ out if accumulate)
+__global__ void k_dot_final(int nblk, int p, int nrhs, const double* __restrict__ part, double* __restrict__ out,
+                            int accumulate) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= p * nrhs) return;
+  double s = 0;
+  for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * p * nrhs + g];
+  out[g] = accumulate ? out[g] + s : s;
+}
+
+// w[r, c] -= sum_{j<p} V_j[r, c] * h[j, c]   (dynamic shared memory: p*nrhs doubles)
+__global__ void k_multi_axpy(int64_t n, int nrhs, int p, const double* __restrict__ V, int64_t ldv,
+                             const double* __restrict__ h, double* __restrict__ w) {
+  extern __shared__ double hs[];
+  for (int t = threadIdx.x; t < p * nrhs; t += blockDim.x) hs[t] = h[t];
+  __syncthreads();
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n * nrhs) return;
+  const int c = (int)(g % nrhs);
+  double acc = w[g];
+  for (int j = 0; j < p; ++j) acc -= V[j * ldv + g] * hs[j * nrhs + c];
+  w[g] = acc;
+}
+
+// v[r, c] *= s[c]   (s from KState.scale)
+__global__ void k_scale_cols(int64_t n, int nrhs, const KState* __restrict__ st, double* __restrict__ v) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n * nrhs) return;
+  v[g] *= st[g % nrhs].scale;
+}
+
+// FGMRES start (fgmres.hpp:318-359): normb, normr from squared norms.
+__global__ void k_fgmres_init(int nrhs, const double* __restrict__ nb2, const double* __restrict__ nr2, double tol,
+                              int absolute, KState* __restrict__ st, double* __restrict__ x_zero_flag) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  const double eps = 2.220446049250313e-16;
+  KState s;
+  s.normb = sqrt(nb2[c]);
+  s.normr = sqrt(nr2[c]);
+  s.tolr = absolute ? tol : tol * s.normb;
+  s.scale = 0.0;
+  s.iters = 0;
+  s.icur = 0;
+  s.active = 0;
+  s.status = 0;
+  x_zero_flag[c] = 0.0;
+  if (s.normb < eps) {  // zero right-hand side: x = 0 (fgmres.hpp:321-328)
+    s.status = 4;
+    x_zero_flag[c] = 1.0;
+  } else if (s.normr < eps) {  // initial guess already exact (fgmres.hpp:341-348)
+    s.status = 4;
+  } else {
+    s.active = 1;
+  }
+  st[c] = s;
+}
+
+// x[r, c] = 0 where flag[c] != 0
+__global__ void k_zero_cols(int64_t n, int nrhs, const double* __restrict__ flag, double* __restrict__ x) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n * nrhs) return;
+  if (flag[g % nrhs] != 0.0) x[g] = 0.0;
+}
+
+// Start of a restart cycle (fgmres.hpp:382-386): rs[0] = normr, v0 /= normr.
+__global__ void k_cycle_start(int nrhs, int m, KState* __restrict__ st, double* __restrict__ rs) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  KState s = st[c];
+  if (s.status == 0) {
+    s.active = 1;
+    s.icur = 0;
+    rs[(int64_t)c * (m + 1)] = s.normr;
+    s.scale = 1.0 / s.normr;
+  } else {
+    s.active = 0;
+    s.scale = 0.0;
+  }
+  st[c] = s;
+}
+
+// Hessenberg column + Givens update for inner step i (1-based), fgmres.hpp:424-461.
+// hcol[j*nrhs + c], j <= i-1 : orthogonalization coefficients; t2[c] = ||w||^2.
+__global__ void k_givens(int nrhs, int m, int i, int maxits, const double* __restrict__ hcol,
+                         const double* __restrict__ t2, double* __restrict__ H, double* __restrict__ cs,
+                         double* __restrict__ sn, double* __restrict__ rs, KState* __restrict__ st) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  KState s = st[c];
+  if (!s.active) {
+    s.scale = 0.0;
+    st[c] = s;
+    return;
+  }
+  const double eps = 2.220446049250313e-16;
+  double* Hc = H + (int64_t)c * (m + 1) * m;
+  double* csc = cs + (int64_t)c * m;
+  double* snc = sn + (int64_t)c * m;
+  double* rsc = rs + (int64_t)c * (m + 1);
+  const int col = i - 1;
+  for (int j = 0; j < i; ++j) Hc[j + col * (m + 1)] = hcol[j * nrhs + c];
+  const double t = sqrt(t2[c]);
+  Hc[i + col * (m + 1)] = t;
+  s.iters += 1;
+  s.icur = i;
+  for (int j = 1; j < i; ++j) {
+    double hii = Hc[(j - 1) + col * (m + 1)];
+    double hj = Hc[j + col * (m + 1)];
+    Hc[(j - 1) + col * (m + 1)] = csc[j - 1] * hii + snc[j - 1] * hj;
+    Hc[j + col * (m + 1)] = -snc[j - 1] * hii + csc[j - 1] * hj;
+  }
+  double hii = Hc[(i - 1) + col * (m + 1)];
+  double hii1 = Hc[i + col * (m + 1)];
+  double gam = sqrt(hii * hii + hii1 * hii1);
+  if (fabs(gam) < eps) {  // breakdown: the reference leaves x untouched (fgmres.hpp:440-443)
+    s.status = 2;
+    s.active = 0;
+    s.scale = 0.0;
+    st[c] = s;
+    return;
+  }
+  csc[i - 1] = hii / gam;
+  snc[i - 1] = hii1 / gam;
+  rsc[i] = -snc[i - 1] * rsc[i - 1];
+  rsc[i - 1] = csc[i - 1] * rsc[i - 1];
+  Hc[(i - 1) + col * (m + 1)] = csc[i - 1] * hii + snc[i - 1] * hii1;
+  s.normr = fabs(rsc[i]);
+  s.scale = (t > 0.0) ? 1.0 / t : 0.0;
+  if (s.normr <= s.tolr) {
+    s.status = 1;
+    s.active = 0;
+  } else if (i >= m || s.iters >= maxits) {
+    s.active = 0;  // cycle ends for this column
+  }
+  st[c] = s;
+}
+
+// Back substitution on the triangularized H (fgmres.hpp:471-480); coef[j*nrhs+c].
+__global__ void k_backsolve(int nrhs, int m, const double* __restrict__ H, double* __restrict__ rs,
+                            const KState* __restrict__ st, double* __restrict__ coef, int* __restrict__ upd) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  KState s = st[c];
+  for (int j = 0; j < m; ++j) coef[j * nrhs + c] = 0.0;
+  upd[c] = 0;
+  if (s.status == 2 || s.status == 4 || s.icur == 0) return;
+  const double* Hc = H + (int64_t)c * (m + 1) * m;
+  double* rsc = rs + (int64_t)c * (m + 1);
+  const int i = s.icur;
+  rsc[i - 1] /= Hc[(i - 1) + (i - 1) * (m + 1)];
+  for (int k = i - 2; k >= 0; --k) {
+    for (int j = k + 1; j < i; ++j) rsc[k] -= Hc[k + j * (m + 1)] * rsc[j];
+    rsc[k] /= Hc[k + k * (m + 1)];
+  }
+  for (int j = 0; j < i; ++j) coef[j * nrhs + c] = rsc[j];
+  upd[c] = 1;
+}
+
+// x[r, c] += sum_j coef[j, c] * Z_j[r, c] for columns with upd[c]
+// (dynamic shared memory: p*nrhs doubles)
+template <typename VZ>
+__global__ void k_update_x(int64_t n, int nrhs, int p, const VZ* __restrict__ Z, int64_t ldz,
+                           const double* __restrict__ coef, const int* __restrict__ upd, double* __restrict__ x) {
+  extern __shared__ double cs[];
+  for (int t = threadIdx.x; t < p * nrhs; t += blockDim.x) cs[t] = coef[t];
+  __syncthreads();
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n * nrhs) return;
+  const int c = (int)(g % nrhs);
+  if (!upd[c]) return;
+  double acc = x[g];
+  for (int j = 0; j < p; ++j) acc += cs[j * nrhs + c] * (double)Z[j * ldz + g];
+  x[g] = acc;
+}
+
+// Restart bookkeeping: any column still running?  flag[0] = number of running columns.
+__global__ void k_count_running(int nrhs, const KState* __restrict__ st, int maxits, int* __restrict__ flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int cnt = 0;
+  for (int c = 0; c < nrhs; ++c)
+    if (st[c].status == 0 && st[c].iters < maxits) ++cnt;
+  flag[0] = cnt;
+}
+
+__global__ void k_count_active(int nrhs, const KState* __restrict__ st, int* __restrict__ flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int cnt = 0;
+  for (int c = 0; c < nrhs; ++c) cnt += st[c].active ? 1 : 0;
+  flag[0] = cnt;
+}
+
+// copy v (double) into z (VZ) : identity preconditioner
+template <typename VZ>
+__global__ void k_copy_conv(int64_t m, const double* __restrict__ in, VZ* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) out[t] = (VZ)in[t];
+}
+
+}  // namespace dev
+}  // namespace pgs

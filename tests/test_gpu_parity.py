@@ -1,0 +1,146 @@
+"""Parity of the HIP solve path (through the C ABI) with the oracle."""
+import numpy as np
+import pytest
+
+import pargemslr_amd as pg
+from oracle import oracle as orc
+from tests import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(x, dev):
+    return pg.DeviceArray.from_numpy(np.ascontiguousarray(x, dtype=np.float64), device=dev)
+
+
+def _host(t):
+    pg.synchronize(t.device)
+    return t.numpy()
+
+
+@pytest.fixture(scope="module")
+def small(gpu):
+    A, G = P.build(P.C2_SMALL, alphas=(0.05, -0.02, 0.01))
+    D = pg.DevOp(G)
+    W = pg.Workspace(D, max_nrhs=8, kdim=50)
+    return A, G, D, W
+
+
+@pytest.fixture(scope="module")
+def c1(gpu):
+    A, G = P.build(P.C1)
+    D = pg.DevOp(G)
+    W = pg.Workspace(D, max_nrhs=4, kdim=30)
+    return A, G, D, W
+
+
+def test_spmv_parity(small, gpu):
+    A, G, D, W = small
+    x = pg.rng_normal(1, A.n)
+    y0 = pg.rng_normal(2, A.n)
+    ref = orc.csr_matvec(A.ia, A.ja, A.a, x, alpha=-0.5, beta=2.0, y=y0)
+    xd, yd = _dev(x, gpu), _dev(y0, gpu)
+    W.spmv(xd, yd, alpha=-0.5, beta=2.0)
+    y = _host(yd)
+    assert np.linalg.norm(y - ref) <= 1e-14 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("which", ["c1", "small"])
+def test_apply_fp64_parity(which, request, gpu):
+    A, G, D, W = request.getfixturevalue(which)
+    O = P.oracle_of(G)
+    for seed in (3, 4):
+        v = pg.rng_normal(seed, A.n)
+        ref = O.apply(v)
+        zd = _dev(np.zeros(A.n), gpu)
+        W.apply(_dev(v, gpu), zd, prec=pg.PGS_PREC_FP64)
+        z = _host(zd)
+        assert np.linalg.norm(z - ref) <= 1e-12 * np.linalg.norm(ref), (which, seed)
+
+
+def test_apply_fp32_close(small, gpu):
+    A, G, D, W = small
+    v = pg.rng_normal(5, A.n)
+    ref = P.oracle_of(G).apply(v)
+    zd = _dev(np.zeros(A.n), gpu)
+    W.apply(_dev(v, gpu), zd, prec=pg.PGS_PREC_FP32)
+    z = _host(zd)
+    assert np.linalg.norm(z - ref) <= 1e-4 * np.linalg.norm(ref)
+
+
+def test_apply_multi_rhs_matches_columns(small, gpu):
+    A, G, D, W = small
+    nr = 8
+    V = np.stack([pg.rng_normal(10 + c, A.n) for c in range(nr)], axis=1)
+    Zd = _dev(np.zeros((A.n, nr)), gpu)
+    W.apply(_dev(V, gpu), Zd, nrhs=nr, prec=pg.PGS_PREC_FP64)
+    Z = _host(Zd)
+    O = P.oracle_of(G)
+    for c in range(nr):
+        ref = O.apply(V[:, c])
+        assert np.linalg.norm(Z[:, c] - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("which,orth", [("c1", pg.PGS_ORTH_MGS), ("c1", pg.PGS_ORTH_CGS2),
+                                        ("small", pg.PGS_ORTH_CGS2)])
+def test_fgmres_fp64_parity(which, orth, request, gpu):
+    A, G, D, W = request.getfixturevalue(which)
+    kdim = 30 if which == "c1" else 50
+    xs = pg.rng_uniform(0, A.n)
+    b = A.matvec(xs)
+    xo, st, it_o, rr_o, _ = orc.fgmres(A, b, prec=P.oracle_of(G), kdim=kdim, maxits=500, tol=1e-8, orth=0)
+    assert st == 0
+    xd = _dev(np.zeros(A.n), gpu)
+    (status, it, rr), = W.fgmres(_dev(b, gpu), xd, kdim=kdim, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP64, orth=orth)
+    x = _host(xd)
+    assert status == 0
+    assert abs(it - it_o) <= 1, (it, it_o)
+    true = np.linalg.norm(b - A.matvec(x)) / np.linalg.norm(b)
+    assert true <= 1e-8
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_fgmres_mixed_precision(small, gpu):
+    A, G, D, W = small
+    b = pg.rng_normal(21, A.n)
+    xo, st, it_o, _, _ = orc.fgmres(A, b, prec=P.oracle_of(G), kdim=50, maxits=500, tol=1e-10)
+    xd = _dev(np.zeros(A.n), gpu)
+    (status, it, rr), = W.fgmres(_dev(b, gpu), xd, kdim=50, maxits=500, tol=1e-10, prec=pg.PGS_PREC_FP32)
+    x = _host(xd)
+    assert status == 0
+    assert np.linalg.norm(b - A.matvec(x)) <= 1e-10 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xo) <= 1e-7 * np.linalg.norm(xo)
+    assert it <= it_o + 3
+
+
+def test_fgmres_multi_rhs_columns_match_single(small, gpu):
+    A, G, D, W = small
+    nr = 4
+    B = np.stack([pg.rng_normal(30 + c, A.n) for c in range(nr)], axis=1)
+    B[:, 2] = 0.0  # zero column: x = 0, no iterations
+    Xd = _dev(np.zeros((A.n, nr)), gpu)
+    res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=50, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP32)
+    X = _host(Xd)
+    for c in range(nr):
+        st, it, rr = res[c]
+        assert st == 0
+        if c == 2:
+            assert it == 0 and not X[:, c].any()
+            continue
+        x1 = _dev(np.zeros(A.n), gpu)
+        (st1, it1, _), = W.fgmres(_dev(B[:, c], gpu), x1, kdim=50, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP32)
+        assert abs(it1 - it) <= 1
+        xs1 = _host(x1)
+        assert np.linalg.norm(B[:, c] - A.matvec(X[:, c])) <= 1e-8 * np.linalg.norm(B[:, c])
+        assert np.linalg.norm(X[:, c] - xs1) <= 1e-6 * np.linalg.norm(xs1)
+
+
+def test_fgmres_unpreconditioned_matches_oracle(c1, gpu):
+    A, G, D, W = c1
+    b = pg.rng_normal(41, A.n)
+    xo, st, it_o, rr_o, _ = orc.fgmres(A, b, prec=None, kdim=30, maxits=60, tol=1e-8, orth=0)
+    xd = _dev(np.zeros(A.n), gpu)
+    (status, it, rr), = W.fgmres(_dev(b, gpu), xd, kdim=30, maxits=60, tol=1e-8, use_precond=False,
+                                 orth=pg.PGS_ORTH_MGS)
+    assert it == it_o == 60 and status == pg.capi.PGS_ERR_NOT_CONVERGED
+    assert rr == pytest.approx(rr_o, rel=1e-6)
