@@ -97,10 +97,14 @@ void upload_csr(DevCsr& d, const Csr<double>& A, int64_t* counter) {
   d.lpr = avg <= 2 ? 2 : avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
 }
 
+// Pipelined triangular-solve geometry: 512 threads, 4 lanes per row, 5 slots
+// per lane -> steps of at most 128 rows and 20 slots.
+constexpr int kTrsvPipeLPR = 4, kTrsvPipeEPL = 5;
+
 struct DevTrsv {
-  int nblocks = 0, nloc = 0, max_rows = 0, max_sets = 0;
+  int nblocks = 0, nloc = 0, max_rows = 0, max_steps = 0, max_width = 0;
   int64_t nnz_padded = 0, nnz_real = 0, nrows_total = 0;
-  DevMem row0, setL, setU, desc, rows, col, val, dinv;
+  DevMem row0, step, stepU, ent0, ri0, desc, ent, ri;
 };
 
 template <typename V>
@@ -108,18 +112,32 @@ void upload_trsv(DevTrsv& d, const TrsvPack<double>& P, int64_t* counter) {
   d.nblocks = P.nblocks;
   d.nloc = P.nloc;
   d.max_rows = P.max_block_rows;
-  d.max_sets = P.max_sets;
+  d.max_steps = P.max_steps;
+  d.max_width = P.max_width;
   d.nnz_padded = (int64_t)P.col.size();
   d.nnz_real = P.nnz_real;
   d.nrows_total = (int64_t)P.rows.size();
   upload(d.row0, P.blk_row0, counter);
-  upload(d.setL, P.blk_setL, counter);
-  upload(d.setU, P.blk_setU, counter);
+  upload(d.step, P.blk_step, counter);
+  upload(d.stepU, P.blk_stepU, counter);
+  upload(d.ent0, P.blk_ent0, counter);
+  upload(d.ri0, P.blk_ri0, counter);
   upload(d.desc, P.desc, counter);
-  upload(d.rows, P.rows, counter);
-  upload(d.col, P.col, counter);
-  upload(d.val, convert_vec<V>(P.val), counter);
-  upload(d.dinv, convert_vec<V>(P.dinv), counter);
+  std::vector<TPair<V>> ent(P.col.size() + 1), ri(P.rows.size() + 1);
+  for (size_t i = 0; i < P.col.size(); ++i) {
+    ent[i] = TPair<V>();
+    ent[i].col = P.col[i];
+    ent[i].val = (V)P.val[i];
+  }
+  for (size_t i = 0; i < P.rows.size(); ++i) {
+    ri[i] = TPair<V>();
+    ri[i].col = P.rows[i];
+    ri[i].val = (V)P.dinv[i];
+  }
+  ent.back() = TPair<V>();
+  ri.back() = TPair<V>();
+  upload(d.ent, ent, counter);
+  upload(d.ri, ri, counter);
 }
 
 struct DevLevel {
@@ -143,6 +161,8 @@ struct pgs_dev_op_s {
   int nlev = 0;
   int is_complex = 0;
   int lds_max = 65536;
+  int trsv_variant = 0;  // 0: pipelined per-(block, column) kernel; 1: basic level-set kernel
+  int trsv_nt = 128;     // threads of the pipelined kernel (steps hold at most trsv_nt / 4 rows)
   int64_t bytes = 0;
   DevCsr A;  // FP64
   DevMem perm;
@@ -256,19 +276,37 @@ void launch_trsv(pgs_dev_op_s* op, pgs_ws_s* ws, int level, const DevTrsv& T, co
   }
 }
 
+template <typename V, bool XLDS, int NT>
+void launch_trsv_pipe_nt(const DevTrsv& T, const V* rhs, V* x, int nrhs, size_t lds, hipStream_t s) {
+  constexpr int D = sizeof(V) == 4 ? 8 : 6;
+  auto kern = k_trsv_pipe<V, XLDS, NT, kTrsvPipeLPR, kTrsvPipeEPL, D>;
+  if (XLDS && lds > 65536)
+    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const unsigned grid = (unsigned)T.nblocks * (unsigned)nrhs;
+  kern<<<grid, NT, XLDS ? lds : 0, s>>>(T.nblocks, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
+                                        T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), T.desc.as<int4>(),
+                                        T.ent.as<TPair<V>>(), T.ri.as<TPair<V>>(), rhs, x, nrhs);
+}
+
+template <typename V, bool XLDS>
+void launch_trsv_pipe(int nt, const DevTrsv& T, const V* rhs, V* x, int nrhs, size_t lds, hipStream_t s) {
+  if (nt == 64) launch_trsv_pipe_nt<V, XLDS, 64>(T, rhs, x, nrhs, lds, s);
+  else if (nt == 256) launch_trsv_pipe_nt<V, XLDS, 256>(T, rhs, x, nrhs, lds, s);
+  else launch_trsv_pipe_nt<V, XLDS, 128>(T, rhs, x, nrhs, lds, s);
+}
+
 template <typename V>
 void launch_trsv_impl(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
   if (T.nblocks == 0) return;
   const size_t lds = (size_t)T.max_rows * sizeof(V);
-  if (nrhs == 1 && lds <= (size_t)op->lds_max) {
-    auto kern = k_trsv_lds<V, kTrsvNT, 4>;
-    if (lds > 65536) HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    kern<<<T.nblocks, kTrsvNT, lds, s>>>(T.row0.as<int>(), T.setL.as<int>(), T.setU.as<int>(), T.desc.as<int4>(),
-                                         T.rows.as<int>(), T.col.as<int>(), T.val.as<V>(), T.dinv.as<V>(), rhs, x);
+  if (op->trsv_variant == 0 && T.max_width <= kTrsvPipeLPR * kTrsvPipeEPL) {
+    if (lds <= (size_t)op->lds_max) launch_trsv_pipe<V, true>(op->trsv_nt, T, rhs, x, nrhs, lds, s);
+    else launch_trsv_pipe<V, false>(op->trsv_nt, T, rhs, x, nrhs, lds, s);
   } else {
-    k_trsv_glb<V, kTrsvNT><<<T.nblocks, kTrsvNT, 0, s>>>(T.row0.as<int>(), T.setL.as<int>(), T.setU.as<int>(),
-                                                         T.desc.as<int4>(), T.rows.as<int>(), T.col.as<int>(),
-                                                         T.val.as<V>(), T.dinv.as<V>(), rhs, x, nrhs);
+    k_trsv_direct<V, kTrsvNT><<<T.nblocks, kTrsvNT, 0, s>>>(T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
+                                                            T.ent0.as<int64_t>(), T.ri0.as<int64_t>(),
+                                                            T.desc.as<int4>(), T.ent.as<TPair<V>>(),
+                                                            T.ri.as<TPair<V>>(), rhs, x, nrhs);
   }
   HIPCHK(hipGetLastError());
 }
@@ -364,6 +402,11 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     op->lds_max = lds;
+    if (const char* v = std::getenv("PGS_TRSV_VARIANT")) op->trsv_variant = std::atoi(v);
+    if (const char* v = std::getenv("PGS_TRSV_NT")) {
+      const int nt = std::atoi(v);
+      if (nt == 64 || nt == 128 || nt == 256) op->trsv_nt = nt;
+    }
     const HostOp<double>& H = *h->op_d;
     op->n = H.n;
     op->nlev = H.nlev;
@@ -380,7 +423,7 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
         L->lev_start = H.lev_ptr[l];
         L->nloc = H.nloc(l);
         L->nrem = H.nrem(l);
-        TrsvPack<double> pk = trsv_pack(HL, H.lev_ptr[l], H.nloc(l));
+        TrsvPack<double> pk = trsv_pack(HL, H.lev_ptr[l], H.nloc(l), op->trsv_nt / kTrsvPipeLPR);
         upload_trsv<V>(L->B, pk, &op->bytes);
         if (l + 1 < H.nlev) {
           upload_csr<V>(L->E, HL.E, &op->bytes);

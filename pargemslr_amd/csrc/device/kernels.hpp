@@ -9,6 +9,8 @@
 
 #include <cstdint>
 
+#include "kernels_trsv.hpp"
+
 namespace pgs {
 namespace dev {
 
@@ -89,105 +91,7 @@ __global__ void k_spmvN(int nrows, int nrhs, const int* __restrict__ ia, const i
   else y[t] = cvt<VY>(alpha * acc + beta * (VACC)y[t]);
 }
 
-// ---------------------------------------------------------------------------
-// Block-diagonal ILU solve x = U^{-1} L^{-1} rhs (IluClass::SolveHost,
-// ParGeMSLR/SRC/preconditioners/ilu.cpp:3752-3783): one workgroup per diagonal
-// block, level sets in sequence separated by workgroup barriers.  x is first
-// loaded with rhs, then both sweeps run in place.
-//   desc[4*s] = {row_off, ent_off, nrows, width}
-// LDS variant: single RHS, block vector held in LDS; LPR lanes per row.
-// ---------------------------------------------------------------------------
-template <typename V, int NT, int LPR>
-__global__ __launch_bounds__(NT) void k_trsv_lds(const int* __restrict__ blk_row0, const int* __restrict__ setL,
-                                                 const int* __restrict__ setU, const int4* __restrict__ desc,
-                                                 const int* __restrict__ rows, const int* __restrict__ col,
-                                                 const V* __restrict__ val, const V* __restrict__ dinv,
-                                                 const V* __restrict__ rhs, V* __restrict__ x) {
-  extern __shared__ unsigned char smem_raw[];
-  V* xs = reinterpret_cast<V*>(smem_raw);
-  const int b = blockIdx.x;
-  const int r0 = blk_row0[b], nb = blk_row0[b + 1] - r0;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < nb; i += NT) xs[i] = rhs[r0 + i];
-  __syncthreads();
-  const int q = tid % LPR;
-  const int t0 = tid / LPR;
-  constexpr int RPP = NT / LPR;  // rows per pass
-  // L sweep
-  const int sL0 = setL[b], sL1 = setU[b], sU1 = setL[b + 1];
-  for (int s = sL0; s < sL1; ++s) {
-    const int4 d = desc[s];
-    for (int t = t0; t < d.z; t += RPP) {
-      V acc = 0;
-      for (int j = q; j < d.w; j += LPR) {
-        const int e = d.y + j * d.z + t;
-        acc += val[e] * xs[col[e]];
-      }
-#pragma unroll
-      for (int m = LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, LPR);
-      if (q == 0) {
-        const int row = rows[d.x + t];
-        xs[row] = xs[row] - acc;
-      }
-    }
-    __syncthreads();
-  }
-  // U sweep
-  for (int s = sL1; s < sU1; ++s) {
-    const int4 d = desc[s];
-    for (int t = t0; t < d.z; t += RPP) {
-      V acc = 0;
-      for (int j = q; j < d.w; j += LPR) {
-        const int e = d.y + j * d.z + t;
-        acc += val[e] * xs[col[e]];
-      }
-#pragma unroll
-      for (int m = LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, LPR);
-      if (q == 0) {
-        const int row = rows[d.x + t];
-        xs[row] = (xs[row] - acc) * dinv[d.x + t];
-      }
-    }
-    __syncthreads();
-  }
-  for (int i = tid; i < nb; i += NT) x[r0 + i] = xs[i];
-}
-
-// Global-memory variant: any block size, nrhs columns (row-interleaved); one
-// thread per (row of the set, column).
-template <typename V, int NT>
-__global__ __launch_bounds__(NT) void k_trsv_glb(const int* __restrict__ blk_row0, const int* __restrict__ setL,
-                                                 const int* __restrict__ setU, const int4* __restrict__ desc,
-                                                 const int* __restrict__ rows, const int* __restrict__ col,
-                                                 const V* __restrict__ val, const V* __restrict__ dinv,
-                                                 const V* __restrict__ rhs, V* __restrict__ x, int nrhs) {
-  const int b = blockIdx.x;
-  const int r0 = blk_row0[b], nb = blk_row0[b + 1] - r0;
-  const int tid = threadIdx.x;
-  const V* rb = rhs + (int64_t)r0 * nrhs;
-  V* xb = x + (int64_t)r0 * nrhs;
-  for (int i = tid; i < nb * nrhs; i += NT) xb[i] = rb[i];
-  __syncthreads();
-  const int sL0 = setL[b], sL1 = setU[b], sU1 = setL[b + 1];
-  for (int s = sL0; s < sU1; ++s) {
-    const int4 d = desc[s];
-    const bool upper = (s >= sL1);
-    const int items = d.z * nrhs;
-    for (int it = tid; it < items; it += NT) {
-      const int t = it / nrhs;
-      const int c = it - t * nrhs;
-      const int row = rows[d.x + t];
-      V acc = xb[(int64_t)row * nrhs + c];
-      for (int j = 0; j < d.w; ++j) {
-        const int e = d.y + j * d.z + t;
-        acc -= val[e] * xb[(int64_t)col[e] * nrhs + c];
-      }
-      if (upper) acc *= dinv[d.x + t];
-      xb[(int64_t)row * nrhs + c] = acc;
-    }
-    __syncthreads();
-  }
-}
+// (block-diagonal ILU solves: kernels_trsv.hpp)
 
 // ---------------------------------------------------------------------------
 // Low-rank correction z += WH (W^H z) (GemslrClass::SolveApplyLowRankLevel,

@@ -1,0 +1,233 @@
+// Block-diagonal ILU solve x = U^{-1} L^{-1} rhs for one GeMSLR level
+// (IluClass::SolveHost, ParGeMSLR/SRC/preconditioners/ilu.cpp:3752-3783):
+// L unit lower (strict part stored), U strictly upper, Dinv = 1/pivots.
+//
+// Layout (built on the host by trsv_pack): per diagonal block a sequence of
+// steps (level sets, L sweep then U sweep); step s is
+//   desc[s] = {row_off, ent_off, nrows, width}   (block-relative offsets)
+//   ent[ent_off + j * nrows + t] = (column, value) of slot j of row t
+//   ri[row_off + t]              = (row, 1/pivot)   (1/pivot used on U steps)
+// Padded slots hold (0, 0).  x is first loaded with rhs and both sweeps run in
+// place, so padded slots only ever read finite values.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pgs {
+namespace dev {
+
+template <typename V>
+struct alignas(8) TPair;
+template <>
+struct alignas(8) TPair<float> {
+  int col;
+  float val;
+};
+template <>
+struct alignas(16) TPair<double> {
+  int col;
+  int pad;
+  double val;
+};
+
+constexpr int kBufferRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit dwords, no swizzle
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufferRsrcWord3);
+}
+
+// Buffer loads: reads outside [0, bytes) of the resource return zeros.
+__device__ __forceinline__ int4 bload_i4(__amdgpu_buffer_rsrc_t r, int off) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+}
+__device__ __forceinline__ TPair<float> bload_pair(__amdgpu_buffer_rsrc_t r, int off, float) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  TPair<float> p;
+  p.col = (int)v[0];
+  p.val = __uint_as_float(v[1]);
+  return p;
+}
+__device__ __forceinline__ TPair<double> bload_pair(__amdgpu_buffer_rsrc_t r, int off, double) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  TPair<double> p;
+  p.col = (int)v[0];
+  p.pad = 0;
+  p.val = __hiloint2double((int)v[3], (int)v[2]);
+  return p;
+}
+
+// Sum over the 4 lanes of a quad with DPP quad permutations (xor 1, xor 2).
+__device__ __forceinline__ float quad_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+  return v;
+}
+__device__ __forceinline__ double quad_perm_f64(double v, int ctrl_sel) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  if (ctrl_sel == 1) {
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
+  }
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double quad_sum(double v) {
+  v += quad_perm_f64(v, 1);
+  v += quad_perm_f64(v, 2);
+  return v;
+}
+
+// Direct kernel: any width, nrhs columns (row-interleaved); one thread per
+// (row of the step, column), plain loads.  Used for levels whose rows are too
+// wide for the pipelined kernel.
+template <typename V, int NT>
+__global__ __launch_bounds__(NT) void k_trsv_direct(const int* __restrict__ blk_row0, const int* __restrict__ blk_step,
+                                                    const int* __restrict__ blk_stepU,
+                                                    const int64_t* __restrict__ blk_ent0,
+                                                    const int64_t* __restrict__ blk_ri0, const int4* __restrict__ desc,
+                                                    const TPair<V>* __restrict__ ent, const TPair<V>* __restrict__ ri,
+                                                    const V* __restrict__ rhs, V* __restrict__ x, int nrhs) {
+  const int b = blockIdx.x;
+  const int r0 = blk_row0[b], nb = blk_row0[b + 1] - r0;
+  const int tid = threadIdx.x;
+  const V* rb = rhs + (int64_t)r0 * nrhs;
+  V* xb = x + (int64_t)r0 * nrhs;
+  const TPair<V>* eb = ent + blk_ent0[b];
+  const TPair<V>* rib = ri + blk_ri0[b];
+  for (int i = tid; i < nb * nrhs; i += NT) xb[i] = rb[i];
+  __syncthreads();
+  const int s0 = blk_step[b], sU = blk_stepU[b], s1 = blk_step[b + 1];
+  for (int s = s0; s < s1; ++s) {
+    const int4 d = desc[s];
+    const bool upper = (s >= sU);
+    const int items = d.z * nrhs;
+    for (int it = tid; it < items; it += NT) {
+      const int t = it / nrhs;
+      const int c = it - t * nrhs;
+      const TPair<V> rr = rib[d.x + t];
+      V acc = xb[(int64_t)rr.col * nrhs + c];
+      for (int j = 0; j < d.w; ++j) {
+        const TPair<V> p = eb[d.y + j * d.z + t];
+        acc -= p.val * xb[(int64_t)p.col * nrhs + c];
+      }
+      if (upper) acc *= rr.val;
+      xb[(int64_t)rr.col * nrhs + c] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Pipelined kernel: one workgroup per (diagonal block, RHS column); grid
+// index = column * nblocks + block, so all columns of a block are dispatched
+// to the same XCD (round-robin over 8 XCDs when nblocks % 8 == 0) and share
+// its L2 copy of the factors.  A step is solved by LPR lanes per row with EPL
+// slots per lane (steps are at most NT/LPR rows and LPR*EPL slots wide).  The
+// slots, row pairs and descriptors of the next D steps are in flight while
+// the current step runs; every load is unconditional (buffer loads, indices
+// clamped into the step) so the hardware wait counters advance in order and
+// only the data of the current step is waited for.
+// XLDS: the block's column of x lives in LDS (else in global memory).
+template <typename V, bool XLDS, int NT, int LPR, int EPL, int D>
+__global__ __launch_bounds__(NT) void k_trsv_pipe(int nblocks, const int* __restrict__ blk_row0,
+                                                  const int* __restrict__ blk_step, const int* __restrict__ blk_stepU,
+                                                  const int64_t* __restrict__ blk_ent0,
+                                                  const int64_t* __restrict__ blk_ri0, const int4* __restrict__ desc,
+                                                  const TPair<V>* __restrict__ ent, const TPair<V>* __restrict__ ri,
+                                                  const V* __restrict__ rhs, V* __restrict__ x, int nrhs) {
+  extern __shared__ unsigned char smem_raw[];
+  const int b = blockIdx.x % nblocks;
+  const int c = blockIdx.x / nblocks;
+  const int r0 = blk_row0[b], nb = blk_row0[b + 1] - r0;
+  const int sbeg = blk_step[b], send = blk_step[b + 1], sU = blk_stepU[b];
+  const int64_t e0 = blk_ent0[b], e1 = blk_ent0[b + 1];
+  const int64_t q0 = blk_ri0[b], q1 = blk_ri0[b + 1];
+  const __amdgpu_buffer_rsrc_t rs_ent = make_rsrc(ent + e0, (e1 - e0) * (int64_t)sizeof(TPair<V>));
+  const __amdgpu_buffer_rsrc_t rs_ri = make_rsrc(ri + q0, (q1 - q0) * (int64_t)sizeof(TPair<V>));
+  const __amdgpu_buffer_rsrc_t rs_desc = make_rsrc(desc + sbeg, (int64_t)(send - sbeg) * 16);
+  const int tid = threadIdx.x;
+  const int q = tid % LPR;
+  const int t0 = tid / LPR;
+  V* xg = x + (int64_t)r0 * nrhs + c;
+  V* xs = XLDS ? reinterpret_cast<V*>(smem_raw) : nullptr;
+
+  TPair<V> pe[D][EPL];
+  TPair<V> pr[D];
+  int4 dd[D];  // descriptor of the step held by stage k
+  int4 dn[D];  // descriptor of the step stage k loads next
+  constexpr int PS = (int)sizeof(TPair<V>);
+
+#define PGS_LOAD_STAGE(K, DESC)                                                 \
+  do {                                                                          \
+    const int4 d_ = (DESC);                                                     \
+    dd[K] = d_;                                                                 \
+    const int tt_ = min(t0, max(d_.z - 1, 0));                                  \
+    pr[K] = bload_pair(rs_ri, (d_.x + tt_) * PS, V(0));                         \
+    _Pragma("unroll") for (int e_ = 0; e_ < EPL; ++e_) {                        \
+      const int j_ = min(q + e_ * LPR, max(d_.w - 1, 0));                       \
+      pe[K][e_] = bload_pair(rs_ent, (d_.y + j_ * d_.z + tt_) * PS, V(0));      \
+    }                                                                           \
+  } while (0)
+
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    PGS_LOAD_STAGE(k, bload_i4(rs_desc, k * 16));
+    dn[k] = bload_i4(rs_desc, (k + D) * 16);
+  }
+  // block column <- rhs column
+  const V* rb = rhs + (int64_t)r0 * nrhs + c;
+  if (XLDS) {
+    for (int i = tid; i < nb; i += NT) xs[i] = rb[(int64_t)i * nrhs];
+  } else {
+    for (int i = tid; i < nb; i += NT) xg[(int64_t)i * nrhs] = rb[(int64_t)i * nrhs];
+  }
+  __syncthreads();
+  for (int base = sbeg; base < send; base += D) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const int s = base + k;
+      const int4 d = dd[k];
+      // gather first (all slots, columns are always in range), then mask the values
+      const int row = pr[k].col;
+      const V xrow = XLDS ? xs[row] : xg[(int64_t)row * nrhs];
+      V xv[EPL];
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) {
+        const int cj = pe[k][e].col;
+        xv[e] = XLDS ? xs[cj] : xg[(int64_t)cj * nrhs];
+      }
+      V acc = 0;
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) {
+        const V w = (q + e * LPR < d.w) ? pe[k][e].val : (V)0;
+        acc += w * xv[e];
+      }
+      if constexpr (LPR == 4) {
+        acc = quad_sum(acc);
+      } else {
+#pragma unroll
+        for (int m = LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, LPR);
+      }
+      if (s < send && q == 0 && t0 < d.z) {
+        V v = xrow - acc;
+        if (s >= sU) v *= pr[k].val;
+        if (XLDS) xs[row] = v;
+        else xg[(int64_t)row * nrhs] = v;
+      }
+      __syncthreads();
+      PGS_LOAD_STAGE(k, dn[k]);
+      dn[k] = bload_i4(rs_desc, (s - sbeg + 2 * D) * 16);
+    }
+  }
+#undef PGS_LOAD_STAGE
+  if (XLDS) {
+    for (int i = tid; i < nb; i += NT) xg[(int64_t)i * nrhs] = xs[i];
+  }
+}
+
+}  // namespace dev
+}  // namespace pgs

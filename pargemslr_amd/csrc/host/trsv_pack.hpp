@@ -5,11 +5,14 @@
 // (ParGeMSLR/SRC/preconditioners/ilu.cpp:832-1057) and leaves the device solve
 // to cuSPARSE csrsv2 with USE_LEVEL analysis (ilu_cuda.cu:225-459).  Here every
 // diagonal block gets its own level sets (L: level(i) = 1 + max level(j) over
-// L(i,j) != 0; U: the same over U(i,j) != 0 walking from the last row), and each
-// level set is stored as a small column-major ELL slab:
-//   entry (row t of the set, slot j) at ent_off + j * nrows + t,
-// padded with (col 0, value 0).  The device kernel processes one block per
-// workgroup and one level set per step.
+// L(i,j) != 0; U: the same over U(i,j) != 0 walking from the last row).  A
+// level set with more than `max_rows` rows is cut into consecutive steps (rows
+// of one level are independent, so any split is valid).  Each step is stored
+// as a small column-major ELL slab of (column, value) pairs:
+//   entry (row t of the step, slot j) at ent_off + j * nrows + t,
+// padded with (column 0, value 0), plus one (row, 1/pivot) pair per row.
+// Offsets are relative to the block's first entry / first row pair so that a
+// device kernel can address a block with 32-bit offsets.
 #pragma once
 
 #include <cstdint>
@@ -24,18 +27,22 @@ struct TrsvPack {
   int nblocks = 0;
   int nloc = 0;
   int max_block_rows = 0;
-  int max_sets = 0;                  // deepest block (L or U)
+  int max_steps = 0;                 // deepest block, L + U steps
+  int max_width = 0;                 // widest step
   std::vector<int> blk_row0;         // nblocks+1, level-local first row of each block
-  std::vector<int> blk_setL, blk_setU;  // nblocks+1, ranges into desc
-  std::vector<int> desc;             // 4 per set: row_off, ent_off, nrows, width
-  std::vector<int> rows;             // block-local row ids, set by set
-  std::vector<int> col;              // block-local column ids (ELL)
+  std::vector<int> blk_step;         // nblocks+1, step ranges into desc
+  std::vector<int> blk_stepU;        // nblocks, first U step of each block
+  std::vector<int64_t> blk_ent0;     // nblocks+1, first entry of each block
+  std::vector<int64_t> blk_ri0;      // nblocks+1, first row pair of each block
+  std::vector<int> desc;             // 4 per step: row_off, ent_off (block relative), nrows, width
+  std::vector<int> col;              // ELL columns (block-local row ids)
   std::vector<T> val;                // ELL values
-  std::vector<T> dinv;               // aligned with rows; Dinv of the row for U sets, 0 for L sets
+  std::vector<int> rows;             // per step row: block-local row id
+  std::vector<T> dinv;               // per step row: 1/pivot for U steps, 0 for L steps
   int64_t nnz_real = 0;              // entries before padding
 };
 
 template <typename T>
-TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc);
+TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_rows);
 
 }  // namespace pgs
