@@ -99,7 +99,7 @@ void upload_csr(DevCsr& d, const Csr<double>& A, int64_t* counter) {
 
 // Pipelined triangular-solve geometry: 512 threads, 4 lanes per row, 5 slots
 // per lane -> steps of at most 128 rows and 20 slots.
-constexpr int kTrsvPipeLPR = 4, kTrsvPipeEPL = 5;
+constexpr int kTrsvMaxWidth = 20;  // widest factor row the pipelined kernel takes
 
 struct DevTrsv {
   int nblocks = 0, nloc = 0, max_rows = 0, max_steps = 0, max_width = 0;
@@ -162,7 +162,8 @@ struct pgs_dev_op_s {
   int is_complex = 0;
   int lds_max = 65536;
   int trsv_variant = 0;  // 0: pipelined per-(block, column) kernel; 1: basic level-set kernel
-  int trsv_nt = 128;     // threads of the pipelined kernel (steps hold at most trsv_nt / 4 rows)
+  int trsv_nt = 128;     // threads of the pipelined kernel
+  int trsv_lpr = 4;      // lanes per row (steps hold at most trsv_nt / trsv_lpr rows)
   int64_t bytes = 0;
   DevCsr A;  // FP64
   DevMem perm;
@@ -276,10 +277,13 @@ void launch_trsv(pgs_dev_op_s* op, pgs_ws_s* ws, int level, const DevTrsv& T, co
   }
 }
 
-template <typename V, bool XLDS, int NT>
+template <typename V, bool XLDS, int NT, int LPR>
 void launch_trsv_pipe_nt(const DevTrsv& T, const V* rhs, V* x, int nrhs, size_t lds, hipStream_t s) {
-  constexpr int D = sizeof(V) == 4 ? 8 : 6;
-  auto kern = k_trsv_pipe<V, XLDS, NT, kTrsvPipeLPR, kTrsvPipeEPL, D>;
+  // slots per lane so that LPR * EPL covers rows of 20 entries; stage depth
+  // keeps the in-flight vector loads (EPL + 2 per stage) under 64
+  constexpr int EPL = 20 / LPR;
+  constexpr int D = LPR == 4 ? (sizeof(V) == 4 ? 8 : 6) : (sizeof(V) == 4 ? 5 : 4);
+  auto kern = k_trsv_pipe<V, XLDS, NT, LPR, EPL, D>;
   if (XLDS && lds > 65536)
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const unsigned grid = (unsigned)T.nblocks * (unsigned)nrhs;
@@ -289,19 +293,23 @@ void launch_trsv_pipe_nt(const DevTrsv& T, const V* rhs, V* x, int nrhs, size_t 
 }
 
 template <typename V, bool XLDS>
-void launch_trsv_pipe(int nt, const DevTrsv& T, const V* rhs, V* x, int nrhs, size_t lds, hipStream_t s) {
-  if (nt == 64) launch_trsv_pipe_nt<V, XLDS, 64>(T, rhs, x, nrhs, lds, s);
-  else if (nt == 256) launch_trsv_pipe_nt<V, XLDS, 256>(T, rhs, x, nrhs, lds, s);
-  else launch_trsv_pipe_nt<V, XLDS, 128>(T, rhs, x, nrhs, lds, s);
+void launch_trsv_pipe(int nt, int lpr, const DevTrsv& T, const V* rhs, V* x, int nrhs, size_t lds, hipStream_t s) {
+  if (lpr == 2) {
+    if (nt == 64) launch_trsv_pipe_nt<V, XLDS, 64, 2>(T, rhs, x, nrhs, lds, s);
+    else launch_trsv_pipe_nt<V, XLDS, 128, 2>(T, rhs, x, nrhs, lds, s);
+  } else {
+    if (nt == 256) launch_trsv_pipe_nt<V, XLDS, 256, 4>(T, rhs, x, nrhs, lds, s);
+    else launch_trsv_pipe_nt<V, XLDS, 128, 4>(T, rhs, x, nrhs, lds, s);
+  }
 }
 
 template <typename V>
 void launch_trsv_impl(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
   if (T.nblocks == 0) return;
   const size_t lds = (size_t)T.max_rows * sizeof(V);
-  if (op->trsv_variant == 0 && T.max_width <= kTrsvPipeLPR * kTrsvPipeEPL) {
-    if (lds <= (size_t)op->lds_max) launch_trsv_pipe<V, true>(op->trsv_nt, T, rhs, x, nrhs, lds, s);
-    else launch_trsv_pipe<V, false>(op->trsv_nt, T, rhs, x, nrhs, lds, s);
+  if (op->trsv_variant == 0 && T.max_width <= kTrsvMaxWidth) {
+    if (lds <= (size_t)op->lds_max) launch_trsv_pipe<V, true>(op->trsv_nt, op->trsv_lpr, T, rhs, x, nrhs, lds, s);
+    else launch_trsv_pipe<V, false>(op->trsv_nt, op->trsv_lpr, T, rhs, x, nrhs, lds, s);
   } else {
     k_trsv_direct<V, kTrsvNT><<<T.nblocks, kTrsvNT, 0, s>>>(T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
                                                             T.ent0.as<int64_t>(), T.ri0.as<int64_t>(),
@@ -407,6 +415,12 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
       const int nt = std::atoi(v);
       if (nt == 64 || nt == 128 || nt == 256) op->trsv_nt = nt;
     }
+    if (const char* v = std::getenv("PGS_TRSV_LPR")) {
+      const int lpr = std::atoi(v);
+      if (lpr == 2 || lpr == 4) op->trsv_lpr = lpr;
+    }
+    if (op->trsv_lpr == 2 && op->trsv_nt == 256) op->trsv_nt = 128;
+    if (op->trsv_lpr == 4 && op->trsv_nt == 64) op->trsv_nt = 128;
     const HostOp<double>& H = *h->op_d;
     op->n = H.n;
     op->nlev = H.nlev;
@@ -423,7 +437,7 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
         L->lev_start = H.lev_ptr[l];
         L->nloc = H.nloc(l);
         L->nrem = H.nrem(l);
-        TrsvPack<double> pk = trsv_pack(HL, H.lev_ptr[l], H.nloc(l), op->trsv_nt / kTrsvPipeLPR);
+        TrsvPack<double> pk = trsv_pack(HL, H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr);
         upload_trsv<V>(L->B, pk, &op->bytes);
         if (l + 1 < H.nlev) {
           upload_csr<V>(L->E, HL.E, &op->bytes);
@@ -500,7 +514,9 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
     ws->V.alloc((size_t)(kdim + 1) * vb);
     ws->Z.alloc((size_t)kdim * vb);
     ws->v_bytes = (int64_t)vb;
-    const int64_t rows_per_blk = std::max<int64_t>(2048, (n + 1023) / 1024);
+    // dot products use at most 1024 workgroups (four per CU); the partials
+    // are summed by one wave per output
+    const int64_t rows_per_blk = std::max<int64_t>(256, (n + 1023) / 1024);
     ws->part_blocks = (int)((n + rows_per_blk - 1) / rows_per_blk);
     ws->part.alloc((size_t)ws->part_blocks * (kdim + 1) * nr * sizeof(double));
     ws->hbuf.alloc((size_t)(kdim + 1) * nr * sizeof(double));
@@ -585,7 +601,18 @@ void multidot(pgs_ws_s* ws, int64_t n, int nrhs, int p, const VV* V, int64_t ldv
   const int64_t rows_per_blk = (n + ws->part_blocks - 1) / ws->part_blocks;
   const int nblk = (int)((n + rows_per_blk - 1) / rows_per_blk);
   k_multidot<VV, VW, kNT><<<nblk, kNT, 0, s>>>(n, nrhs, p, V, ldv, w, ws->part.as<double>(), rows_per_blk);
-  k_dot_final<<<nblocks_for((int64_t)p * nrhs, kNT), kNT, 0, s>>>(nblk, p, nrhs, ws->part.as<double>(), out, accumulate);
+  k_dot_final_wave<<<(p * nrhs + 3) / 4, 256, 0, s>>>(nblk, p, nrhs, ws->part.as<double>(), out, accumulate);
+  HIPCHK(hipGetLastError());
+}
+
+// w -= V(:, 0:p) h; with want_norm, t2[c] = ||w(:, c)||^2 of the result
+void multi_axpy(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64_t ldv, const double* h, double* w,
+                double* t2, hipStream_t s) {
+  // enough workgroups to keep every CU busy on the streaming update
+  const int nblk = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n * nrhs + kNT - 1) / kNT));
+  const size_t shm = sizeof(double) * ((size_t)p * nrhs + (kNT / 64) * nrhs);
+  k_multi_axpy_norm<kNT><<<nblk, kNT, shm, s>>>(n, nrhs, p, V, ldv, h, w, t2 ? ws->part.as<double>() : nullptr);
+  if (t2) k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, ws->part.as<double>(), t2, 0);
   HIPCHK(hipGetLastError());
 }
 
@@ -645,19 +672,22 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int 
       launch_spmv<double, VZ, double, double>(op->A, zc, w, (const double*)nullptr, 1.0, 0.0, 0, nrhs, s);
       double* h = ws->hbuf.as<double>();
       double* h2 = ws->h2buf.as<double>();
+      double* t2 = ws->t2.as<double>();
       if (o.orth == PGS_ORTH_CGS2) {
+        // PargemslrCgs2 (matrixops.cpp:10588-10709): two projection passes, the
+        // norm of the result accumulated by the second update
         multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h, 0, s);
-        k_multi_axpy<<<nb, kNT, sizeof(double) * i * nrhs, s>>>(n, nrhs, i, V, ldv, h, w);
+        multi_axpy(ws, n, nrhs, i, V, ldv, h, w, nullptr, s);
         multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h2, 0, s);
-        k_multi_axpy<<<nb, kNT, sizeof(double) * i * nrhs, s>>>(n, nrhs, i, V, ldv, h2, w);
-        k_dot_final<<<nblocks_for((int64_t)i * nrhs, kNT), kNT, 0, s>>>(1, i, nrhs, h2, h, 1);
+        multi_axpy(ws, n, nrhs, i, V, ldv, h2, w, t2, s);
+        k_dot_final_wave<<<(i * nrhs + 3) / 4, 256, 0, s>>>(1, i, nrhs, h2, h, 1);
       } else {
+        // PargemslrMgs (matrixops.cpp:10728-10807): one (dot, axpy) pair per basis vector
         for (int j = 0; j < i; ++j) {
           multidot<double, double>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, w, h + (int64_t)j * nrhs, 0, s);
-          k_multi_axpy<<<nb, kNT, sizeof(double) * nrhs, s>>>(n, nrhs, 1, V + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w);
+          multi_axpy(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w, j == i - 1 ? t2 : nullptr, s);
         }
       }
-      multidot<double, double>(ws, n, nrhs, 1, w, ldv, w, ws->t2.as<double>(), 0, s);
       k_givens<<<1, 64, 0, s>>>(nrhs, m, i, maxits, h, ws->t2.as<double>(), H, ws->cs.as<double>(),
                                 ws->sn.as<double>(), rs, st);
       k_scale_cols<<<nb, kNT, 0, s>>>(n, nrhs, st, w);

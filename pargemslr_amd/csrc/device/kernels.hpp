@@ -198,6 +198,59 @@ __global__ void k_dot_final(int nblk, int p, int nrhs, const double* __restrict_
   out[g] = accumulate ? out[g] + s : s;
 }
 
+// out[g] = sum over the nblk partials of output g (one wave per output; the
+// partials are summed in a fixed order, so the result does not depend on
+// timing).  Grid: ceil(p*nrhs / 4) workgroups of 256 threads.
+__global__ __launch_bounds__(256) void k_dot_final_wave(int nblk, int p, int nrhs, const double* __restrict__ part,
+                                                        double* __restrict__ out, int accumulate) {
+  const int g = blockIdx.x * 4 + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int total = p * nrhs;
+  if (g >= total) return;  // whole wave exits together
+  double s = 0;
+  for (int b = lane; b < nblk; b += kWave) s += part[(int64_t)b * total + g];
+#pragma unroll
+  for (int m = kWave / 2; m > 0; m >>= 1) s += __shfl_xor(s, m);
+  if (lane == 0) out[g] = accumulate ? out[g] + s : s;
+}
+
+// w[r, c] -= sum_{j<p} V_j[r, c] * h[j, c]  and, when part != nullptr, the
+// per-workgroup partial sums of w[r, c]^2 (part[blk * nrhs + c]).
+// Dynamic shared memory: p*nrhs + (NT/64)*nrhs doubles.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_multi_axpy_norm(int64_t n, int nrhs, int p, const double* __restrict__ V,
+                                                        int64_t ldv, const double* __restrict__ h,
+                                                        double* __restrict__ w, double* __restrict__ part) {
+  extern __shared__ double hs[];
+  double* red = hs + p * nrhs;
+  for (int t = threadIdx.x; t < p * nrhs; t += NT) hs[t] = h[t];
+  __syncthreads();
+  const int tid = threadIdx.x;
+  const int c = tid % nrhs;
+  const int64_t per_blk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * per_blk;
+  const int64_t i1 = min(n, i0 + per_blk);
+  const int lanes = NT / nrhs;
+  double sq = 0;
+  for (int64_t i = i0 + tid / nrhs; i < i1; i += lanes) {
+    const int64_t g = i * nrhs + c;
+    double acc = w[g];
+    for (int j = 0; j < p; ++j) acc -= V[j * ldv + g] * hs[j * nrhs + c];
+    w[g] = acc;
+    sq += acc * acc;
+  }
+  if (part == nullptr) return;  // uniform
+  for (int m = kWave / 2; m >= nrhs; m >>= 1) sq += __shfl_xor(sq, m);
+  const int wid = tid / kWave, lane = tid % kWave;
+  if (lane < nrhs) red[wid * nrhs + lane] = sq;
+  __syncthreads();
+  if (tid < nrhs) {
+    double s = 0;
+    for (int q = 0; q < NT / kWave; ++q) s += red[q * nrhs + tid];
+    part[(int64_t)blockIdx.x * nrhs + tid] = s;
+  }
+}
+
 // w[r, c] -= sum_{j<p} V_j[r, c] * h[j, c]   (dynamic shared memory: p*nrhs doubles)
 __global__ void k_multi_axpy(int64_t n, int nrhs, int p, const double* __restrict__ V, int64_t ldv,
                              const double* __restrict__ h, double* __restrict__ w) {

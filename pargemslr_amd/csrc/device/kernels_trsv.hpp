@@ -81,6 +81,11 @@ __device__ __forceinline__ double quad_sum(double v) {
   v += quad_perm_f64(v, 2);
   return v;
 }
+// Sum over lane pairs (xor 1).
+__device__ __forceinline__ float pair_sum(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double pair_sum(double v) { return v + quad_perm_f64(v, 1); }
 
 // Direct kernel: any width, nrhs columns (row-interleaved); one thread per
 // (row of the step, column), plain loads.  Used for levels whose rows are too
@@ -208,19 +213,32 @@ __global__ __launch_bounds__(NT) void k_trsv_pipe(int nblocks, const int* __rest
       }
       if constexpr (LPR == 4) {
         acc = quad_sum(acc);
+      } else if constexpr (LPR == 2) {
+        acc = pair_sum(acc);
       } else {
 #pragma unroll
         for (int m = LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, LPR);
       }
+      const V dv = pr[k].val;
       if (s < send && q == 0 && t0 < d.z) {
         V v = xrow - acc;
-        if (s >= sU) v *= pr[k].val;
+        if (s >= sU) v *= dv;
         if (XLDS) xs[row] = v;
         else xg[(int64_t)row * nrhs] = v;
       }
-      __syncthreads();
-      PGS_LOAD_STAGE(k, dn[k]);
-      dn[k] = bload_i4(rs_desc, (s - sbeg + 2 * D) * 16);
+      if constexpr (XLDS) {
+        // the stage registers are free again: start the loads of step s + D
+        // before waiting at the barrier (the barrier only waits for LDS)
+        PGS_LOAD_STAGE(k, dn[k]);
+        dn[k] = bload_i4(rs_desc, (s - sbeg + 2 * D) * 16);
+        __syncthreads();
+      } else {
+        // x lives in global memory: the barrier's release fence waits for all
+        // outstanding vector memory operations, so issue the loads after it
+        __syncthreads();
+        PGS_LOAD_STAGE(k, dn[k]);
+        dn[k] = bload_i4(rs_desc, (s - sbeg + 2 * D) * 16);
+      }
     }
   }
 #undef PGS_LOAD_STAGE
