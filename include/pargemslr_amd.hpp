@@ -1,0 +1,421 @@
+// C++ class surface of pargemslr_amd, header-only over the C ABI of
+// pargemslr_amd.h.  Mirrors the reference classes a sequential (np = 1)
+// driver uses, so that such a driver switches by changing its include and
+// namespace:
+//   SequentialVectorClass  ParGeMSLR/SRC/vectors/sequential_vector.hpp
+//   CsrMatrixClass         ParGeMSLR/SRC/matrices/csr_matrix.hpp:26-716 (MatVec 562-575)
+//   GemslrClass            ParGeMSLR/SRC/preconditioners/gemslr.hpp:1511-3190
+//   IluClass               ParGeMSLR/SRC/preconditioners/ilu.hpp:118-1206
+//   FlexGmresClass         ParGeMSLR/SRC/solvers/fgmres.hpp:27-688
+// with the SolverClass contract (ParGeMSLR/SRC/solvers/solver.hpp:45-534):
+// Setup(x, rhs) / Solve(x, rhs) / SetSolveLocation / SetWithParameterArray,
+// int return codes, non-owning matrix and preconditioner pointers.
+//
+// Values are double (real operators).  A vector lives on the host
+// (kMemoryHost) or on the device (kMemoryDevice); the solver's location
+// decides where the solve runs: kMemoryDevice runs the HIP path,
+// kMemoryHost runs the library's host apply (GemslrClass::Solve only).
+#ifndef PARGEMSLR_AMD_HPP
+#define PARGEMSLR_AMD_HPP
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "pargemslr_amd.h"
+
+namespace pargemslr_amd {
+
+// memory locations (ParGeMSLR/SRC/utils/memory.hpp:21-27)
+enum MemoryLocationEnum { kMemoryHost = 0, kMemoryDevice = 1 };
+
+class SequentialVectorClass {
+ public:
+  SequentialVectorClass() = default;
+  SequentialVectorClass(const SequentialVectorClass&) = delete;
+  SequentialVectorClass& operator=(const SequentialVectorClass&) = delete;
+  ~SequentialVectorClass() { Clear(); }
+
+  int Setup(int n, int location = kMemoryHost, bool setzero = true) {
+    Clear();
+    n_ = n;
+    location_ = location;
+    own_ = true;
+    if (location == kMemoryHost) {
+      host_.assign((size_t)n, 0.0);
+      data_ = host_.data();
+      return 0;
+    }
+    void* p = nullptr;
+    int err = pgs_dev_malloc(device_, (int64_t)n * 8, &p);
+    if (err) return err;
+    data_ = static_cast<double*>(p);
+    return setzero ? pgs_memset_zero(device_, data_, (int64_t)n * 8, nullptr) : 0;
+  }
+  // non-owning view (SequentialVectorClass::SetupPtr, sequential_vector.cpp:337-395)
+  int SetupPtr(double* data, int n, int location) {
+    Clear();
+    data_ = data;
+    n_ = n;
+    location_ = location;
+    own_ = false;
+    return 0;
+  }
+  void Clear() {
+    if (own_ && location_ == kMemoryDevice && data_) pgs_dev_free(device_, data_);
+    host_.clear();
+    data_ = nullptr;
+    n_ = 0;
+    own_ = false;
+  }
+  double* GetData() const { return data_; }
+  int GetLengthLocal() const { return n_; }
+  int GetDataLocation() const { return location_; }
+  int Fill(double v) {
+    if (location_ == kMemoryHost) {
+      for (int i = 0; i < n_; ++i) data_[i] = v;
+      return 0;
+    }
+    std::vector<double> tmp((size_t)n_, v);
+    return pgs_memcpy_htod(device_, data_, tmp.data(), (int64_t)n_ * 8, nullptr);
+  }
+  // copies to / from host arrays regardless of location
+  int CopyFromHost(const double* src) {
+    if (location_ == kMemoryHost) {
+      std::memcpy(data_, src, sizeof(double) * n_);
+      return 0;
+    }
+    return pgs_memcpy_htod(device_, data_, src, (int64_t)n_ * 8, nullptr);
+  }
+  int CopyToHost(double* dst) const {
+    if (location_ == kMemoryHost) {
+      std::memcpy(dst, data_, sizeof(double) * n_);
+      return 0;
+    }
+    return pgs_memcpy_dtoh(device_, dst, data_, (int64_t)n_ * 8, nullptr);
+  }
+  // SequentialVectorClass::MoveData
+  int MoveData(int location) {
+    if (location == location_) return 0;
+    std::vector<double> tmp((size_t)n_);
+    int err = CopyToHost(tmp.data());
+    if (err) return err;
+    const int n = n_;
+    err = Setup(n, location, false);
+    if (err) return err;
+    return CopyFromHost(tmp.data());
+  }
+
+ private:
+  double* data_ = nullptr;
+  int n_ = 0;
+  int location_ = kMemoryHost;
+  int device_ = 0;
+  bool own_ = false;
+  std::vector<double> host_;
+};
+
+class GemslrClass;
+
+class CsrMatrixClass {
+ public:
+  // copies a 0-based CSR (host arrays)
+  int Setup(int n, const int* ia, const int* ja, const double* a) {
+    n_ = n;
+    ia_.assign(ia, ia + n + 1);
+    ja_.assign(ja, ja + ia[n]);
+    a_.assign(a, a + ia[n]);
+    return 0;
+  }
+  // CooMatrixClass::Laplacian (coo_matrix.cpp:519-640), argument order of the reference
+  int Laplacian(int nx, int ny, int nz, double alphax, double alphay, double alphaz, double shift) {
+    int n = 0;
+    int64_t nnz = 0;
+    int *ia = nullptr, *ja = nullptr;
+    double* a = nullptr;
+    int err = pgs_gen_laplacian(nx, ny, nz, shift, 0.0, alphax, alphay, alphaz, 0, &n, &nnz, &ia, &ja, &a);
+    if (err) return err;
+    err = Setup(n, ia, ja, a);
+    pgs_free(ia);
+    pgs_free(ja);
+    pgs_free(a);
+    return err;
+  }
+  int GetNumRowsLocal() const { return n_; }
+  int64_t GetNumNonzeros() const { return (int64_t)ja_.size(); }
+  const int* GetI() const { return ia_.data(); }
+  const int* GetJ() const { return ja_.data(); }
+  const double* GetData() const { return a_.data(); }
+  // y = alpha * A * x + beta * y, trans 'N' (CsrMatrixClass::MatVec,
+  // csr_matrix.cpp:1340-1365).  Host vectors run on the host; device vectors
+  // need a device operator from a GemslrClass set up on this matrix.
+  int MatVec(char trans, double alpha, const SequentialVectorClass& x, double beta, SequentialVectorClass& y) const;
+
+ private:
+  friend class GemslrClass;
+  int n_ = 0;
+  std::vector<int> ia_, ja_;
+  std::vector<double> a_;
+  mutable pgs_dev_op_t dop_ = nullptr;  // set by GemslrClass::Setup on the device
+  mutable pgs_ws_t ws_ = nullptr;
+};
+
+// Preconditioner slot (SolverClass): Setup builds the operator on the host
+// and uploads it; Solve applies z = M^{-1} rhs.
+class GemslrClass {
+ public:
+  GemslrClass() { pgs_params_default(params_); }
+  GemslrClass(const GemslrClass&) = delete;
+  GemslrClass& operator=(const GemslrClass&) = delete;
+  virtual ~GemslrClass() { Clear(); }
+  void Clear() {
+    if (matrix_ && matrix_->dop_ == dop_) {
+      matrix_->dop_ = nullptr;
+      matrix_->ws_ = nullptr;
+    }
+    if (ws_) pgs_ws_destroy(ws_);
+    if (dop_) pgs_dev_op_destroy(dop_);
+    if (host_) pgs_gemslr_destroy(host_);
+    ws_ = nullptr;
+    dop_ = nullptr;
+    host_ = nullptr;
+    ready_ = false;
+  }
+  int SetMatrix(CsrMatrixClass& A) {
+    Clear();
+    matrix_ = &A;
+    return 0;
+  }
+  // GemslrClass::SetWithParameterArray (gemslr.hpp:2069-2161)
+  virtual int SetWithParameterArray(const double* params) {
+    std::memcpy(params_, params, sizeof(params_));
+    return 0;
+  }
+  int SetSolveLocation(int location) {
+    location_ = location;
+    return 0;
+  }
+  // PGS_PREC_FP64 (default) or PGS_PREC_FP32 arithmetic for the device apply
+  int SetApplyPrecision(int prec) {
+    prec_ = prec;
+    return 0;
+  }
+  void SetSeed(uint64_t seed) { seed_ = seed; }
+  virtual int Setup(SequentialVectorClass&, SequentialVectorClass&) {
+    if (ready_) return 0;
+    if (!matrix_) return PGS_ERROR_INVALID_PARAM;
+    int err = pgs_gemslr_setup(matrix_->n_, matrix_->ia_.data(), matrix_->ja_.data(), matrix_->a_.data(), 0, params_,
+                               seed_, &host_);
+    if (err) return err;
+    if (location_ == kMemoryDevice) {
+      err = pgs_dev_op_create(host_, 0, PGS_PREC_FP64 | PGS_PREC_FP32, &dop_);
+      if (err) return err;
+      err = pgs_ws_create(dop_, 1, kdim_ws_, &ws_);
+      if (err) return err;
+      matrix_->dop_ = dop_;
+      matrix_->ws_ = ws_;
+    }
+    ready_ = true;
+    return 0;
+  }
+  // z = M^{-1} rhs (GemslrClass::Solve, gemslr.cpp:3431-3467)
+  virtual int Solve(SequentialVectorClass& x, SequentialVectorClass& rhs) {
+    if (!ready_) return PGS_ERROR_FUNCTION_CALL;
+    if (x.GetLengthLocal() != matrix_->n_ || rhs.GetLengthLocal() != matrix_->n_) return PGS_ERROR_INVALID_PARAM;
+    if (x.GetDataLocation() == kMemoryDevice && rhs.GetDataLocation() == kMemoryDevice) {
+      if (!dop_) return PGS_ERROR_MEMORY_LOCATION;
+      return pgs_apply(dop_, ws_, rhs.GetData(), x.GetData(), 1, prec_, nullptr);
+    }
+    if (x.GetDataLocation() == kMemoryHost && rhs.GetDataLocation() == kMemoryHost)
+      return pgs_gemslr_host_apply(host_, rhs.GetData(), x.GetData());
+    return PGS_ERROR_MEMORY_LOCATION;
+  }
+  // nnz of the ILU factors and of the low-rank terms (GemslrClass::GetNumNonzeros)
+  int64_t GetNumNonzeros(int64_t& nnz_ilu, int64_t& nnz_lr) const {
+    pgs_gemslr_info_t inf;
+    if (!host_ || pgs_gemslr_info(host_, &inf)) return 0;
+    nnz_ilu = inf.nnz_ilu;
+    nnz_lr = inf.nnz_lr;
+    return inf.nnz_ilu + inf.nnz_lr;
+  }
+  pgs_gemslr_t GetHostOperator() const { return host_; }
+  pgs_dev_op_t GetDeviceOperator() const { return dop_; }
+  int GetApplyPrecision() const { return prec_; }
+  void SetWorkspaceKrylovDimension(int kdim) { kdim_ws_ = kdim; }
+
+ protected:
+  CsrMatrixClass* matrix_ = nullptr;
+  double params_[PGS_IO_SIZE];
+  int location_ = kMemoryDevice;
+  int prec_ = PGS_PREC_FP64;
+  int kdim_ws_ = 50;
+  uint64_t seed_ = 0;
+  bool ready_ = false;
+  pgs_gemslr_t host_ = nullptr;
+  pgs_dev_op_t dop_ = nullptr;
+  pgs_ws_t ws_ = nullptr;
+};
+
+// ILUT preconditioner of the whole matrix (IluClass with kIluOptionILUT):
+// a one-level GeMSLR operator, i.e. a single ILUT of P A P^T with the RCM
+// ordering P.  Droptol / max row nnz follow PGS_IO_ILU_DROPTOL_B_GLOBAL /
+// PGS_IO_ILU_ROWNNZ_B_GLOBAL.
+class IluClass : public GemslrClass {
+ public:
+  IluClass() {
+    params_[PGS_IO_PREPOSS_NLEV_GLOBAL] = 1;
+    params_[PGS_IO_SCHUR_ENABLE] = 0;
+  }
+  int SetDropTolerance(double tol) {
+    params_[PGS_IO_ILU_DROPTOL_B_GLOBAL] = tol;
+    params_[PGS_IO_ILU_DROPTOL_C_GLOBAL] = tol;
+    return 0;
+  }
+  int SetMaxNnzPerRow(int lfil) {
+    params_[PGS_IO_ILU_ROWNNZ_B_GLOBAL] = lfil;
+    params_[PGS_IO_ILU_ROWNNZ_C_GLOBAL] = lfil;
+    return 0;
+  }
+  int SetWithParameterArray(const double* params) override {
+    GemslrClass::SetWithParameterArray(params);
+    params_[PGS_IO_PREPOSS_NLEV_GLOBAL] = 1;
+    params_[PGS_IO_SCHUR_ENABLE] = 0;
+    params_[PGS_IO_ILU_DROPTOL_C_GLOBAL] = params_[PGS_IO_ILU_DROPTOL_B_GLOBAL];
+    params_[PGS_IO_ILU_ROWNNZ_C_GLOBAL] = params_[PGS_IO_ILU_ROWNNZ_B_GLOBAL];
+    return 0;
+  }
+};
+
+inline int CsrMatrixClass::MatVec(char trans, double alpha, const SequentialVectorClass& x, double beta,
+                                  SequentialVectorClass& y) const {
+  if (trans != 'N' && trans != 'n') return PGS_ERROR_INVALID_OPTION;
+  if (x.GetLengthLocal() != n_ || y.GetLengthLocal() != n_) return PGS_ERROR_INVALID_PARAM;
+  if (x.GetDataLocation() == kMemoryDevice && y.GetDataLocation() == kMemoryDevice) {
+    if (!dop_) return PGS_ERROR_MEMORY_LOCATION;
+    return pgs_spmv(dop_, ws_, &alpha, x.GetData(), &beta, y.GetData(), 1, nullptr);
+  }
+  if (x.GetDataLocation() != kMemoryHost || y.GetDataLocation() != kMemoryHost) return PGS_ERROR_MEMORY_LOCATION;
+  const double* xv = x.GetData();
+  double* yv = y.GetData();
+  for (int i = 0; i < n_; ++i) {
+    double s = 0.0;
+    for (int k = ia_[i]; k < ia_[i + 1]; ++k) s += a_[k] * xv[ja_[k]];
+    yv[i] = (beta == 0.0 ? 0.0 : beta * yv[i]) + alpha * s;
+  }
+  return 0;
+}
+
+// Flexible GMRES(m) with right preconditioning (FlexGmresClass,
+// fgmres.hpp:230-558).  Runs on the device (x, rhs device vectors).
+class FlexGmresClass {
+ public:
+  int SetMatrix(CsrMatrixClass& A) {
+    matrix_ = &A;
+    ready_ = false;
+    return 0;
+  }
+  int SetPreconditioner(GemslrClass& P) {
+    precond_ = &P;
+    ready_ = false;
+    return 0;
+  }
+  // FlexGmresClass::SetWithParameterArray (fgmres.hpp:595-603)
+  int SetWithParameterArray(const double* params) {
+    tol_ = params[PGS_IO_SOLVER_TOL];
+    maxits_ = (int)params[PGS_IO_SOLVER_MAXITS];
+    kdim_ = (int)params[PGS_IO_SOLVER_KDIM];
+    absolute_tol_ = params[PGS_IO_SOLVER_ATOL] != 0.0;
+    return 0;
+  }
+  int SetKrylovSubspaceDimension(int kdim) {
+    kdim_ = kdim;
+    return 0;
+  }
+  int SetMaxNumberIterations(int maxits) {
+    maxits_ = maxits;
+    return 0;
+  }
+  int SetTolerance(double tol) {
+    tol_ = tol;
+    return 0;
+  }
+  int SetAbsoluteTol(bool abs) {
+    absolute_tol_ = abs;
+    return 0;
+  }
+  int SetSolveLocation(int location) {
+    location_ = location;
+    return 0;
+  }
+  // PGS_ORTH_MGS (the reference's modified Gram-Schmidt) or PGS_ORTH_CGS2
+  int SetOrthogonalization(int orth) {
+    orth_ = orth;
+    return 0;
+  }
+  int Setup(SequentialVectorClass& x, SequentialVectorClass& rhs) {
+    if (ready_) return 0;
+    if (!matrix_) return PGS_ERROR_INVALID_PARAM;
+    if (precond_) {
+      precond_->SetSolveLocation(location_);
+      precond_->SetWorkspaceKrylovDimension(kdim_);
+      int err = precond_->Setup(x, rhs);
+      if (err) return err;
+    }
+    ready_ = true;
+    return 0;
+  }
+  int Solve(SequentialVectorClass& x, SequentialVectorClass& rhs) {
+    if (!ready_) return PGS_ERROR_FUNCTION_CALL;
+    if (location_ != kMemoryDevice || x.GetDataLocation() != kMemoryDevice ||
+        rhs.GetDataLocation() != kMemoryDevice)
+      return PGS_ERROR_MEMORY_LOCATION;
+    pgs_dev_op_t dop = precond_ ? precond_->GetDeviceOperator() : nullptr;
+    if (!dop) return PGS_ERROR_FUNCTION_CALL;
+    if (!ws_ || ws_kdim_ < kdim_) {
+      if (ws_) pgs_ws_destroy(ws_);
+      int err = pgs_ws_create(dop, 1, kdim_, &ws_);
+      if (err) return err;
+      ws_kdim_ = kdim_;
+    }
+    pgs_fgmres_opts_t o;
+    o.kdim = kdim_;
+    o.maxits = maxits_;
+    o.tol = tol_;
+    o.absolute_tol = absolute_tol_ ? 1 : 0;
+    o.prec = precond_->GetApplyPrecision();
+    o.orth = orth_;
+    o.use_precond = 1;
+    o.check_every = 1;
+    pgs_fgmres_result_t r;
+    int err = pgs_fgmres(dop, ws_, rhs.GetData(), x.GetData(), 1, &o, &r, nullptr);
+    if (err) return err;
+    iters_ = r.iters;
+    rel_res_ = r.rel_res;
+    status_ = r.status;
+    return 0;
+  }
+  int GetNumberIterations() const { return iters_; }
+  double GetFinalRelativeResidual() const { return rel_res_; }
+  int GetStatus() const { return status_; }
+  ~FlexGmresClass() {
+    if (ws_) pgs_ws_destroy(ws_);
+  }
+
+ private:
+  CsrMatrixClass* matrix_ = nullptr;
+  GemslrClass* precond_ = nullptr;
+  int kdim_ = 50, maxits_ = 100;
+  double tol_ = 1e-6;
+  bool absolute_tol_ = false;
+  int location_ = kMemoryDevice;
+  int orth_ = PGS_ORTH_CGS2;
+  bool ready_ = false;
+  pgs_ws_t ws_ = nullptr;
+  int ws_kdim_ = 0;
+  int iters_ = 0, status_ = 0;
+  double rel_res_ = 0.0;
+};
+
+}  // namespace pargemslr_amd
+
+#endif  // PARGEMSLR_AMD_HPP

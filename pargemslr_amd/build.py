@@ -50,6 +50,13 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, LIB)
+    # C++ driver on the class surface (include/pargemslr_amd.hpp), run by the tests
+    root = os.path.dirname(HERE)
+    drv_src = os.path.join(root, "tests", "cpp", "driver_laplacian.cpp")
+    if os.path.exists(drv_src):
+        drv = os.path.join(root, "tests", "cpp", "driver_laplacian")
+        _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), drv_src, "-o", drv, "-L", HERE,
+              "-lpargemslr_amd", "-Wl,-rpath,$ORIGIN/../../pargemslr_amd"])
     return LIB
 
 
