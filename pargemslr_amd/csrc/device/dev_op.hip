@@ -103,6 +103,7 @@ constexpr int kTrsvMaxWidth = 20;  // widest factor row the pipelined kernel tak
 
 struct DevTrsv {
   int nblocks = 0, nloc = 0, max_rows = 0, max_steps = 0, max_width = 0;
+  int window = 0;  // ring size when the packed columns are ring slots, else 0
   int64_t nnz_padded = 0, nnz_real = 0, nrows_total = 0;
   DevMem row0, step, stepU, ent0, ri0, desc, ent, ri;
 };
@@ -114,6 +115,7 @@ void upload_trsv(DevTrsv& d, const TrsvPack<double>& P, int64_t* counter) {
   d.max_rows = P.max_block_rows;
   d.max_steps = P.max_steps;
   d.max_width = P.max_width;
+  d.window = P.window;
   d.nnz_padded = (int64_t)P.col.size();
   d.nnz_real = P.nnz_real;
   d.nrows_total = (int64_t)P.rows.size();
@@ -161,7 +163,8 @@ struct pgs_dev_op_s {
   int nlev = 0;
   int is_complex = 0;
   int lds_max = 65536;
-  int trsv_variant = 0;  // 0: pipelined per-(block, column) kernel; 1: basic level-set kernel
+  int trsv_variant = 0;  // 0: pipelined kernels (ring when packed for it); 1: basic level-set kernel
+  int trsv_cg = 4;       // most right-hand-side columns per workgroup of the ring kernel
   int trsv_nt = 128;     // threads of the pipelined kernel
   int trsv_lpr = 4;      // lanes per row (steps hold at most trsv_nt / trsv_lpr rows)
   int64_t bytes = 0;
@@ -303,9 +306,41 @@ void launch_trsv_pipe(int nt, int lpr, const DevTrsv& T, const V* rhs, V* x, int
   }
 }
 
+// Ring kernel geometry: 128 threads, 4 lanes per row (steps of at most 32
+// rows), 5 slots per lane (rows of at most 20 entries).
+constexpr int kRingNT = 128, kRingLPR = 4, kRingEPL = 5;
+
+template <typename V, int CG>
+void launch_trsv_ring_cg(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
+  constexpr int D = sizeof(V) == 4 ? 8 : 6;
+  auto kern = k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, CG>;
+  static const hipError_t attr =
+      hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, op->lds_max);
+  HIPCHK(attr);
+  const size_t lds = (size_t)T.window * CG * sizeof(V);
+  const unsigned grid = (unsigned)T.nblocks * (unsigned)(nrhs / CG);
+  kern<<<grid, kRingNT, lds, s>>>(T.nblocks, T.window - 1, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
+                                  T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), T.desc.as<int4>(),
+                                  T.ent.as<TPair<V>>(), T.ri.as<TPair<V>>(), rhs, x, nrhs);
+}
+
+template <typename V>
+void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
+  int cg = std::min(nrhs, op->trsv_cg);
+  while (cg > 1 && (size_t)T.window * cg * sizeof(V) > (size_t)op->lds_max) cg >>= 1;
+  if (cg >= 4) launch_trsv_ring_cg<V, 4>(op, T, rhs, x, nrhs, s);
+  else if (cg == 2) launch_trsv_ring_cg<V, 2>(op, T, rhs, x, nrhs, s);
+  else launch_trsv_ring_cg<V, 1>(op, T, rhs, x, nrhs, s);
+}
+
 template <typename V>
 void launch_trsv_impl(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
   if (T.nblocks == 0) return;
+  if (T.window > 0) {
+    launch_trsv_ring<V>(op, T, rhs, x, nrhs, s);
+    HIPCHK(hipGetLastError());
+    return;
+  }
   const size_t lds = (size_t)T.max_rows * sizeof(V);
   if (op->trsv_variant == 0 && T.max_width <= kTrsvMaxWidth) {
     if (lds <= (size_t)op->lds_max) launch_trsv_pipe<V, true>(op->trsv_nt, op->trsv_lpr, T, rhs, x, nrhs, lds, s);
@@ -427,6 +462,24 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
     op->lev_ptr = H.lev_ptr;
     upload_csr<double>(op->A, H.A, &op->bytes);
     upload(op->perm, H.perm, &op->bytes);
+    if (const char* v = std::getenv("PGS_TRSV_CG")) {
+      const int cg = std::atoi(v);
+      if (cg == 1 || cg == 2 || cg == 4) op->trsv_cg = cg;
+    }
+    // ring packing (ring kernel) unless another kernel geometry was asked for;
+    // the ring holds at least one FP64 column
+    const bool ring = op->trsv_variant == 0 && op->trsv_nt == kRingNT && op->trsv_lpr == kRingLPR;
+    int max_window = 0;
+    if (ring) {
+      max_window = 1;
+      while (2 * max_window * (int)sizeof(double) <= op->lds_max) max_window *= 2;
+    }
+    std::vector<TrsvPack<double>> packs;
+    for (int l = 0; l < H.nlev; ++l) {
+      packs.push_back(trsv_pack(H.levels[l], H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr, max_window));
+      if (packs.back().window > 0 && packs.back().max_width > kRingLPR * kRingEPL)
+        packs.back() = trsv_pack(H.levels[l], H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr, 0);
+    }
     auto build = [&](auto& P, auto tag) {
       using V = decltype(tag);
       P.present = true;
@@ -437,8 +490,7 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
         L->lev_start = H.lev_ptr[l];
         L->nloc = H.nloc(l);
         L->nrem = H.nrem(l);
-        TrsvPack<double> pk = trsv_pack(HL, H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr);
-        upload_trsv<V>(L->B, pk, &op->bytes);
+        upload_trsv<V>(L->B, packs[l], &op->bytes);
         if (l + 1 < H.nlev) {
           upload_csr<V>(L->E, HL.E, &op->bytes);
           upload_csr<V>(L->F, HL.F, &op->bytes);

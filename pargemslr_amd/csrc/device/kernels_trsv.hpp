@@ -247,5 +247,154 @@ __global__ __launch_bounds__(NT) void k_trsv_pipe(int nblocks, const int* __rest
   }
 }
 
+// Ring kernel: one workgroup per (diagonal block, group of CG right-hand-side
+// columns); grid index = group * nblocks + block (all groups of a block on one
+// XCD, as above).  The packed columns are ring slots: the value of the row
+// stored at pair index p lives in LDS slot p & wmask (CG values per slot)
+// from its step until its last reader, so the LDS footprint is the sweep
+// window, not the block.  The L sweep reads rhs and writes y to x in global
+// memory; the U sweep reads y back from x and overwrites it with the solution
+// (a row's y is read by the lanes that later write that row).  Rows are
+// prefetched D steps ahead (descriptor, row pair, slots); the right-hand side
+// of a step, whose address depends on the row pair, is loaded D/2 steps ahead.
+// Lane q of a row's LPR lanes finalizes columns q, q + LPR, ...
+template <typename V, int CG>
+struct alignas(CG * sizeof(V) < 16 ? CG * sizeof(V) : 16) RingRow {
+  V v[CG];
+};
+
+__device__ __forceinline__ float bload_val(__amdgpu_buffer_rsrc_t r, int off, float) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ double bload_val(__amdgpu_buffer_rsrc_t r, int off, double) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  return __hiloint2double((int)v[1], (int)v[0]);
+}
+
+template <typename V, int NT, int LPR, int EPL, int D, int CG, bool UPPER>
+__device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int wmask, int nrhs, int c0,
+                                           __amdgpu_buffer_rsrc_t rs_desc, __amdgpu_buffer_rsrc_t rs_ri,
+                                           __amdgpu_buffer_rsrc_t rs_ent, __amdgpu_buffer_rsrc_t rs_src, V* xs,
+                                           V* __restrict__ dst) {
+  constexpr int NCL = CG >= LPR ? CG / LPR : 1;  // columns finalized per lane
+  constexpr int PS = (int)sizeof(TPair<V>);
+  constexpr int VS = (int)sizeof(V);
+  static_assert(D % 2 == 0, "stage depth must be even");
+  const int tid = threadIdx.x;
+  const int q = tid % LPR;
+  const int t0 = tid / LPR;
+  int cl[NCL];  // this lane's columns within the group
+#pragma unroll
+  for (int j = 0; j < NCL; ++j) cl[j] = CG >= LPR ? q + j * LPR : min(q, CG - 1);
+  const bool writer = CG >= LPR || q < CG;
+
+  TPair<V> pe[D][EPL];
+  TPair<V> pr[D];
+  V xr[D][NCL];
+  int4 dd[D];
+  int4 dn[D];
+
+#define PGS_RING_STAGE(K, DESC)                                                 \
+  do {                                                                          \
+    const int4 d_ = (DESC);                                                     \
+    dd[K] = d_;                                                                 \
+    const int tt_ = min(t0, max(d_.z - 1, 0));                                  \
+    pr[K] = bload_pair(rs_ri, (d_.x + tt_) * PS, V(0));                         \
+    _Pragma("unroll") for (int e_ = 0; e_ < EPL; ++e_) {                        \
+      const int j_ = min(q + e_ * LPR, max(d_.w - 1, 0));                       \
+      pe[K][e_] = bload_pair(rs_ent, (d_.y + j_ * d_.z + tt_) * PS, V(0));      \
+    }                                                                           \
+  } while (0)
+#define PGS_RING_RHS(K)                                                           \
+  do {                                                                            \
+    _Pragma("unroll") for (int j_ = 0; j_ < NCL; ++j_) xr[K][j_] =                \
+        bload_val(rs_src, (pr[K].col * nrhs + c0 + cl[j_]) * VS, V(0));           \
+  } while (0)
+
+  const int rel = s_begin - sbeg;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    PGS_RING_STAGE(k, bload_i4(rs_desc, (rel + k) * 16));
+    dn[k] = bload_i4(rs_desc, (rel + k + D) * 16);
+  }
+#pragma unroll
+  for (int k = 0; k < D / 2; ++k) PGS_RING_RHS(k);
+
+  for (int base = s_begin; base < s_end; base += D) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const int s = base + k;
+      const int4 d = dd[k];
+      V acc[CG];
+#pragma unroll
+      for (int c = 0; c < CG; ++c) acc[c] = 0;
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) {
+        const RingRow<V, CG> xv = reinterpret_cast<const RingRow<V, CG>*>(xs)[pe[k][e].col];
+        const V w = (q + e * LPR < d.w) ? pe[k][e].val : (V)0;
+#pragma unroll
+        for (int c = 0; c < CG; ++c) acc[c] += w * xv.v[c];
+      }
+#pragma unroll
+      for (int c = 0; c < CG; ++c) {
+        if constexpr (LPR == 4) acc[c] = quad_sum(acc[c]);
+        else if constexpr (LPR == 2) acc[c] = pair_sum(acc[c]);
+      }
+      const int tt = min(t0, max(d.z - 1, 0));
+      const int slot = (d.x + tt) & wmask;
+      const int row = pr[k].col;
+      const V dv = pr[k].val;
+      if (s < s_end && t0 < d.z && writer) {
+#pragma unroll
+        for (int j = 0; j < NCL; ++j) {
+          V a = acc[0];
+#pragma unroll
+          for (int c = 1; c < CG; ++c) a = (c == cl[j]) ? acc[c] : a;
+          V v = xr[k][j] - a;
+          if (UPPER) v *= dv;
+          xs[slot * CG + cl[j]] = v;
+          dst[(int64_t)row * nrhs + c0 + cl[j]] = v;
+        }
+      }
+      // refill this stage with step s + D, and load the right-hand side of
+      // step s + D/2, whose row pair arrived D/2 steps ago
+      PGS_RING_STAGE(k, dn[k]);
+      dn[k] = bload_i4(rs_desc, (s - sbeg + 2 * D) * 16);
+      PGS_RING_RHS((k + D / 2) % D);
+      __syncthreads();
+    }
+  }
+#undef PGS_RING_STAGE
+#undef PGS_RING_RHS
+}
+
+template <typename V, int NT, int LPR, int EPL, int D, int CG>
+__global__ __launch_bounds__(NT) void k_trsv_ring(int nblocks, int wmask, const int* __restrict__ blk_row0,
+                                                  const int* __restrict__ blk_step, const int* __restrict__ blk_stepU,
+                                                  const int64_t* __restrict__ blk_ent0,
+                                                  const int64_t* __restrict__ blk_ri0, const int4* __restrict__ desc,
+                                                  const TPair<V>* __restrict__ ent, const TPair<V>* __restrict__ ri,
+                                                  const V* __restrict__ rhs, V* __restrict__ x, int nrhs) {
+  extern __shared__ unsigned char smem_raw[];
+  V* xs = reinterpret_cast<V*>(smem_raw);
+  const int b = blockIdx.x % nblocks;
+  const int c0 = (blockIdx.x / nblocks) * CG;
+  const int r0 = blk_row0[b], nb = blk_row0[b + 1] - r0;
+  const int sbeg = blk_step[b], send = blk_step[b + 1], sU = blk_stepU[b];
+  const int64_t e0 = blk_ent0[b], e1 = blk_ent0[b + 1];
+  const int64_t q0 = blk_ri0[b], q1 = blk_ri0[b + 1];
+  const __amdgpu_buffer_rsrc_t rs_ent = make_rsrc(ent + e0, (e1 - e0) * (int64_t)sizeof(TPair<V>));
+  const __amdgpu_buffer_rsrc_t rs_ri = make_rsrc(ri + q0, (q1 - q0) * (int64_t)sizeof(TPair<V>));
+  const __amdgpu_buffer_rsrc_t rs_desc = make_rsrc(desc + sbeg, (int64_t)(send - sbeg) * 16);
+  const int64_t vbytes = (int64_t)nb * nrhs * (int64_t)sizeof(V);
+  V* xb = x + (int64_t)r0 * nrhs;
+  for (int i = threadIdx.x; i < (wmask + 1) * CG; i += NT) xs[i] = 0;
+  __syncthreads();
+  ring_sweep<V, NT, LPR, EPL, D, CG, false>(sbeg, sU, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
+                                            make_rsrc(rhs + (int64_t)r0 * nrhs, vbytes), xs, xb);
+  ring_sweep<V, NT, LPR, EPL, D, CG, true>(sU, send, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
+                                           make_rsrc(xb, vbytes), xs, xb);
+}
+
 }  // namespace dev
 }  // namespace pgs

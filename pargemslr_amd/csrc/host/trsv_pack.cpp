@@ -6,61 +6,111 @@ namespace pgs {
 
 namespace {
 
+// Rows of one triangular factor in sweep order, cut into steps.
+struct SweepOrder {
+  std::vector<int> order;  // rows in sweep order
+  std::vector<int> step;   // step boundaries into order (nsteps + 1)
+  std::vector<int> pos;    // pos[row] = index in order
+  int window = 0;
+};
+
+// Forward topological order: L rows increasing (dependencies j < i), U rows
+// decreasing (dependencies j > i).
+inline int topo_row(int s, int n, bool upper) { return upper ? n - 1 - s : s; }
+
 template <typename T>
-void pack_sweep(const Csr<T>& M, const std::vector<T>* dinv, bool upper, int max_rows, int64_t ent0, int64_t ri0,
-                TrsvPack<T>& P) {
+int levels_asap(const Csr<T>& M, bool upper, std::vector<int>& lev) {
   const int n = M.nrows;
-  std::vector<int> lev(n, 0);
+  lev.assign(n, 0);
   int nl = 0;
-  if (!upper) {
-    for (int i = 0; i < n; ++i) {
-      int l = 0;
-      for (int k = M.ia[i]; k < M.ia[i + 1]; ++k) l = std::max(l, lev[M.ja[k]] + 1);
-      lev[i] = l;
-      nl = std::max(nl, l + 1);
-    }
-  } else {
-    for (int i = n - 1; i >= 0; --i) {
-      int l = 0;
-      for (int k = M.ia[i]; k < M.ia[i + 1]; ++k) l = std::max(l, lev[M.ja[k]] + 1);
-      lev[i] = l;
-      nl = std::max(nl, l + 1);
-    }
+  for (int s = 0; s < n; ++s) {
+    const int i = topo_row(s, n, upper);
+    int l = 0;
+    for (int k = M.ia[i]; k < M.ia[i + 1]; ++k) l = std::max(l, lev[M.ja[k]] + 1);
+    lev[i] = l;
+    nl = std::max(nl, l + 1);
   }
-  // bucket rows by level (stable in row order)
+  return nl;
+}
+
+template <typename T>
+void levels_alap(const Csr<T>& M, bool upper, int nl, std::vector<int>& lev) {
+  const int n = M.nrows;
+  lev.assign(n, std::max(nl - 1, 0));
+  for (int s = n - 1; s >= 0; --s) {
+    const int i = topo_row(s, n, upper);
+    for (int k = M.ia[i]; k < M.ia[i + 1]; ++k) lev[M.ja[k]] = std::min(lev[M.ja[k]], lev[i] - 1);
+  }
+}
+
+template <typename T>
+SweepOrder make_order(const Csr<T>& M, const std::vector<int>& lev, int nl, int max_rows) {
+  const int n = M.nrows;
+  SweepOrder o;
   std::vector<int> cnt(nl + 1, 0);
   for (int i = 0; i < n; ++i) cnt[lev[i] + 1]++;
   for (int l = 0; l < nl; ++l) cnt[l + 1] += cnt[l];
-  std::vector<int> order(n), pos(cnt.begin(), cnt.end() - 1);
-  for (int i = 0; i < n; ++i) order[pos[lev[i]]++] = i;
-  for (int l = 0; l < nl; ++l) {
-    for (int c0 = cnt[l]; c0 < cnt[l + 1]; c0 += max_rows) {
-      const int nr = std::min(max_rows, cnt[l + 1] - c0);
-      int width = 0;
-      for (int t = 0; t < nr; ++t) {
-        int i = order[c0 + t];
-        width = std::max(width, M.ia[i + 1] - M.ia[i]);
+  o.order.resize(n);
+  o.pos.resize(n);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int i = 0; i < n; ++i) o.order[fill[lev[i]]++] = i;  // stable in row order
+  for (int k = 0; k < n; ++k) o.pos[o.order[k]] = k;
+  o.step.push_back(0);
+  for (int l = 0; l < nl; ++l)
+    for (int c0 = cnt[l]; c0 < cnt[l + 1]; c0 += max_rows) o.step.push_back(std::min(cnt[l + 1], c0 + max_rows));
+  // window: largest (end of a step) - (position of a row it reads)
+  for (size_t s = 0; s + 1 < o.step.size(); ++s) {
+    const int b = o.step[s + 1];
+    for (int k = o.step[s]; k < b; ++k) {
+      const int i = o.order[k];
+      for (int e = M.ia[i]; e < M.ia[i + 1]; ++e) o.window = std::max(o.window, b - o.pos[M.ja[e]]);
+    }
+  }
+  return o;
+}
+
+// The schedule of the two with the smaller window (as soon as possible on a tie).
+template <typename T>
+SweepOrder best_order(const Csr<T>& M, bool upper, int max_rows) {
+  std::vector<int> lev;
+  const int nl = levels_asap(M, upper, lev);
+  SweepOrder a = make_order(M, lev, nl, max_rows);
+  if (a.window == 0) return a;
+  levels_alap(M, upper, nl, lev);
+  SweepOrder b = make_order(M, lev, nl, max_rows);
+  return b.window < a.window ? b : a;
+}
+
+template <typename T>
+void emit_sweep(const Csr<T>& M, const std::vector<T>* dinv, const SweepOrder& o, int base, int wmask,
+                int64_t ent0, int64_t ri0, TrsvPack<T>& P) {
+  for (size_t s = 0; s + 1 < o.step.size(); ++s) {
+    const int k0 = o.step[s], nr = o.step[s + 1] - k0;
+    int width = 0;
+    for (int t = 0; t < nr; ++t) {
+      const int i = o.order[k0 + t];
+      width = std::max(width, M.ia[i + 1] - M.ia[i]);
+    }
+    const int64_t row_off = (int64_t)P.rows.size() - ri0, ent_off = (int64_t)P.col.size() - ent0;
+    P.desc.push_back((int)row_off);
+    P.desc.push_back((int)ent_off);
+    P.desc.push_back(nr);
+    P.desc.push_back(width);
+    P.max_width = std::max(P.max_width, width);
+    const size_t e = P.col.size();
+    P.col.resize(e + (size_t)width * nr, 0);
+    P.val.resize(e + (size_t)width * nr, T(0));
+    for (int t = 0; t < nr; ++t) {
+      const int i = o.order[k0 + t];
+      P.rows.push_back(i);
+      P.dinv.push_back(dinv ? (*dinv)[i] : T(0));
+      int j = 0;
+      for (int k = M.ia[i]; k < M.ia[i + 1]; ++k, ++j) {
+        const int c = M.ja[k];
+        P.col[e + (size_t)j * nr + t] = wmask ? ((base + o.pos[c]) & wmask) : c;
+        P.val[e + (size_t)j * nr + t] = M.a[k];
       }
-      const int64_t row_off = (int64_t)P.rows.size() - ri0, ent_off = (int64_t)P.col.size() - ent0;
-      P.desc.push_back((int)row_off);
-      P.desc.push_back((int)ent_off);
-      P.desc.push_back(nr);
-      P.desc.push_back(width);
-      P.max_width = std::max(P.max_width, width);
-      const size_t base = P.col.size();
-      P.col.resize(base + (size_t)width * nr, 0);
-      P.val.resize(base + (size_t)width * nr, T(0));
-      for (int t = 0; t < nr; ++t) {
-        int i = order[c0 + t];
-        P.rows.push_back(i);
-        P.dinv.push_back(dinv ? (*dinv)[i] : T(0));
-        int j = 0;
-        for (int k = M.ia[i]; k < M.ia[i + 1]; ++k, ++j) {
-          P.col[base + (size_t)j * nr + t] = M.ja[k];
-          P.val[base + (size_t)j * nr + t] = M.a[k];
-        }
-        P.nnz_real += M.ia[i + 1] - M.ia[i];
-      }
+      P.nnz_real += M.ia[i + 1] - M.ia[i];
     }
   }
 }
@@ -68,10 +118,21 @@ void pack_sweep(const Csr<T>& M, const std::vector<T>* dinv, bool upper, int max
 }  // namespace
 
 template <typename T>
-TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_rows) {
+TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_rows, int max_window) {
   TrsvPack<T> P;
   P.nblocks = L.ncomp;
   P.nloc = nloc;
+  std::vector<SweepOrder> oL(L.ncomp), oU(L.ncomp);
+  for (int b = 0; b < L.ncomp; ++b) {
+    const IluFactor<T>& F = L.ilu[b];
+    oL[b] = best_order(F.L, false, max_rows);
+    oU[b] = best_order(F.U, true, max_rows);
+    P.window_needed = std::max(P.window_needed, std::max(oL[b].window, oU[b].window));
+  }
+  int W = 1;
+  while (W < P.window_needed) W <<= 1;
+  P.window = (max_window > 0 && W <= max_window) ? W : 0;
+  const int wmask = P.window ? P.window - 1 : 0;
   P.blk_row0.resize(L.ncomp + 1);
   P.blk_step.resize(L.ncomp + 1);
   P.blk_stepU.resize(L.ncomp);
@@ -85,10 +146,12 @@ TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_ro
     P.blk_ent0[b] = ent0;
     P.blk_ri0[b] = ri0;
     P.blk_step[b] = (int)P.desc.size() / 4;
-    pack_sweep<T>(F.L, nullptr, false, max_rows, ent0, ri0, P);
+    emit_sweep<T>(F.L, nullptr, oL[b], 0, wmask, ent0, ri0, P);
     P.blk_stepU[b] = (int)P.desc.size() / 4;
-    pack_sweep<T>(F.U, &F.Dinv, true, max_rows, ent0, ri0, P);
+    emit_sweep<T>(F.U, &F.Dinv, oU[b], F.n, wmask, ent0, ri0, P);
     P.max_steps = std::max(P.max_steps, (int)P.desc.size() / 4 - P.blk_step[b]);
+    oL[b] = SweepOrder();
+    oU[b] = SweepOrder();
   }
   P.blk_row0[L.ncomp] = nloc;
   P.blk_step[L.ncomp] = (int)P.desc.size() / 4;
@@ -97,7 +160,7 @@ TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_ro
   return P;
 }
 
-template TrsvPack<double> trsv_pack<double>(const HostLevel<double>&, int, int, int);
-template TrsvPack<zdouble> trsv_pack<zdouble>(const HostLevel<zdouble>&, int, int, int);
+template TrsvPack<double> trsv_pack<double>(const HostLevel<double>&, int, int, int, int);
+template TrsvPack<zdouble> trsv_pack<zdouble>(const HostLevel<zdouble>&, int, int, int, int);
 
 }  // namespace pgs
