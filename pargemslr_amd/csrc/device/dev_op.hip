@@ -185,6 +185,7 @@ struct pgs_ws_s {
   DevMem V, Z, w, part, hbuf, h2buf, t2, H, cs, sn, rs, coef, upd, st, zflag, nb2, nr2, flag;
   int* flag_host = nullptr;
   int part_blocks = 0;
+  bool fused_cgs2 = true;  // CGS2 as three k_basis_reg sweeps instead of four
   int64_t v_bytes = 0;
   // kernel timing of the level-0 triangular solve
   bool prof = false;
@@ -202,6 +203,7 @@ namespace {
 constexpr int kNT = 256;
 constexpr int kTrsvNT = 256;
 constexpr int kPMAX = 128;  // Krylov dimension + 1 limit of the shared-memory coefficient tables
+constexpr int kLrBlocks = 256;  // most workgroups of the low-rank partial sums
 
 template <typename V>
 DevPrec<V>& prec_of(pgs_dev_op_s* op);
@@ -326,6 +328,9 @@ void launch_trsv_ring_cg(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x,
 
 template <typename V>
 void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
+  // the kernel addresses a block's rows of all columns with 32-bit offsets
+  if ((int64_t)T.max_rows * nrhs * (int64_t)sizeof(V) >= (int64_t)INT32_MAX)
+    throw std::runtime_error("diagonal block too large for 32-bit offsets");
   int cg = std::min(nrhs, op->trsv_cg);
   while (cg > 1 && (size_t)T.window * cg * sizeof(V) > (size_t)op->lds_max) cg >>= 1;
   if (cg >= 4) launch_trsv_ring_cg<V, 4>(op, T, rhs, x, nrhs, s);
@@ -358,12 +363,22 @@ template <typename V>
 void launch_lowrank(pgs_ws_s* ws, const DevLevel& L, V* zl, int nrhs, hipStream_t s) {
   const int m = L.nrem, k = L.rank;
   if (k <= 0 || m <= 0) return;
-  const int rows_per_blk = 4096;
-  const int nblk = (m + rows_per_blk - 1) / rows_per_blk;
   V* part = ws->lrpart.as<V>();
   V* t = ws->lrt.as<V>();
-  k_lr_dot<V, kNT><<<nblk, kNT, 0, s>>>(m, k, nrhs, L.W.as<V>(), zl, part, rows_per_blk);
-  k_lr_final<V><<<nblocks_for((int64_t)k * nrhs, kNT), kNT, 0, s>>>(nblk, k, nrhs, part, t);
+  int nblk;
+  if (k <= 32) {
+    // register partial sums, up to kLrBlocks workgroups
+    nblk = (int)std::min<int64_t>(kLrBlocks, ((int64_t)m * nrhs + kNT - 1) / kNT);
+    const size_t lds = sizeof(V) * (size_t)(kNT / 64) * k * nrhs;
+    if (k <= 8) k_lr_dot_reg<V, kNT, 8><<<nblk, kNT, lds, s>>>(m, k, nrhs, L.W.as<V>(), zl, part);
+    else if (k <= 16) k_lr_dot_reg<V, kNT, 16><<<nblk, kNT, lds, s>>>(m, k, nrhs, L.W.as<V>(), zl, part);
+    else k_lr_dot_reg<V, kNT, 32><<<nblk, kNT, lds, s>>>(m, k, nrhs, L.W.as<V>(), zl, part);
+  } else {
+    const int rows_per_blk = std::max(4096, (m + kLrBlocks - 1) / kLrBlocks);
+    nblk = (m + rows_per_blk - 1) / rows_per_blk;
+    k_lr_dot<V, kNT><<<nblk, kNT, 0, s>>>(m, k, nrhs, L.W.as<V>(), zl, part, rows_per_blk);
+  }
+  k_lr_final<V><<<(k * nrhs + 3) / 4, 256, 0, s>>>(nblk, k, nrhs, part, t);
   k_lr_update<V><<<nblocks_for((int64_t)m * nrhs, kNT), kNT, 0, s>>>(m, k, nrhs, L.WH.as<V>(), t, zl);
   HIPCHK(hipGetLastError());
 }
@@ -559,7 +574,7 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
       maxrank = std::max(maxrank, rk);
       maxm = std::max(maxm, n > 0 ? (int)(n - op->lev_ptr[l + 1]) : 0);
     }
-    const int lr_blocks = (maxm + 4095) / 4096 + 1;
+    const int lr_blocks = kLrBlocks + 1;
     ws->lrpart.alloc((size_t)std::max(1, lr_blocks * maxrank * nr) * sizeof(double));
     ws->lrt.alloc((size_t)std::max(1, maxrank * nr) * sizeof(double));
     // Krylov
@@ -570,6 +585,7 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
     // are summed by one wave per output
     const int64_t rows_per_blk = std::max<int64_t>(256, (n + 1023) / 1024);
     ws->part_blocks = (int)((n + rows_per_blk - 1) / rows_per_blk);
+    if (const char* v = std::getenv("PGS_FUSED_CGS2")) ws->fused_cgs2 = std::atoi(v) != 0;
     ws->part.alloc((size_t)ws->part_blocks * (kdim + 1) * nr * sizeof(double));
     ws->hbuf.alloc((size_t)(kdim + 1) * nr * sizeof(double));
     ws->h2buf.alloc((size_t)(kdim + 1) * nr * sizeof(double));
@@ -668,6 +684,58 @@ void multi_axpy(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64
   HIPCHK(hipGetLastError());
 }
 
+constexpr int kBasisNT = 256;
+constexpr int kBasisPM = 64;  // most basis vectors k_basis_reg takes
+
+template <int PM, int MODE, int LPE>
+void launch_basis_mode(int lds_max, int nblk, size_t lds, int64_t n, int nrhs, int p, const double* V, int64_t ldv,
+                       const double* h, double* w, double* part, hipStream_t s) {
+  static const hipError_t attr = hipFuncSetAttribute((const void*)k_basis_reg<kBasisNT, PM, MODE, LPE>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+  HIPCHK(attr);
+  k_basis_reg<kBasisNT, PM, MODE, LPE><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, h, w, part);
+}
+
+size_t basis_lds(int p, int nrhs, int mode) {
+  const int q = mode == 2 ? 1 : p;
+  return sizeof(double) * ((mode ? (size_t)p * nrhs : 0) + (size_t)(kBasisNT / kWave) * q * nrhs);
+}
+
+// k_basis_reg takes p basis vectors of nrhs columns in every mode
+bool basis_fits(const pgs_ws_s* ws, int p, int nrhs) {
+  return p <= kBasisPM && basis_lds(p, nrhs, 1) <= (size_t)ws->op->lds_max;
+}
+
+// One sweep of k_basis_reg over V_0..V_{p-1} (basis_fits must hold): mode 0
+// out[j, c] = V_j^T w; mode 1 w -= V h, then out[j, c] = V_j^T w; mode 2
+// w -= V h, then out[c] = ||w(:, c)||^2.  Past 32 vectors the dot sweeps
+// spread an element over two lanes (32 basis values each) to keep register
+// use, and with it occupancy, at the level of the 32-vector case.
+void basis_reg(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64_t ldv, const double* h, double* w,
+               double* out, int mode, hipStream_t s) {
+  const int q = mode == 2 ? 1 : p;
+  const size_t lds = basis_lds(p, nrhs, mode);
+  const int lds_max = ws->op->lds_max;
+  const bool two = p > 32 && mode != 2 && nrhs <= 32;
+  const int64_t nthr = (n * nrhs * (two ? 2 : 1) + kBasisNT - 1) / kBasisNT;
+  const int nblk = (int)std::min<int64_t>(ws->part_blocks, nthr);
+  double* part = ws->part.as<double>();
+#define PGS_BASIS(PM, LPE)                                                                                   \
+  do {                                                                                                       \
+    if (mode == 0) launch_basis_mode<PM, 0, LPE>(lds_max, nblk, lds, n, nrhs, p, V, ldv, h, w, part, s);     \
+    else if (mode == 1) launch_basis_mode<PM, 1, LPE>(lds_max, nblk, lds, n, nrhs, p, V, ldv, h, w, part, s); \
+    else launch_basis_mode<PM, 2, 1>(lds_max, nblk, lds, n, nrhs, p, V, ldv, h, w, part, s);                \
+  } while (0)
+  if (p <= 8) PGS_BASIS(8, 1);
+  else if (p <= 16) PGS_BASIS(16, 1);
+  else if (p <= 32) PGS_BASIS(32, 1);
+  else if (two) PGS_BASIS(32, 2);
+  else PGS_BASIS(64, 1);
+#undef PGS_BASIS
+  k_dot_final_wave<<<(q * nrhs + 3) / 4, 256, 0, s>>>(nblk, q, nrhs, part, out, 0);
+  HIPCHK(hipGetLastError());
+}
+
 template <typename VZ>
 int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int nrhs, const pgs_fgmres_opts_t& o,
                 pgs_fgmres_result_t* res, hipStream_t s) {
@@ -727,11 +795,18 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int 
       double* t2 = ws->t2.as<double>();
       if (o.orth == PGS_ORTH_CGS2) {
         // PargemslrCgs2 (matrixops.cpp:10588-10709): two projection passes, the
-        // norm of the result accumulated by the second update
-        multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h, 0, s);
-        multi_axpy(ws, n, nrhs, i, V, ldv, h, w, nullptr, s);
-        multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h2, 0, s);
-        multi_axpy(ws, n, nrhs, i, V, ldv, h2, w, t2, s);
+        // norm of the result accumulated by the second update.  Three sweeps
+        // over the basis: h = V^T w; w -= V h fused with h2 = V^T w; w -= V h2.
+        if (ws->fused_cgs2 && basis_fits(ws, i, nrhs)) {
+          basis_reg(ws, n, nrhs, i, V, ldv, nullptr, w, h, 0, s);
+          basis_reg(ws, n, nrhs, i, V, ldv, h, w, h2, 1, s);
+          basis_reg(ws, n, nrhs, i, V, ldv, h2, w, t2, 2, s);
+        } else {
+          multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h, 0, s);
+          multi_axpy(ws, n, nrhs, i, V, ldv, h, w, nullptr, s);
+          multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h2, 0, s);
+          multi_axpy(ws, n, nrhs, i, V, ldv, h2, w, t2, s);
+        }
         k_dot_final_wave<<<(i * nrhs + 3) / 4, 256, 0, s>>>(1, i, nrhs, h2, h, 1);
       } else {
         // PargemslrMgs (matrixops.cpp:10728-10807): one (dot, axpy) pair per basis vector

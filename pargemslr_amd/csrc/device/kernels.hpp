@@ -125,13 +125,57 @@ __global__ __launch_bounds__(NT) void k_lr_dot(int m, int k, int nrhs, const V* 
   }
 }
 
+// Stage 1 with the k partial sums of an element in registers (KM >= k,
+// loops unrolled to KM with the uniform guard j < k): every lane walks the
+// elements g = (i, c) of z with a grid stride and adds W[i, j] * z[i, c] to
+// sum j; the sums are then reduced over the lanes of a column and over the
+// waves into part[blk * k * nrhs + j * nrhs + c].  Dynamic LDS:
+// (NT / 64) * k * nrhs values of V.
+template <typename V, int NT, int KM>
+__global__ __launch_bounds__(NT) void k_lr_dot_reg(int m, int k, int nrhs, const V* __restrict__ W,
+                                                   const V* __restrict__ z, V* __restrict__ part) {
+  extern __shared__ unsigned char lr_smem[];
+  V* red = reinterpret_cast<V*>(lr_smem);
+  const int tid = threadIdx.x;
+  V acc[KM];
+#pragma unroll
+  for (int j = 0; j < KM; ++j) acc[j] = 0;
+  const int64_t total = (int64_t)m * nrhs;
+  for (int64_t g = (int64_t)blockIdx.x * NT + tid; g < total; g += (int64_t)gridDim.x * NT) {
+    const V zv = z[g];
+    const V* wr = W + (g / nrhs) * k;
+#pragma unroll
+    for (int j = 0; j < KM; ++j)
+      if (j < k) acc[j] += wr[j] * zv;
+  }
+  const int wid = tid / 64, lane = tid % 64;
+#pragma unroll
+  for (int j = 0; j < KM; ++j) {
+    if (j < k) {
+      V s = acc[j];
+      for (int o = 32; o >= nrhs; o >>= 1) s += __shfl_xor(s, o);
+      if (lane < nrhs) red[(wid * k + j) * nrhs + lane] = s;
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < k * nrhs; t += NT) {
+    V s = 0;
+    for (int q = 0; q < NT / 64; ++q) s += red[q * k * nrhs + t];
+    part[(int64_t)blockIdx.x * k * nrhs + t] = s;
+  }
+}
+
 template <typename V>
 __global__ void k_lr_final(int nblk, int k, int nrhs, const V* __restrict__ part, V* __restrict__ t) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= k * nrhs) return;
+  // one wave per output, four per workgroup of 256; fixed summation order
+  const int g = blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (g >= k * nrhs) return;  // whole wave exits together
   V s = 0;
-  for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * k * nrhs + g];
-  t[g] = s;
+  for (int b = lane; b < nblk; b += 64) s += part[(int64_t)b * k * nrhs + g];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) t[g] = s;
 }
 
 template <typename V>
@@ -248,6 +292,91 @@ __global__ __launch_bounds__(NT) void k_multi_axpy_norm(int64_t n, int nrhs, int
     double s = 0;
     for (int q = 0; q < NT / kWave; ++q) s += red[q * nrhs + tid];
     part[(int64_t)blockIdx.x * nrhs + tid] = s;
+  }
+}
+
+// One classical Gram-Schmidt sweep over the basis V_0..V_{p-1} with an
+// element's basis values and the partial sums held in registers.  LPE lanes
+// share an element (consecutive lanes); lane `sub` of an element holds V_j for
+// j = u * LPE + sub, u < pu = ceil(p / LPE) <= PM (compile-time bound; loops
+// are unrolled to PM with the wave-uniform guard u < pu, and all loads of an
+// element are issued before any is used).  With LPE = 2 and p odd, lane 1
+// reads V_p (which must be readable) and gives it weight zero.
+//   MODE 0: d[j, c] = sum_rows V_j[r, c] * w[r, c]
+//   MODE 1: w -= sum_j V_j h[j, :], then d as in mode 0 with the updated w
+//           from the same registers (V is read once for both)
+//   MODE 2: w -= sum_j V_j h[j, :], then d[0, c] = sum_rows w[r, c]^2
+// The update sums V_j h[j, :] in increasing j within a lane (with LPE = 2 the
+// two lanes' sums are added) and subtracts the sum.  Every lane walks the
+// elements (blockIdx.x * NT + tid) / LPE + k * gridDim.x * NT / LPE; at the
+// end the sums are reduced over the lanes of a column, then over the waves,
+// into part[blk * q * nrhs + j * nrhs + c], q = p (modes 0, 1) or 1 (mode 2),
+// in a fixed order (deterministic).  Requires nrhs * LPE <= 64.
+// Dynamic LDS: (MODE ? p * nrhs : 0) + (NT / 64) * q * nrhs doubles.
+template <int NT, int PM, int MODE, int LPE>
+__global__ __launch_bounds__(NT) void k_basis_reg(int64_t n, int nrhs, int p, const double* __restrict__ V,
+                                                  int64_t ldv, const double* __restrict__ h, double* __restrict__ w,
+                                                  double* __restrict__ part) {
+  static_assert(MODE != 2 || LPE == 1, "the norm sweep uses one lane per element");
+  constexpr int NA = MODE == 2 ? 1 : PM;  // partial sums per lane
+  const int q = MODE == 2 ? 1 : p;        // outputs per column
+  extern __shared__ double sm[];
+  double* hs = sm;
+  double* red = sm + (MODE ? p * nrhs : 0);
+  const int tid = threadIdx.x;
+  if (MODE)
+    for (int t = tid; t < p * nrhs; t += NT) hs[t] = h[t];
+  __syncthreads();
+  const int sub = tid % LPE;
+  const int c = (tid / LPE) % nrhs;
+  const int pu = (p + LPE - 1) / LPE;
+  const double* Vl = V + (int64_t)sub * ldv;
+  double acc[NA];
+#pragma unroll
+  for (int u = 0; u < NA; ++u) acc[u] = 0;
+  const int64_t total = n * nrhs;
+  const int64_t stride = (int64_t)gridDim.x * (NT / LPE);
+  for (int64_t g = ((int64_t)blockIdx.x * NT + tid) / LPE; g < total; g += stride) {
+    double wv = w[g];
+    double v[PM];
+#pragma unroll
+    for (int u = 0; u < PM; ++u)
+      if (u < pu) v[u] = Vl[(int64_t)(u * LPE) * ldv + g];
+    if constexpr (MODE != 0) {
+      double sv = 0;
+#pragma unroll
+      for (int u = 0; u < PM; ++u)
+        if (u < pu) {
+          const int j = u * LPE + sub;
+          sv += v[u] * (j < p ? hs[j * nrhs + c] : 0.0);
+        }
+      if constexpr (LPE == 2) sv += __shfl_xor(sv, 1);
+      wv -= sv;
+      if (sub == 0) w[g] = wv;
+    }
+    if constexpr (MODE == 2) {
+      acc[0] += wv * wv;
+    } else {
+#pragma unroll
+      for (int u = 0; u < PM; ++u)
+        if (u < pu) acc[u] += v[u] * wv;
+    }
+  }
+  const int wid = tid / kWave, lane = tid % kWave;
+#pragma unroll
+  for (int u = 0; u < NA; ++u) {
+    if (u < (MODE == 2 ? 1 : pu)) {
+      const int j = MODE == 2 ? 0 : u * LPE + sub;
+      double s = acc[u];
+      for (int m = kWave / 2; m >= nrhs * LPE; m >>= 1) s += __shfl_xor(s, m);
+      if (lane < nrhs * LPE && j < q) red[(wid * q + j) * nrhs + lane / LPE] = s;
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < q * nrhs; t += NT) {
+    double s = 0;
+    for (int k = 0; k < NT / kWave; ++k) s += red[k * q * nrhs + t];
+    part[(int64_t)blockIdx.x * q * nrhs + t] = s;
   }
 }
 

@@ -5,10 +5,14 @@
 // Layout (built on the host by trsv_pack): per diagonal block a sequence of
 // steps (level sets, L sweep then U sweep); step s is
 //   desc[s] = {row_off, ent_off, nrows, width}   (block-relative offsets)
-//   ent[ent_off + j * nrows + t] = (column, value) of slot j of row t
+//   ent[ent_off + t * width + j] = (column, value) of slot j of row t
 //   ri[row_off + t]              = (row, 1/pivot)   (1/pivot used on U steps)
 // Padded slots hold (0, 0).  x is first loaded with rhs and both sweeps run in
-// place, so padded slots only ever read finite values.
+// place, so padded slots only ever read finite values.  The pipelined kernels
+// let a lane read slot q + e * LPR of its row unclamped: past the row's width
+// that is a padded or a neighbouring entry (a valid column) or, at the end of
+// the block, a zero from an out-of-range buffer load, and its weight is
+// replaced by zero.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -117,7 +121,7 @@ __global__ __launch_bounds__(NT) void k_trsv_direct(const int* __restrict__ blk_
       const TPair<V> rr = rib[d.x + t];
       V acc = xb[(int64_t)rr.col * nrhs + c];
       for (int j = 0; j < d.w; ++j) {
-        const TPair<V> p = eb[d.y + j * d.z + t];
+        const TPair<V> p = eb[d.y + t * d.w + j];
         acc -= p.val * xb[(int64_t)p.col * nrhs + c];
       }
       if (upper) acc *= rr.val;
@@ -172,10 +176,9 @@ __global__ __launch_bounds__(NT) void k_trsv_pipe(int nblocks, const int* __rest
     dd[K] = d_;                                                                 \
     const int tt_ = min(t0, max(d_.z - 1, 0));                                  \
     pr[K] = bload_pair(rs_ri, (d_.x + tt_) * PS, V(0));                         \
-    _Pragma("unroll") for (int e_ = 0; e_ < EPL; ++e_) {                        \
-      const int j_ = min(q + e_ * LPR, max(d_.w - 1, 0));                       \
-      pe[K][e_] = bload_pair(rs_ent, (d_.y + j_ * d_.z + tt_) * PS, V(0));      \
-    }                                                                           \
+    const int eb_ = (d_.y + tt_ * d_.w + q) * PS;                               \
+    _Pragma("unroll") for (int e_ = 0; e_ < EPL; ++e_)                          \
+      pe[K][e_] = bload_pair(rs_ent, eb_ + e_ * LPR * PS, V(0));                \
   } while (0)
 
 #pragma unroll
@@ -270,12 +273,19 @@ __device__ __forceinline__ double bload_val(__amdgpu_buffer_rsrc_t r, int off, d
   auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
   return __hiloint2double((int)v[1], (int)v[0]);
 }
+// Buffer stores (vector memory): writes outside the resource are dropped.
+__device__ __forceinline__ void bstore_val(__amdgpu_buffer_rsrc_t r, int off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
+}
+__device__ __forceinline__ void bstore_val(__amdgpu_buffer_rsrc_t r, int off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, off, 0, 0);
+}
 
 template <typename V, int NT, int LPR, int EPL, int D, int CG, bool UPPER>
 __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int wmask, int nrhs, int c0,
                                            __amdgpu_buffer_rsrc_t rs_desc, __amdgpu_buffer_rsrc_t rs_ri,
                                            __amdgpu_buffer_rsrc_t rs_ent, __amdgpu_buffer_rsrc_t rs_src, V* xs,
-                                           V* __restrict__ dst) {
+                                           __amdgpu_buffer_rsrc_t rs_dst) {
   constexpr int NCL = CG >= LPR ? CG / LPR : 1;  // columns finalized per lane
   constexpr int PS = (int)sizeof(TPair<V>);
   constexpr int VS = (int)sizeof(V);
@@ -300,10 +310,9 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
     dd[K] = d_;                                                                 \
     const int tt_ = min(t0, max(d_.z - 1, 0));                                  \
     pr[K] = bload_pair(rs_ri, (d_.x + tt_) * PS, V(0));                         \
-    _Pragma("unroll") for (int e_ = 0; e_ < EPL; ++e_) {                        \
-      const int j_ = min(q + e_ * LPR, max(d_.w - 1, 0));                       \
-      pe[K][e_] = bload_pair(rs_ent, (d_.y + j_ * d_.z + tt_) * PS, V(0));      \
-    }                                                                           \
+    const int eb_ = (d_.y + tt_ * d_.w + q) * PS;                               \
+    _Pragma("unroll") for (int e_ = 0; e_ < EPL; ++e_)                          \
+      pe[K][e_] = bload_pair(rs_ent, eb_ + e_ * LPR * PS, V(0));                \
   } while (0)
 #define PGS_RING_RHS(K)                                                           \
   do {                                                                            \
@@ -325,42 +334,56 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
     for (int k = 0; k < D; ++k) {
       const int s = base + k;
       const int4 d = dd[k];
+      // gather this step's slots from the ring and take what the arithmetic
+      // needs out of the stage registers
+      RingRow<V, CG> xv[EPL];
+      V wgt[EPL];
+#pragma unroll
+      for (int e = 0; e < EPL; ++e) {
+        xv[e] = reinterpret_cast<const RingRow<V, CG>*>(xs)[pe[k][e].col];
+        wgt[e] = (q + e * LPR < d.w) ? pe[k][e].val : (V)0;
+      }
+      // keep the gathers ahead of the refill loads below
+      __builtin_amdgcn_sched_barrier(0);
+      const int tt = min(t0, max(d.z - 1, 0));
+      const int slot = (d.x + tt) & wmask;
+      const int row = pr[k].col;
+      const V dv = pr[k].val;
+      const bool wr = s < s_end && t0 < d.z && writer;
+      V xin[NCL];
+#pragma unroll
+      for (int j = 0; j < NCL; ++j) xin[j] = xr[k][j];
+      // refill this stage with step s + D and load the right-hand side of
+      // step s + D/2, whose row pair arrived D/2 steps ago; the loads are in
+      // flight while the LDS gathers return
+      PGS_RING_STAGE(k, dn[k]);
+      dn[k] = bload_i4(rs_desc, (s - sbeg + 2 * D) * 16);
+      PGS_RING_RHS((k + D / 2) % D);
       V acc[CG];
 #pragma unroll
       for (int c = 0; c < CG; ++c) acc[c] = 0;
 #pragma unroll
       for (int e = 0; e < EPL; ++e) {
-        const RingRow<V, CG> xv = reinterpret_cast<const RingRow<V, CG>*>(xs)[pe[k][e].col];
-        const V w = (q + e * LPR < d.w) ? pe[k][e].val : (V)0;
 #pragma unroll
-        for (int c = 0; c < CG; ++c) acc[c] += w * xv.v[c];
+        for (int c = 0; c < CG; ++c) acc[c] += wgt[e] * xv[e].v[c];
       }
 #pragma unroll
       for (int c = 0; c < CG; ++c) {
         if constexpr (LPR == 4) acc[c] = quad_sum(acc[c]);
         else if constexpr (LPR == 2) acc[c] = pair_sum(acc[c]);
       }
-      const int tt = min(t0, max(d.z - 1, 0));
-      const int slot = (d.x + tt) & wmask;
-      const int row = pr[k].col;
-      const V dv = pr[k].val;
-      if (s < s_end && t0 < d.z && writer) {
+      if (wr) {
 #pragma unroll
         for (int j = 0; j < NCL; ++j) {
           V a = acc[0];
 #pragma unroll
           for (int c = 1; c < CG; ++c) a = (c == cl[j]) ? acc[c] : a;
-          V v = xr[k][j] - a;
+          V v = xin[j] - a;
           if (UPPER) v *= dv;
           xs[slot * CG + cl[j]] = v;
-          dst[(int64_t)row * nrhs + c0 + cl[j]] = v;
+          bstore_val(rs_dst, (row * nrhs + c0 + cl[j]) * VS, v);
         }
       }
-      // refill this stage with step s + D, and load the right-hand side of
-      // step s + D/2, whose row pair arrived D/2 steps ago
-      PGS_RING_STAGE(k, dn[k]);
-      dn[k] = bload_i4(rs_desc, (s - sbeg + 2 * D) * 16);
-      PGS_RING_RHS((k + D / 2) % D);
       __syncthreads();
     }
   }
@@ -391,9 +414,9 @@ __global__ __launch_bounds__(NT) void k_trsv_ring(int nblocks, int wmask, const 
   for (int i = threadIdx.x; i < (wmask + 1) * CG; i += NT) xs[i] = 0;
   __syncthreads();
   ring_sweep<V, NT, LPR, EPL, D, CG, false>(sbeg, sU, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
-                                            make_rsrc(rhs + (int64_t)r0 * nrhs, vbytes), xs, xb);
+                                            make_rsrc(rhs + (int64_t)r0 * nrhs, vbytes), xs, make_rsrc(xb, vbytes));
   ring_sweep<V, NT, LPR, EPL, D, CG, true>(sU, send, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
-                                           make_rsrc(xb, vbytes), xs, xb);
+                                           make_rsrc(xb, vbytes), xs, make_rsrc(xb, vbytes));
 }
 
 }  // namespace dev

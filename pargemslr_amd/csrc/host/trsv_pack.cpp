@@ -107,8 +107,8 @@ void emit_sweep(const Csr<T>& M, const std::vector<T>* dinv, const SweepOrder& o
       int j = 0;
       for (int k = M.ia[i]; k < M.ia[i + 1]; ++k, ++j) {
         const int c = M.ja[k];
-        P.col[e + (size_t)j * nr + t] = wmask ? ((base + o.pos[c]) & wmask) : c;
-        P.val[e + (size_t)j * nr + t] = M.a[k];
+        P.col[e + (size_t)t * width + j] = wmask ? ((base + o.pos[c]) & wmask) : c;
+        P.val[e + (size_t)t * width + j] = M.a[k];
       }
       P.nnz_real += M.ia[i + 1] - M.ia[i];
     }

@@ -11,9 +11,9 @@
 // of levels.  The one whose values stay alive over fewer sweep positions is
 // used (see window below).  A level set with more than `max_rows` rows is cut
 // into consecutive steps (rows of one level are independent, so any split is
-// valid).  Each step is stored as a small column-major ELL slab of
+// valid).  Each step is stored as a small row-major ELL slab of
 // (column, value) pairs:
-//   entry (row t of the step, slot j) at ent_off + j * nrows + t,
+//   entry (row t of the step, slot j) at ent_off + t * width + j,
 // padded with (column 0, value 0), plus one (row, 1/pivot) pair per row.
 // Offsets are relative to the block's first entry / first row pair so that a
 // device kernel can address a block with 32-bit offsets.

@@ -144,3 +144,67 @@ def test_fgmres_unpreconditioned_matches_oracle(c1, gpu):
                                  orth=pg.PGS_ORTH_MGS)
     assert it == it_o == 60 and status == pg.capi.PGS_ERR_NOT_CONVERGED
     assert rr == pytest.approx(rr_o, rel=1e-6)
+
+
+@pytest.mark.parametrize("env", [{}, {"PGS_TRSV_CG": "1"}, {"PGS_TRSV_CG": "2"}, {"PGS_TRSV_NT": "256"},
+                                 {"PGS_TRSV_VARIANT": "1"}],
+                         ids=["ring4", "ring1", "ring2", "pipelined", "direct"])
+def test_apply_trsv_kernels(env, small, gpu, monkeypatch):
+    """Every triangular-solve kernel (ring with 1, 2 or 4 columns per
+    workgroup, the pipelined kernel, the direct kernel) gives the oracle's
+    apply, for one and for eight right-hand sides."""
+    A, G, _, _ = small
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    D = pg.DevOp(G)
+    W = pg.Workspace(D, max_nrhs=8, kdim=2)
+    O = P.oracle_of(G)
+    nr = 8
+    V = np.stack([pg.rng_normal(60 + c, A.n) for c in range(nr)], axis=1)
+    refs = [O.apply(V[:, c]) for c in range(nr)]
+    for prec, tol in ((pg.PGS_PREC_FP64, 1e-12), (pg.PGS_PREC_FP32, 1e-4)):
+        Zd = _dev(np.zeros((A.n, nr)), gpu)
+        W.apply(_dev(V, gpu), Zd, nrhs=nr, prec=prec)
+        Z = _host(Zd)
+        for c in range(nr):
+            assert np.linalg.norm(Z[:, c] - refs[c]) <= tol * np.linalg.norm(refs[c]), (env, prec, c)
+        zd = _dev(np.zeros(A.n), gpu)
+        W.apply(_dev(V[:, 0], gpu), zd, prec=prec)
+        assert np.linalg.norm(_host(zd) - refs[0]) <= tol * np.linalg.norm(refs[0]), (env, prec)
+
+
+def test_apply_wide_rows(gpu):
+    """Exact block factors (no dropping) have rows wider than the pipelined
+    kernels take: the direct kernel solves them."""
+    A = pg.laplacian(10, 10, 10)
+    G = P.setup(A, 2, 4, 0.0, 10000, 0)
+    assert max(int(np.diff(d["L"][0]).max()) for d in G.levels()) > 20
+    D = pg.DevOp(G)
+    W = pg.Workspace(D, max_nrhs=2, kdim=2)
+    O = P.oracle_of(G)
+    V = np.stack([pg.rng_normal(70 + c, A.n) for c in range(2)], axis=1)
+    Zd = _dev(np.zeros((A.n, 2)), gpu)
+    W.apply(_dev(V, gpu), Zd, nrhs=2, prec=pg.PGS_PREC_FP64)
+    Z = _host(Zd)
+    for c in range(2):
+        ref = O.apply(V[:, c])
+        assert np.linalg.norm(Z[:, c] - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("nr", [4, 64])
+def test_fgmres_cgs2_long_basis(nr, c1, gpu):
+    """Unpreconditioned FGMRES(50) with CGS2: the basis grows past 32 and 64
+    vectors' worth of register sweeps; every column's Givens residual after 60
+    iterations equals the oracle's (GMRES residuals are unique)."""
+    A = c1[0]
+    D = pg.DevOp(c1[1])
+    W = pg.Workspace(D, max_nrhs=nr, kdim=50)
+    B = np.stack([pg.rng_normal(80 + c, A.n) for c in range(nr)], axis=1)
+    Xd = _dev(np.zeros((A.n, nr)), gpu)
+    res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=50, maxits=60, tol=1e-12, use_precond=False,
+                   orth=pg.PGS_ORTH_CGS2)
+    for c in (0, 1, nr - 1):
+        _, st, it_o, rr_o, _ = orc.fgmres(A, B[:, c], prec=None, kdim=50, maxits=60, tol=1e-12, orth=1)
+        st_g, it_g, rr_g = res[c]
+        assert it_g == it_o == 60
+        assert rr_g == pytest.approx(rr_o, rel=1e-6), c
