@@ -162,10 +162,14 @@ int pgs_dev_op_create(pgs_gemslr_t op, int device, int prec_mask, pgs_dev_op_t* 
 int pgs_dev_op_destroy(pgs_dev_op_t dop);
 /* device bytes held by the operator */
 int pgs_dev_op_bytes(pgs_dev_op_t dop, int64_t* bytes);
+/* order n and device of the operator */
+int pgs_dev_op_dims(pgs_dev_op_t dop, int* n, int* device);
 
 /* Workspace for up to max_nrhs right-hand sides and Krylov dimension kdim. */
 int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out);
 int pgs_ws_destroy(pgs_ws_t ws);
+/* device bytes pgs_ws_create would allocate for these arguments */
+int pgs_ws_estimate(pgs_dev_op_t dop, int max_nrhs, int kdim, int64_t* bytes);
 
 /* y = alpha * A * x + beta * y  (CsrMatrixClass::MatVec('N', ...),
  * ParGeMSLR/SRC/matrices/csr_matrix.cpp:1340-1365), FP64 / complex128. */
@@ -219,6 +223,47 @@ int pgs_ws_profile_read(pgs_ws_t ws, pgs_kernel_profile_t* out);
 const char* pgs_strerror(int code);
 /* message of the last HIP error seen by this thread */
 const char* pgs_last_hip_error(void);
+
+/* ---------------- request server (continuous batching) ----------------
+ * No counterpart in the reference (one right-hand side per Solve call).
+ * Several uploaded operators stay resident; requests of 1..max_batch
+ * right-hand sides are queued in arrival order.  Each of `slots` workers (one
+ * stream and one host thread each) takes the oldest request plus every later
+ * queued request for the same operator that fits in max_batch columns and
+ * solves them together with pgs_fgmres, so work of several batches overlaps on
+ * the device and requests arriving meanwhile join the next free worker.  A
+ * worker keeps one workspace per operator it has served; it admits a new one
+ * only when the device's free memory minus mem_reserve holds it, dropping its
+ * own idle workspaces and then waiting for other batches if needed. */
+typedef struct pgs_server_s* pgs_server_t;
+typedef struct {
+  int max_batch;       /* most right-hand sides per batch (power of two <= 64) */
+  int slots;           /* batches in flight at once (1..16) */
+  int64_t mem_reserve; /* device bytes left free when admitting a workspace */
+  int kdim, maxits;    /* FGMRES(kdim), iteration cap per column */
+  double tol;          /* relative tolerance on ||b - A x|| / ||b|| */
+  int prec, orth;      /* PGS_PREC_* of the apply, PGS_ORTH_* */
+  int check_every;     /* as in pgs_fgmres_opts_t */
+} pgs_server_opts_t;
+typedef struct {
+  int status;          /* 0 when every column converged, else the first column's error code */
+  int nrhs;            /* columns of the request */
+  int batch_nrhs;      /* columns of the batch it was solved in */
+  double queue_ms;     /* submit to batch start */
+  double solve_ms;     /* batch start to results written */
+  double total_ms;     /* submit to results written */
+} pgs_request_info_t;
+int pgs_server_create(int device, const pgs_server_opts_t* opts, pgs_server_t* out);
+/* the operator must outlive the server */
+int pgs_server_add_operator(pgs_server_t srv, pgs_dev_op_t dop, int* op_id);
+/* b, x: host arrays n x nrhs, column c at [c * n]; iters: nrhs ints or NULL.
+ * All three stay owned by the caller and must stay valid until
+ * pgs_server_wait returns for the request; x and iters are written then. */
+int pgs_server_submit(pgs_server_t srv, int op_id, int nrhs, const double* b, double* x, int* iters, int64_t* req);
+/* blocks until the request is done; a request can be waited for once */
+int pgs_server_wait(pgs_server_t srv, int64_t req, pgs_request_info_t* info);
+/* finishes every queued request, then stops the workers */
+int pgs_server_destroy(pgs_server_t srv);
 
 /* ---------------- problem generators (test and benchmark inputs) ---------------- */
 

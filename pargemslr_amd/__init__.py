@@ -7,6 +7,6 @@ ctypes.  Importing does not touch the GPU.
 """
 from .capi import (  # noqa: F401
     PGS_ORTH_CGS2, PGS_ORTH_MGS, PGS_PREC_FP32, PGS_PREC_FP64, CsrHost, DeviceArray, DevOp, Gemslr, PgsError,
-    Stream, Workspace, config_params, convdiff, default_params, device_count, laplacian, lib, rng_normal,
+    Server, Stream, Workspace, config_params, convdiff, default_params, device_count, laplacian, lib, rng_normal,
     rng_uniform, synchronize,
 )

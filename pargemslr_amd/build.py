@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libpargemslr_amd.so")
 
 HOST_SRC = ["host/partition.cpp", "host/rcm_ilut.cpp", "host/dense.cpp", "host/gemslr_setup.cpp",
             "host/gen.cpp", "host/trsv_pack.cpp", "host/capi_host.cpp", "host/capi_gen.cpp"]
-HIP_SRC = ["device/dev_op.hip"]
+HIP_SRC = ["device/dev_op.hip", "device/server.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PGS_OFFLOAD_ARCH", "gfx950")

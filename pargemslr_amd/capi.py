@@ -67,6 +67,17 @@ class KernelProfile(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("total_bytes", C.c_double)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("max_batch", C.c_int), ("slots", C.c_int), ("mem_reserve", C.c_int64), ("kdim", C.c_int),
+                ("maxits", C.c_int), ("tol", C.c_double), ("prec", C.c_int), ("orth", C.c_int),
+                ("check_every", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("nrhs", C.c_int), ("batch_nrhs", C.c_int), ("queue_ms", C.c_double),
+                ("solve_ms", C.c_double), ("total_ms", C.c_double)]
+
+
 # every extern "C" entry point of include/pargemslr_amd.h
 EXPORTED = [
     "pgs_params_default", "pgs_gemslr_setup", "pgs_gemslr_destroy", "pgs_gemslr_info", "pgs_gemslr_level_view",
@@ -75,6 +86,8 @@ EXPORTED = [
     "pgs_last_hip_error", "pgs_gen_laplacian", "pgs_gen_convdiff", "pgs_free", "pgs_rng_uniform", "pgs_rng_normal",
     "pgs_ws_profile", "pgs_ws_profile_read", "pgs_dev_malloc", "pgs_dev_free", "pgs_memcpy_htod", "pgs_memcpy_dtoh",
     "pgs_memset_zero", "pgs_stream_create", "pgs_stream_destroy", "pgs_stream_synchronize", "pgs_device_synchronize",
+    "pgs_dev_op_dims", "pgs_ws_estimate", "pgs_server_create", "pgs_server_add_operator", "pgs_server_submit",
+    "pgs_server_wait", "pgs_server_destroy",
 ]
 
 _lib = None
@@ -129,6 +142,13 @@ def lib():
     L.pgs_stream_destroy.argtypes = [C.c_int, vp]
     L.pgs_stream_synchronize.argtypes = [C.c_int, vp]
     L.pgs_device_synchronize.argtypes = [C.c_int]
+    L.pgs_dev_op_dims.argtypes = [vp, ip, ip]
+    L.pgs_ws_estimate.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int64)]
+    L.pgs_server_create.argtypes = [C.c_int, C.POINTER(ServerOpts), C.POINTER(vp)]
+    L.pgs_server_add_operator.argtypes = [vp, vp, ip]
+    L.pgs_server_submit.argtypes = [vp, C.c_int, C.c_int, dp, dp, ip, C.POINTER(C.c_int64)]
+    L.pgs_server_wait.argtypes = [vp, C.c_int64, C.POINTER(RequestInfo)]
+    L.pgs_server_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -478,6 +498,62 @@ class Workspace:
         _chk(lib().pgs_fgmres(self.dop._h, self._h, _tptr(b), _tptr(x), nrhs, C.byref(o), res, _sh(stream)),
              "pgs_fgmres")
         return [(r.status, r.iters, r.rel_res) for r in res]
+
+
+class Server:
+    """Request server over resident operators (pgs_server_*): continuous
+    batching of solve requests; see include/pargemslr_amd.h."""
+
+    def __init__(self, device=0, max_batch=16, slots=3, mem_reserve=8 << 30, kdim=50, maxits=1000, tol=1e-8,
+                 prec=PGS_PREC_FP32, orth=PGS_ORTH_CGS2, check_every=1):
+        self.device = device
+        o = ServerOpts(max_batch, slots, mem_reserve, kdim, maxits, tol, prec, orth, check_every)
+        self._h = C.c_void_p()
+        self._ops = []
+        self._live = {}  # request id -> (b, x, iters) kept alive until waited for
+        _chk(lib().pgs_server_create(device, C.byref(o), C.byref(self._h)), "pgs_server_create")
+
+    def add_operator(self, dop):
+        i = C.c_int()
+        _chk(lib().pgs_server_add_operator(self._h, dop._h, C.byref(i)), "pgs_server_add_operator")
+        self._ops.append(dop)
+        return i.value
+
+    def submit(self, op_id, B):
+        """B: n x nrhs (or n) host array; returns a request id."""
+        B = np.asarray(B, dtype=np.float64)
+        if B.ndim == 1:
+            B = B[:, None]
+        n, nr = B.shape
+        Bc = np.asfortranarray(B)
+        X = np.zeros((n, nr), dtype=np.float64, order="F")
+        it = np.zeros(nr, dtype=np.int32)
+        r = C.c_int64()
+        _chk(lib().pgs_server_submit(self._h, op_id, nr, Bc.ctypes.data_as(C.POINTER(C.c_double)),
+                                     X.ctypes.data_as(C.POINTER(C.c_double)), _iptr(it), C.byref(r)),
+             "pgs_server_submit")
+        self._live[r.value] = (Bc, X, it)
+        return r.value
+
+    def wait(self, req):
+        """Returns (X, iterations per column, info dict)."""
+        info = RequestInfo()
+        _chk(lib().pgs_server_wait(self._h, req, C.byref(info)), "pgs_server_wait")
+        _, X, it = self._live.pop(req)
+        return X, it, dict(status=info.status, nrhs=info.nrhs, batch_nrhs=info.batch_nrhs, queue_ms=info.queue_ms,
+                           solve_ms=info.solve_ms, total_ms=info.total_ms)
+
+    def close(self):
+        if self._h:
+            lib().pgs_server_destroy(self._h)
+            self._h = C.c_void_p()
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def config_params(nlev, ncomp, droptol, rownnz, rank, kdim=50, maxits=1000, tol=1e-8, lr_tol_eig=1e-3,

@@ -319,7 +319,7 @@ void launch_trsv_ring_cg(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x,
   static const hipError_t attr =
       hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, op->lds_max);
   HIPCHK(attr);
-  const size_t lds = (size_t)T.window * CG * sizeof(V);
+  const size_t lds = (size_t)(T.window + 1) * CG * sizeof(V);  // ring + one dummy slot
   const unsigned grid = (unsigned)T.nblocks * (unsigned)(nrhs / CG);
   kern<<<grid, kRingNT, lds, s>>>(T.nblocks, T.window - 1, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
                                   T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), T.desc.as<int4>(),
@@ -332,7 +332,7 @@ void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
   if ((int64_t)T.max_rows * nrhs * (int64_t)sizeof(V) >= (int64_t)INT32_MAX)
     throw std::runtime_error("diagonal block too large for 32-bit offsets");
   int cg = std::min(nrhs, op->trsv_cg);
-  while (cg > 1 && (size_t)T.window * cg * sizeof(V) > (size_t)op->lds_max) cg >>= 1;
+  while (cg > 1 && (size_t)(T.window + 1) * cg * sizeof(V) > (size_t)op->lds_max) cg >>= 1;
   if (cg >= 4) launch_trsv_ring_cg<V, 4>(op, T, rhs, x, nrhs, s);
   else if (cg == 2) launch_trsv_ring_cg<V, 2>(op, T, rhs, x, nrhs, s);
   else launch_trsv_ring_cg<V, 1>(op, T, rhs, x, nrhs, s);
@@ -487,13 +487,12 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
     int max_window = 0;
     if (ring) {
       max_window = 1;
-      while (2 * max_window * (int)sizeof(double) <= op->lds_max) max_window *= 2;
+      while ((2 * max_window + 1) * (int)sizeof(double) <= op->lds_max) max_window *= 2;  // + dummy slot
     }
     std::vector<TrsvPack<double>> packs;
     for (int l = 0; l < H.nlev; ++l) {
-      packs.push_back(trsv_pack(H.levels[l], H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr, max_window));
-      if (packs.back().window > 0 && packs.back().max_width > kRingLPR * kRingEPL)
-        packs.back() = trsv_pack(H.levels[l], H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr, 0);
+      packs.push_back(trsv_pack(H.levels[l], H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr, max_window,
+                                kRingLPR * kRingEPL));
     }
     auto build = [&](auto& P, auto tag) {
       using V = decltype(tag);
@@ -541,6 +540,28 @@ int pgs_dev_op_destroy(pgs_dev_op_t dop) {
 int pgs_dev_op_bytes(pgs_dev_op_t dop, int64_t* bytes) {
   if (!dop || !bytes) return PGS_ERROR_INVALID_PARAM;
   *bytes = reinterpret_cast<pgs_dev_op_s*>(dop)->bytes;
+  return 0;
+}
+
+int pgs_dev_op_dims(pgs_dev_op_t dop, int* n, int* device) {
+  if (!dop || !n || !device) return PGS_ERROR_INVALID_PARAM;
+  *n = reinterpret_cast<pgs_dev_op_s*>(dop)->n;
+  *device = reinterpret_cast<pgs_dev_op_s*>(dop)->device;
+  return 0;
+}
+
+int pgs_ws_estimate(pgs_dev_op_t dop, int max_nrhs, int kdim, int64_t* bytes) {
+  if (!dop || !bytes || max_nrhs < 1 || max_nrhs > 64 || kdim < 1 || kdim + 1 > kPMAX) return PGS_ERROR_INVALID_PARAM;
+  const pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
+  int nr = 1;
+  while (nr < max_nrhs) nr <<= 1;
+  const int64_t n = op->n, vb = n * nr * 8;
+  int64_t b = 2 * vb + (2 * (int64_t)kdim + 1) * vb;  // r0, x0, V, Z
+  for (int l = 0; l < op->nlev; ++l) b += (n - op->lev_ptr[l]) * nr * 8;
+  b += (int64_t)1025 * (kdim + 1) * nr * 8;                // reduction partials
+  b += (int64_t)(kLrBlocks + 2) * 64 * nr * 8;             // low-rank partials
+  b += (int64_t)(kdim + 4) * (kdim + 1) * nr * 8 + 4096;   // Hessenberg, rotations, state
+  *bytes = b;
   return 0;
 }
 

@@ -30,10 +30,10 @@ struct alignas(8) TPair<float> {
   float val;
 };
 template <>
-struct alignas(16) TPair<double> {
+struct alignas(16) TPair<double> {  // value first: a 12-byte load takes (value, column)
+  double val;
   int col;
   int pad;
-  double val;
 };
 
 constexpr int kBufferRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit dwords, no swizzle
@@ -55,12 +55,17 @@ __device__ __forceinline__ TPair<float> bload_pair(__amdgpu_buffer_rsrc_t r, int
   return p;
 }
 __device__ __forceinline__ TPair<double> bload_pair(__amdgpu_buffer_rsrc_t r, int off, double) {
-  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  auto v = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
   TPair<double> p;
-  p.col = (int)v[0];
+  p.val = __hiloint2double((int)v[1], (int)v[0]);
+  p.col = (int)v[2];
   p.pad = 0;
-  p.val = __hiloint2double((int)v[3], (int)v[2]);
   return p;
+}
+// The first three fields of a step descriptor (row offset, entry offset, rows).
+__device__ __forceinline__ int3 bload_i3(__amdgpu_buffer_rsrc_t r, int off) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
+  return make_int3((int)v[0], (int)v[1], (int)v[2]);
 }
 
 // Sum over the 4 lanes of a quad with DPP quad permutations (xor 1, xor 2).
@@ -90,6 +95,15 @@ __device__ __forceinline__ float pair_sum(float v) {
   return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
 __device__ __forceinline__ double pair_sum(double v) { return v + quad_perm_f64(v, 1); }
+// The value of the lane q ^ 1 / q ^ 2 of the same quad.
+__device__ __forceinline__ float dpp_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_xor2(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double dpp_xor1(double v) { return quad_perm_f64(v, 1); }
+__device__ __forceinline__ double dpp_xor2(double v) { return quad_perm_f64(v, 2); }
 
 // Direct kernel: any width, nrhs columns (row-interleaved); one thread per
 // (row of the step, column), plain loads.  Used for levels whose rows are too
@@ -286,31 +300,34 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
                                            __amdgpu_buffer_rsrc_t rs_desc, __amdgpu_buffer_rsrc_t rs_ri,
                                            __amdgpu_buffer_rsrc_t rs_ent, __amdgpu_buffer_rsrc_t rs_src, V* xs,
                                            __amdgpu_buffer_rsrc_t rs_dst) {
-  constexpr int NCL = CG >= LPR ? CG / LPR : 1;  // columns finalized per lane
   constexpr int PS = (int)sizeof(TPair<V>);
   constexpr int VS = (int)sizeof(V);
+  constexpr int RW = LPR * EPL;  // slab width of every ring-packed step
   static_assert(D % 2 == 0, "stage depth must be even");
+  static_assert(LPR == 4 && (CG == 1 || CG == 2 || CG == 4), "quad lanes, 1, 2 or 4 columns");
+  constexpr int NCL = 1;  // columns finalized per lane
   const int tid = threadIdx.x;
   const int q = tid % LPR;
   const int t0 = tid / LPR;
-  int cl[NCL];  // this lane's columns within the group
-#pragma unroll
-  for (int j = 0; j < NCL; ++j) cl[j] = CG >= LPR ? q + j * LPR : min(q, CG - 1);
-  const bool writer = CG >= LPR || q < CG;
+  // after the reduction lane q holds column q % CG; lanes q < CG write it
+  int cl[NCL];
+  cl[0] = q % CG;
+  const bool writer = q < CG;
+  const int dummy = (wmask + 1) * CG;  // LDS slot written by lanes with nothing to write
 
   TPair<V> pe[D][EPL];
   TPair<V> pr[D];
   V xr[D][NCL];
-  int4 dd[D];
-  int4 dn[D];
+  int sw[D];   // ring slot of this lane's row in the step held by stage k, -1 past the step's rows
+  int3 dn[D];  // descriptor of the step stage k loads next
 
 #define PGS_RING_STAGE(K, DESC)                                                 \
   do {                                                                          \
-    const int4 d_ = (DESC);                                                     \
-    dd[K] = d_;                                                                 \
+    const int3 d_ = (DESC);                                                     \
     const int tt_ = min(t0, max(d_.z - 1, 0));                                  \
+    sw[K] = t0 < d_.z ? ((d_.x + tt_) & wmask) : -1;                            \
     pr[K] = bload_pair(rs_ri, (d_.x + tt_) * PS, V(0));                         \
-    const int eb_ = (d_.y + tt_ * d_.w + q) * PS;                               \
+    const int eb_ = (d_.y + tt_ * RW + q) * PS;                                 \
     _Pragma("unroll") for (int e_ = 0; e_ < EPL; ++e_)                          \
       pe[K][e_] = bload_pair(rs_ent, eb_ + e_ * LPR * PS, V(0));                \
   } while (0)
@@ -323,8 +340,8 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   const int rel = s_begin - sbeg;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    PGS_RING_STAGE(k, bload_i4(rs_desc, (rel + k) * 16));
-    dn[k] = bload_i4(rs_desc, (rel + k + D) * 16);
+    PGS_RING_STAGE(k, bload_i3(rs_desc, (rel + k) * 16));
+    dn[k] = bload_i3(rs_desc, (rel + k + D) * 16);
   }
 #pragma unroll
   for (int k = 0; k < D / 2; ++k) PGS_RING_RHS(k);
@@ -333,7 +350,6 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       const int s = base + k;
-      const int4 d = dd[k];
       // gather this step's slots from the ring and take what the arithmetic
       // needs out of the stage registers
       RingRow<V, CG> xv[EPL];
@@ -341,15 +357,14 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
 #pragma unroll
       for (int e = 0; e < EPL; ++e) {
         xv[e] = reinterpret_cast<const RingRow<V, CG>*>(xs)[pe[k][e].col];
-        wgt[e] = (q + e * LPR < d.w) ? pe[k][e].val : (V)0;
+        wgt[e] = pe[k][e].val;  // padded slots weigh zero
       }
       // keep the gathers ahead of the refill loads below
       __builtin_amdgcn_sched_barrier(0);
-      const int tt = min(t0, max(d.z - 1, 0));
-      const int slot = (d.x + tt) & wmask;
+      const int slot = sw[k];
       const int row = pr[k].col;
       const V dv = pr[k].val;
-      const bool wr = s < s_end && t0 < d.z && writer;
+      const bool wr = s < s_end && slot >= 0 && writer;
       V xin[NCL];
 #pragma unroll
       for (int j = 0; j < NCL; ++j) xin[j] = xr[k][j];
@@ -357,7 +372,7 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
       // step s + D/2, whose row pair arrived D/2 steps ago; the loads are in
       // flight while the LDS gathers return
       PGS_RING_STAGE(k, dn[k]);
-      dn[k] = bload_i4(rs_desc, (s - sbeg + 2 * D) * 16);
+      dn[k] = bload_i3(rs_desc, (s - sbeg + 2 * D) * 16);
       PGS_RING_RHS((k + D / 2) % D);
       V acc[CG];
 #pragma unroll
@@ -367,23 +382,28 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
 #pragma unroll
         for (int c = 0; c < CG; ++c) acc[c] += wgt[e] * xv[e].v[c];
       }
-#pragma unroll
-      for (int c = 0; c < CG; ++c) {
-        if constexpr (LPR == 4) acc[c] = quad_sum(acc[c]);
-        else if constexpr (LPR == 2) acc[c] = pair_sum(acc[c]);
+      // sum over the quad, scattered so that lane q ends with column q % CG
+      V a;
+      if constexpr (CG == 4) {
+        // xor 1: keep columns with bit 0 = q bit 0, send the other two
+        const bool b0 = q & 1, b1 = q & 2;
+        const V k0 = (b0 ? acc[1] : acc[0]) + dpp_xor1(b0 ? acc[0] : acc[1]);
+        const V k1 = (b0 ? acc[3] : acc[2]) + dpp_xor1(b0 ? acc[2] : acc[3]);
+        // xor 2: keep the pair with bit 1 = q bit 1
+        a = (b1 ? k1 : k0) + dpp_xor2(b1 ? k0 : k1);
+      } else if constexpr (CG == 2) {
+        const bool b0 = q & 1;
+        const V k0 = (b0 ? acc[1] : acc[0]) + dpp_xor1(b0 ? acc[0] : acc[1]);
+        a = k0 + dpp_xor2(k0);
+      } else {
+        a = quad_sum(acc[0]);
       }
-      if (wr) {
-#pragma unroll
-        for (int j = 0; j < NCL; ++j) {
-          V a = acc[0];
-#pragma unroll
-          for (int c = 1; c < CG; ++c) a = (c == cl[j]) ? acc[c] : a;
-          V v = xin[j] - a;
-          if (UPPER) v *= dv;
-          xs[slot * CG + cl[j]] = v;
-          bstore_val(rs_dst, (row * nrhs + c0 + cl[j]) * VS, v);
-        }
-      }
+      V v = xin[0] - a;
+      if (UPPER) v *= dv;
+      // lanes with nothing to write store to the dummy LDS slot and out of
+      // the buffer's range (dropped): no branch
+      xs[wr ? slot * CG + cl[0] : dummy] = v;
+      bstore_val(rs_dst, wr ? (row * nrhs + c0 + cl[0]) * VS : 0x7ffffff0, v);
       __syncthreads();
     }
   }
@@ -411,7 +431,7 @@ __global__ __launch_bounds__(NT) void k_trsv_ring(int nblocks, int wmask, const 
   const __amdgpu_buffer_rsrc_t rs_desc = make_rsrc(desc + sbeg, (int64_t)(send - sbeg) * 16);
   const int64_t vbytes = (int64_t)nb * nrhs * (int64_t)sizeof(V);
   V* xb = x + (int64_t)r0 * nrhs;
-  for (int i = threadIdx.x; i < (wmask + 1) * CG; i += NT) xs[i] = 0;
+  for (int i = threadIdx.x; i < (wmask + 2) * CG; i += NT) xs[i] = 0;  // ring + dummy slot
   __syncthreads();
   ring_sweep<V, NT, LPR, EPL, D, CG, false>(sbeg, sU, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
                                             make_rsrc(rhs + (int64_t)r0 * nrhs, vbytes), xs, make_rsrc(xb, vbytes));

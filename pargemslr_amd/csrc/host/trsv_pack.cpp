@@ -83,11 +83,11 @@ SweepOrder best_order(const Csr<T>& M, bool upper, int max_rows) {
 
 template <typename T>
 void emit_sweep(const Csr<T>& M, const std::vector<T>* dinv, const SweepOrder& o, int base, int wmask,
-                int64_t ent0, int64_t ri0, TrsvPack<T>& P) {
+                int fixed_width, int64_t ent0, int64_t ri0, TrsvPack<T>& P) {
   for (size_t s = 0; s + 1 < o.step.size(); ++s) {
     const int k0 = o.step[s], nr = o.step[s + 1] - k0;
-    int width = 0;
-    for (int t = 0; t < nr; ++t) {
+    int width = fixed_width;
+    for (int t = 0; t < nr && !fixed_width; ++t) {
       const int i = o.order[k0 + t];
       width = std::max(width, M.ia[i + 1] - M.ia[i]);
     }
@@ -118,7 +118,8 @@ void emit_sweep(const Csr<T>& M, const std::vector<T>* dinv, const SweepOrder& o
 }  // namespace
 
 template <typename T>
-TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_rows, int max_window) {
+TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_rows, int max_window,
+                      int ring_width) {
   TrsvPack<T> P;
   P.nblocks = L.ncomp;
   P.nloc = nloc;
@@ -131,7 +132,12 @@ TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_ro
   }
   int W = 1;
   while (W < P.window_needed) W <<= 1;
-  P.window = (max_window > 0 && W <= max_window) ? W : 0;
+  int row_len = 0;
+  for (const IluFactor<T>& F : L.ilu)
+    for (const Csr<T>* M : {&F.L, &F.U})
+      for (int i = 0; i < M->nrows; ++i) row_len = std::max(row_len, M->ia[i + 1] - M->ia[i]);
+  P.window = (max_window > 0 && W <= max_window && row_len <= ring_width) ? W : 0;
+  const int fixed_width = P.window ? ring_width : 0;
   const int wmask = P.window ? P.window - 1 : 0;
   P.blk_row0.resize(L.ncomp + 1);
   P.blk_step.resize(L.ncomp + 1);
@@ -146,9 +152,9 @@ TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_ro
     P.blk_ent0[b] = ent0;
     P.blk_ri0[b] = ri0;
     P.blk_step[b] = (int)P.desc.size() / 4;
-    emit_sweep<T>(F.L, nullptr, oL[b], 0, wmask, ent0, ri0, P);
+    emit_sweep<T>(F.L, nullptr, oL[b], 0, wmask, fixed_width, ent0, ri0, P);
     P.blk_stepU[b] = (int)P.desc.size() / 4;
-    emit_sweep<T>(F.U, &F.Dinv, oU[b], F.n, wmask, ent0, ri0, P);
+    emit_sweep<T>(F.U, &F.Dinv, oU[b], F.n, wmask, fixed_width, ent0, ri0, P);
     P.max_steps = std::max(P.max_steps, (int)P.desc.size() / 4 - P.blk_step[b]);
     oL[b] = SweepOrder();
     oU[b] = SweepOrder();
@@ -160,7 +166,7 @@ TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_ro
   return P;
 }
 
-template TrsvPack<double> trsv_pack<double>(const HostLevel<double>&, int, int, int, int);
-template TrsvPack<zdouble> trsv_pack<zdouble>(const HostLevel<zdouble>&, int, int, int, int);
+template TrsvPack<double> trsv_pack<double>(const HostLevel<double>&, int, int, int, int, int);
+template TrsvPack<zdouble> trsv_pack<zdouble>(const HostLevel<zdouble>&, int, int, int, int, int);
 
 }  // namespace pgs

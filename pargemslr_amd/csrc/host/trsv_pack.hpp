@@ -26,7 +26,9 @@
 // its last reader has run.  With `max_window` > 0 and every sweep's window
 // within it, the ELL columns hold those slots instead of row ids and
 // `window` is the ring size; otherwise columns are block-local row ids and
-// `window` is 0.
+// `window` is 0.  Ring-packed slabs have a fixed width, so a lane that reads
+// slot j of its row never reaches into the next row, and padded slots weigh
+// zero.
 #pragma once
 
 #include <cstdint>
@@ -59,8 +61,10 @@ struct TrsvPack {
 };
 
 // max_window: largest ring (a power of two, in rows) the caller can hold; 0
-// keeps row ids.
+// keeps row ids.  ring_width: with ring slots, every step's slab is exactly
+// this wide (rows padded with (0, 0)), and rows longer than it keep row ids.
 template <typename T>
-TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_rows, int max_window = 0);
+TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_rows, int max_window = 0,
+                      int ring_width = 0);
 
 }  // namespace pgs

@@ -1,0 +1,87 @@
+"""Request server (continuous batching over resident operators): every
+request's result equals the same right-hand sides solved alone."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import pargemslr_amd as pg
+from tests import problems as P
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+
+
+@pytest.fixture(scope="module")
+def operators(gpu):
+    """Two resident operators: 2D Laplacian 64^2 and upwind convection-diffusion 20^3."""
+    out = []
+    for A, cfg in ((pg.laplacian(64, 64, 1), (2, 4, 1e-2, 10, 0)), (pg.convdiff(20, 20, 20, 0.3), (3, 8, 1e-3, 20, 16))):
+        G = P.setup(A, *cfg)
+        out.append((A, G, pg.DevOp(G)))
+    return out
+
+
+def _alone(A, D, b):
+    W = pg.Workspace(D, max_nrhs=1, kdim=50)
+    xd = pg.DeviceArray(A.n)
+    (st, it, _), = W.fgmres(pg.DeviceArray.from_numpy(b), xd, kdim=50, maxits=1000, tol=TOL, prec=pg.PGS_PREC_FP32)
+    pg.synchronize()
+    return st, it, xd.numpy()
+
+
+def test_server_mix_equals_alone(operators, gpu):
+    S = pg.Server(max_batch=8, slots=2, mem_reserve=1 << 30, kdim=50, tol=TOL)
+    ids = [S.add_operator(D) for _, _, D in operators]
+    rng = np.random.default_rng(0)
+    plan = []  # (op, B)
+    for k in range(14):
+        op = int(rng.integers(0, len(ids)))
+        nr = int(rng.integers(1, 5))
+        A = operators[op][0]
+        B = np.stack([pg.rng_normal(1000 + 10 * k + c, A.n) for c in range(nr)], axis=1)
+        plan.append((op, B))
+    reqs = [None] * len(plan)
+
+    # half submitted at once, half from another thread while the first ones run
+    def late():
+        for k in range(len(plan) // 2, len(plan)):
+            time.sleep(0.01)
+            reqs[k] = S.submit(ids[plan[k][0]], plan[k][1])
+
+    for k in range(len(plan) // 2):
+        reqs[k] = S.submit(ids[plan[k][0]], plan[k][1])
+    t = threading.Thread(target=late)
+    t.start()
+    t.join()
+    results = [S.wait(r) for r in reqs]
+    S.close()
+    grouped = False
+    for (op, B), (X, its, info) in zip(plan, results):
+        A, _, D = operators[op]
+        assert info["status"] == 0 and info["nrhs"] == B.shape[1]
+        assert info["total_ms"] >= info["solve_ms"] > 0
+        grouped |= info["batch_nrhs"] > B.shape[1]
+        for c in range(B.shape[1]):
+            b = B[:, c]
+            assert np.linalg.norm(b - A.matvec(X[:, c])) <= TOL * np.linalg.norm(b)
+            st, it, x1 = _alone(A, D, b)
+            assert st == 0 and abs(int(its[c]) - it) <= 1, (its[c], it)
+            assert np.linalg.norm(X[:, c] - x1) <= 1e-6 * np.linalg.norm(x1)
+    assert grouped, "no request shared a batch"
+
+
+def test_server_rejects_bad_requests(operators, gpu):
+    S = pg.Server(max_batch=4, slots=1, mem_reserve=1 << 30)
+    i = S.add_operator(operators[0][2])
+    n = operators[0][0].n
+    with pytest.raises(pg.PgsError):
+        S.submit(i, np.ones((n, 5)))  # more columns than max_batch
+    with pytest.raises(pg.PgsError):
+        S.submit(i + 7, np.ones(n))   # no such operator
+    r = S.submit(i, np.zeros(n))      # zero right-hand side: x = 0, no iterations
+    X, its, info = S.wait(r)
+    assert info["status"] == 0 and not X.any() and its[0] == 0
+    S.close()
