@@ -150,6 +150,7 @@ int pgs_dev_malloc(int device, int64_t bytes, void** ptr);
 int pgs_dev_free(int device, void* ptr);
 int pgs_memcpy_htod(int device, void* dst, const void* src, int64_t bytes, void* stream);
 int pgs_memcpy_dtoh(int device, void* dst, const void* src, int64_t bytes, void* stream);
+int pgs_memcpy_dtod(int device, void* dst, const void* src, int64_t bytes, void* stream);
 int pgs_memset_zero(int device, void* ptr, int64_t bytes, void* stream);
 int pgs_stream_create(int device, void** stream);
 int pgs_stream_destroy(int device, void* stream);
@@ -190,6 +191,9 @@ typedef struct {
   int orth;         /* PGS_ORTH_* */
   int use_precond;  /* 0: no preconditioner (z = v) */
   int check_every;  /* host convergence poll interval in iterations (0 = 1) */
+  int use_graph;    /* 1: each inner step j of a cycle is captured once as a HIP graph
+                       (per workspace and settings) and replayed; results are the
+                       same as with direct launches */
 } pgs_fgmres_opts_t;
 
 typedef struct {
@@ -244,6 +248,7 @@ typedef struct {
   double tol;          /* relative tolerance on ||b - A x|| / ||b|| */
   int prec, orth;      /* PGS_PREC_* of the apply, PGS_ORTH_* */
   int check_every;     /* as in pgs_fgmres_opts_t */
+  int use_graph;       /* as in pgs_fgmres_opts_t */
 } pgs_server_opts_t;
 typedef struct {
   int status;          /* 0 when every column converged, else the first column's error code */

@@ -386,6 +386,7 @@ class FlexGmresClass {
     o.orth = orth_;
     o.use_precond = 1;
     o.check_every = 1;
+    o.use_graph = 0;
     pgs_fgmres_result_t r;
     int err = pgs_fgmres(dop, ws_, rhs.GetData(), x.GetData(), 1, &o, &r, nullptr);
     if (err) return err;

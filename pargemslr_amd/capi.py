@@ -1,7 +1,8 @@
 """ctypes bindings of the pargemslr_amd C ABI (include/pargemslr_amd.h).
 
-Host arrays are numpy; device arrays are torch tensors on the operator's
-device (torch only provides allocation and streams here).
+Host arrays are numpy; device arrays (DeviceArray) and streams (Stream) come
+from the library's own HIP runtime (pgs_dev_malloc, pgs_stream_create), so
+no second HIP runtime shares the process.
 """
 import ctypes as C
 import os
@@ -56,7 +57,8 @@ class LevelView(C.Structure):
 
 class FgmresOpts(C.Structure):
     _fields_ = [("kdim", C.c_int), ("maxits", C.c_int), ("tol", C.c_double), ("absolute_tol", C.c_int),
-                ("prec", C.c_int), ("orth", C.c_int), ("use_precond", C.c_int), ("check_every", C.c_int)]
+                ("prec", C.c_int), ("orth", C.c_int), ("use_precond", C.c_int), ("check_every", C.c_int),
+                ("use_graph", C.c_int)]
 
 
 class FgmresResult(C.Structure):
@@ -70,7 +72,7 @@ class KernelProfile(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("max_batch", C.c_int), ("slots", C.c_int), ("mem_reserve", C.c_int64), ("kdim", C.c_int),
                 ("maxits", C.c_int), ("tol", C.c_double), ("prec", C.c_int), ("orth", C.c_int),
-                ("check_every", C.c_int)]
+                ("check_every", C.c_int), ("use_graph", C.c_int)]
 
 
 class RequestInfo(C.Structure):
@@ -86,7 +88,7 @@ EXPORTED = [
     "pgs_last_hip_error", "pgs_gen_laplacian", "pgs_gen_convdiff", "pgs_free", "pgs_rng_uniform", "pgs_rng_normal",
     "pgs_ws_profile", "pgs_ws_profile_read", "pgs_dev_malloc", "pgs_dev_free", "pgs_memcpy_htod", "pgs_memcpy_dtoh",
     "pgs_memset_zero", "pgs_stream_create", "pgs_stream_destroy", "pgs_stream_synchronize", "pgs_device_synchronize",
-    "pgs_dev_op_dims", "pgs_ws_estimate", "pgs_server_create", "pgs_server_add_operator", "pgs_server_submit",
+    "pgs_memcpy_dtod", "pgs_dev_op_dims", "pgs_ws_estimate", "pgs_server_create", "pgs_server_add_operator", "pgs_server_submit",
     "pgs_server_wait", "pgs_server_destroy",
 ]
 
@@ -137,6 +139,7 @@ def lib():
     L.pgs_dev_free.argtypes = [C.c_int, vp]
     L.pgs_memcpy_htod.argtypes = [C.c_int, vp, vp, C.c_int64, vp]
     L.pgs_memcpy_dtoh.argtypes = [C.c_int, vp, vp, C.c_int64, vp]
+    L.pgs_memcpy_dtod.argtypes = [C.c_int, vp, vp, C.c_int64, vp]
     L.pgs_memset_zero.argtypes = [C.c_int, vp, C.c_int64, vp]
     L.pgs_stream_create.argtypes = [C.c_int, C.POINTER(vp)]
     L.pgs_stream_destroy.argtypes = [C.c_int, vp]
@@ -492,8 +495,8 @@ class Workspace:
         _chk(lib().pgs_apply(self.dop._h, self._h, _tptr(v), _tptr(z), nrhs, prec, _sh(stream)), "pgs_apply")
 
     def fgmres(self, b, x, nrhs=1, kdim=50, maxits=1000, tol=1e-8, prec=PGS_PREC_FP64, orth=PGS_ORTH_CGS2,
-               use_precond=True, absolute=False, check_every=1, stream=None):
-        o = FgmresOpts(kdim, maxits, tol, int(absolute), prec, orth, int(use_precond), check_every)
+               use_precond=True, absolute=False, check_every=1, use_graph=False, stream=None):
+        o = FgmresOpts(kdim, maxits, tol, int(absolute), prec, orth, int(use_precond), check_every, int(use_graph))
         res = (FgmresResult * nrhs)()
         _chk(lib().pgs_fgmres(self.dop._h, self._h, _tptr(b), _tptr(x), nrhs, C.byref(o), res, _sh(stream)),
              "pgs_fgmres")
@@ -505,9 +508,9 @@ class Server:
     batching of solve requests; see include/pargemslr_amd.h."""
 
     def __init__(self, device=0, max_batch=16, slots=3, mem_reserve=8 << 30, kdim=50, maxits=1000, tol=1e-8,
-                 prec=PGS_PREC_FP32, orth=PGS_ORTH_CGS2, check_every=1):
+                 prec=PGS_PREC_FP32, orth=PGS_ORTH_CGS2, check_every=1, use_graph=False):
         self.device = device
-        o = ServerOpts(max_batch, slots, mem_reserve, kdim, maxits, tol, prec, orth, check_every)
+        o = ServerOpts(max_batch, slots, mem_reserve, kdim, maxits, tol, prec, orth, check_every, int(use_graph))
         self._h = C.c_void_p()
         self._ops = []
         self._live = {}  # request id -> (b, x, iters) kept alive until waited for

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -193,8 +194,20 @@ struct pgs_ws_s {
   size_t ev_used = 0;
   double prof_bytes = 0.0, prof_ms = 0.0;
   int64_t prof_launches = 0;
+  // captured inner FGMRES steps (index = step), the settings they were
+  // recorded with, and the stream they are recorded on
+  std::vector<hipGraphExec_t> graph_exec;
+  int graph_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipStream_t cap_stream = nullptr;
+  void drop_graphs() {
+    for (hipGraphExec_t& g : graph_exec)
+      if (g) (void)hipGraphExecDestroy(g);
+    graph_exec.clear();
+  }
   ~pgs_ws_s() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    drop_graphs();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
   }
 };
 
@@ -204,6 +217,8 @@ constexpr int kNT = 256;
 constexpr int kTrsvNT = 256;
 constexpr int kPMAX = 128;  // Krylov dimension + 1 limit of the shared-memory coefficient tables
 constexpr int kLrBlocks = 256;  // most workgroups of the low-rank partial sums
+constexpr int kBasisNT = 256;
+constexpr int kBasisPM = 64;  // most basis vectors k_basis_reg takes
 
 template <typename V>
 DevPrec<V>& prec_of(pgs_dev_op_s* op);
@@ -288,9 +303,7 @@ void launch_trsv_pipe_nt(const DevTrsv& T, const V* rhs, V* x, int nrhs, size_t 
   // keeps the in-flight vector loads (EPL + 2 per stage) under 64
   constexpr int EPL = 20 / LPR;
   constexpr int D = LPR == 4 ? (sizeof(V) == 4 ? 8 : 6) : (sizeof(V) == 4 ? 5 : 4);
-  auto kern = k_trsv_pipe<V, XLDS, NT, LPR, EPL, D>;
-  if (XLDS && lds > 65536)
-    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  auto kern = k_trsv_pipe<V, XLDS, NT, LPR, EPL, D>;  // LDS limit raised in set_kernel_attrs
   const unsigned grid = (unsigned)T.nblocks * (unsigned)nrhs;
   kern<<<grid, NT, XLDS ? lds : 0, s>>>(T.nblocks, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
                                         T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), T.desc.as<int4>(),
@@ -315,10 +328,7 @@ constexpr int kRingNT = 128, kRingLPR = 4, kRingEPL = 5;
 template <typename V, int CG>
 void launch_trsv_ring_cg(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
   constexpr int D = sizeof(V) == 4 ? 8 : 6;
-  auto kern = k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, CG>;
-  static const hipError_t attr =
-      hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, op->lds_max);
-  HIPCHK(attr);
+  auto kern = k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, CG>;  // LDS limit raised in set_kernel_attrs
   const size_t lds = (size_t)(T.window + 1) * CG * sizeof(V);  // ring + one dummy slot
   const unsigned grid = (unsigned)T.nblocks * (unsigned)(nrhs / CG);
   kern<<<grid, kRingNT, lds, s>>>(T.nblocks, T.window - 1, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
@@ -357,6 +367,48 @@ void launch_trsv_impl(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
                                                             T.ri.as<TPair<V>>(), rhs, x, nrhs);
   }
   HIPCHK(hipGetLastError());
+}
+
+// Raises the dynamic-LDS limit of every kernel that may need more than 64 KB,
+// once per process, so that no launch path calls a HIP function other than
+// the launch itself (launches may be captured into graphs).
+template <typename K>
+void set_lds_limit(K kern, int lds_max) {
+  HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
+}
+
+template <typename V>
+void set_trsv_attrs(int lds_max) {
+  constexpr int D = sizeof(V) == 4 ? 8 : 6;
+  set_lds_limit(k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, 2>, lds_max);
+  set_lds_limit(k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, 4>, lds_max);
+  constexpr int D2 = sizeof(V) == 4 ? 5 : 4;
+  set_lds_limit(k_trsv_pipe<V, true, 64, 2, 10, D2>, lds_max);
+  set_lds_limit(k_trsv_pipe<V, true, 128, 2, 10, D2>, lds_max);
+  set_lds_limit(k_trsv_pipe<V, true, 128, 4, 5, D>, lds_max);
+  set_lds_limit(k_trsv_pipe<V, true, 256, 4, 5, D>, lds_max);
+}
+
+template <int PM>
+void set_basis_attrs(int lds_max) {
+  set_lds_limit(k_basis_reg<kBasisNT, PM, 0, 1>, lds_max);
+  set_lds_limit(k_basis_reg<kBasisNT, PM, 1, 1>, lds_max);
+  set_lds_limit(k_basis_reg<kBasisNT, PM, 2, 1>, lds_max);
+}
+
+void set_kernel_attrs(int lds_max) {
+  static std::once_flag once;
+  std::call_once(once, [lds_max] {
+    set_trsv_attrs<float>(lds_max);
+    set_trsv_attrs<double>(lds_max);
+    set_basis_attrs<8>(lds_max);
+    set_basis_attrs<16>(lds_max);
+    set_basis_attrs<32>(lds_max);
+    set_basis_attrs<64>(lds_max);
+    set_lds_limit(k_basis_reg<kBasisNT, 32, 0, 2>, lds_max);
+    set_lds_limit(k_basis_reg<kBasisNT, 32, 1, 2>, lds_max);
+  });
 }
 
 template <typename V>
@@ -460,6 +512,7 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     op->lds_max = lds;
+    set_kernel_attrs(lds);
     if (const char* v = std::getenv("PGS_TRSV_VARIANT")) op->trsv_variant = std::atoi(v);
     if (const char* v = std::getenv("PGS_TRSV_NT")) {
       const int nt = std::atoi(v);
@@ -705,15 +758,11 @@ void multi_axpy(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64
   HIPCHK(hipGetLastError());
 }
 
-constexpr int kBasisNT = 256;
-constexpr int kBasisPM = 64;  // most basis vectors k_basis_reg takes
 
 template <int PM, int MODE, int LPE>
 void launch_basis_mode(int lds_max, int nblk, size_t lds, int64_t n, int nrhs, int p, const double* V, int64_t ldv,
                        const double* h, double* w, double* part, hipStream_t s) {
-  static const hipError_t attr = hipFuncSetAttribute((const void*)k_basis_reg<kBasisNT, PM, MODE, LPE>,
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-  HIPCHK(attr);
+  (void)lds_max;  // LDS limit raised in set_kernel_attrs
   k_basis_reg<kBasisNT, PM, MODE, LPE><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, h, w, part);
 }
 
@@ -790,6 +839,66 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int 
     HIPCHK(hipStreamSynchronize(s));
     return hflag[0];
   };
+  // inner step i of a cycle: z = M^{-1} v_{i-1}, w = A z, orthogonalize w
+  // against v_0..v_{i-1}, Givens update, v_i = w / ||w||; every launch on
+  // stream `ss`, nothing else (it may be captured into a graph)
+  auto inner_step = [&](int i, hipStream_t ss) {
+    double* vprev = V + (int64_t)(i - 1) * ldv;
+    double* w = V + (int64_t)i * ldv;
+    VZ* zc = Z + (int64_t)(i - 1) * ldv;
+    if (o.use_precond) {
+      if (o.prec == PGS_PREC_FP64)
+        apply_impl<double, VZ>(op, ws, vprev, zc, nrhs, ss);
+      else
+        apply_impl<float, VZ>(op, ws, vprev, zc, nrhs, ss);
+    } else {
+      k_copy_conv<VZ><<<nb, kNT, 0, ss>>>(n * nrhs, vprev, zc);
+    }
+    launch_spmv<double, VZ, double, double>(op->A, zc, w, (const double*)nullptr, 1.0, 0.0, 0, nrhs, ss);
+    double* h = ws->hbuf.as<double>();
+    double* h2 = ws->h2buf.as<double>();
+    double* t2 = ws->t2.as<double>();
+    if (o.orth == PGS_ORTH_CGS2) {
+      // PargemslrCgs2 (matrixops.cpp:10588-10709): two projection passes, the
+      // norm of the result accumulated by the second update.  Three sweeps
+      // over the basis: h = V^T w; w -= V h fused with h2 = V^T w; w -= V h2.
+      if (ws->fused_cgs2 && basis_fits(ws, i, nrhs)) {
+        basis_reg(ws, n, nrhs, i, V, ldv, nullptr, w, h, 0, ss);
+        basis_reg(ws, n, nrhs, i, V, ldv, h, w, h2, 1, ss);
+        basis_reg(ws, n, nrhs, i, V, ldv, h2, w, t2, 2, ss);
+      } else {
+        multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h, 0, ss);
+        multi_axpy(ws, n, nrhs, i, V, ldv, h, w, nullptr, ss);
+        multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h2, 0, ss);
+        multi_axpy(ws, n, nrhs, i, V, ldv, h2, w, t2, ss);
+      }
+      k_dot_final_wave<<<(i * nrhs + 3) / 4, 256, 0, ss>>>(1, i, nrhs, h2, h, 1);
+    } else {
+      // PargemslrMgs (matrixops.cpp:10728-10807): one (dot, axpy) pair per basis vector
+      for (int j = 0; j < i; ++j) {
+        multidot<double, double>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, w, h + (int64_t)j * nrhs, 0, ss);
+        multi_axpy(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w, j == i - 1 ? t2 : nullptr,
+                   ss);
+      }
+    }
+    k_givens<<<1, 64, 0, ss>>>(nrhs, m, i, maxits, h, ws->t2.as<double>(), H, ws->cs.as<double>(),
+                               ws->sn.as<double>(), rs, st);
+    k_scale_cols<<<nb, kNT, 0, ss>>>(n, nrhs, st, w);
+    HIPCHK(hipGetLastError());
+  };
+  // graphs of the inner steps, kept per workspace while the settings that
+  // enter the recorded launches stay the same (not while timing kernels)
+  const bool graphs = o.use_graph != 0 && !ws->prof;
+  if (graphs) {
+    const int key[8] = {nrhs, o.prec, o.orth, o.use_precond, maxits, m, ws->fused_cgs2 ? 1 : 0,
+                        (int)sizeof(VZ)};
+    if (!ws->cap_stream) HIPCHK(hipStreamCreateWithFlags(&ws->cap_stream, hipStreamNonBlocking));
+    if (std::memcmp(key, ws->graph_key, sizeof(key)) != 0 || (int)ws->graph_exec.size() != m + 1) {
+      ws->drop_graphs();
+      ws->graph_exec.assign(m + 1, nullptr);
+      std::memcpy(ws->graph_key, key, sizeof(key));
+    }
+  }
   int iters_host = 0;
   while (true) {
     if (read_flag(true) == 0) break;
@@ -799,47 +908,31 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int 
     while (i < m && iters_host < maxits) {
       ++i;
       ++iters_host;
-      double* vprev = V + (int64_t)(i - 1) * ldv;
-      double* w = V + (int64_t)i * ldv;
-      VZ* zc = Z + (int64_t)(i - 1) * ldv;
-      if (o.use_precond) {
-        if (o.prec == PGS_PREC_FP64)
-          apply_impl<double, VZ>(op, ws, vprev, zc, nrhs, s);
-        else
-          apply_impl<float, VZ>(op, ws, vprev, zc, nrhs, s);
-      } else {
-        k_copy_conv<VZ><<<nb, kNT, 0, s>>>(n * nrhs, vprev, zc);
-      }
-      launch_spmv<double, VZ, double, double>(op->A, zc, w, (const double*)nullptr, 1.0, 0.0, 0, nrhs, s);
-      double* h = ws->hbuf.as<double>();
-      double* h2 = ws->h2buf.as<double>();
-      double* t2 = ws->t2.as<double>();
-      if (o.orth == PGS_ORTH_CGS2) {
-        // PargemslrCgs2 (matrixops.cpp:10588-10709): two projection passes, the
-        // norm of the result accumulated by the second update.  Three sweeps
-        // over the basis: h = V^T w; w -= V h fused with h2 = V^T w; w -= V h2.
-        if (ws->fused_cgs2 && basis_fits(ws, i, nrhs)) {
-          basis_reg(ws, n, nrhs, i, V, ldv, nullptr, w, h, 0, s);
-          basis_reg(ws, n, nrhs, i, V, ldv, h, w, h2, 1, s);
-          basis_reg(ws, n, nrhs, i, V, ldv, h2, w, t2, 2, s);
-        } else {
-          multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h, 0, s);
-          multi_axpy(ws, n, nrhs, i, V, ldv, h, w, nullptr, s);
-          multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h2, 0, s);
-          multi_axpy(ws, n, nrhs, i, V, ldv, h2, w, t2, s);
+      if (graphs) {
+        hipGraphExec_t& ex = ws->graph_exec[i];
+        if (!ex) {
+          // record inner step i once on the workspace's capture stream
+          HIPCHK(hipStreamBeginCapture(ws->cap_stream, hipStreamCaptureModeThreadLocal));
+          hipGraph_t g = nullptr;
+          try {
+            inner_step(i, ws->cap_stream);
+          } catch (...) {
+            (void)hipStreamEndCapture(ws->cap_stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          HIPCHK(hipStreamEndCapture(ws->cap_stream, &g));
+          const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          if (e != hipSuccess) {
+            ex = nullptr;
+            throw HipError{e};
+          }
         }
-        k_dot_final_wave<<<(i * nrhs + 3) / 4, 256, 0, s>>>(1, i, nrhs, h2, h, 1);
+        HIPCHK(hipGraphLaunch(ex, s));
       } else {
-        // PargemslrMgs (matrixops.cpp:10728-10807): one (dot, axpy) pair per basis vector
-        for (int j = 0; j < i; ++j) {
-          multidot<double, double>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, w, h + (int64_t)j * nrhs, 0, s);
-          multi_axpy(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w, j == i - 1 ? t2 : nullptr, s);
-        }
+        inner_step(i, s);
       }
-      k_givens<<<1, 64, 0, s>>>(nrhs, m, i, maxits, h, ws->t2.as<double>(), H, ws->cs.as<double>(),
-                                ws->sn.as<double>(), rs, st);
-      k_scale_cols<<<nb, kNT, 0, s>>>(n, nrhs, st, w);
-      HIPCHK(hipGetLastError());
       if (i % check == 0 || i == m) {
         if (read_flag(false) == 0) break;
       }
@@ -931,6 +1024,15 @@ extern "C" int pgs_memcpy_dtoh(int device, void* dst, const void* src, int64_t b
   int rc = hip_rc(hipSetDevice(device));
   if (rc || bytes == 0) return rc;
   rc = hip_rc(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  if (rc) return rc;
+  return hip_rc(hipStreamSynchronize((hipStream_t)stream));
+}
+
+extern "C" int pgs_memcpy_dtod(int device, void* dst, const void* src, int64_t bytes, void* stream) {
+  if (bytes < 0 || (bytes > 0 && (!dst || !src))) return PGS_ERROR_INVALID_PARAM;
+  int rc = hip_rc(hipSetDevice(device));
+  if (rc || bytes == 0) return rc;
+  rc = hip_rc(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   if (rc) return rc;
   return hip_rc(hipStreamSynchronize((hipStream_t)stream));
 }

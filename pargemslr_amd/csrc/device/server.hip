@@ -165,6 +165,7 @@ void run_batch(pgs_server_s* S, void* stream, std::map<int, Slot>& cache, std::v
     o.orth = S->opts.orth;
     o.use_precond = 1;
     o.check_every = S->opts.check_every;
+    o.use_graph = S->opts.use_graph;
     if (!rc) rc = pgs_fgmres(op.dop, sl->ws, sl->B, sl->X, nb, &o, res.data(), stream);
     if (!rc) rc = pgs_memcpy_dtoh(S->device, sl->stage, sl->X, (int64_t)n * nb * 8, stream);
   }

@@ -208,3 +208,22 @@ def test_fgmres_cgs2_long_basis(nr, c1, gpu):
         st_g, it_g, rr_g = res[c]
         assert it_g == it_o == 60
         assert rr_g == pytest.approx(rr_o, rel=1e-6), c
+
+
+@pytest.mark.parametrize("prec", [pg.PGS_PREC_FP64, pg.PGS_PREC_FP32])
+def test_fgmres_graph_replay_matches_direct(prec, small, gpu):
+    """Inner steps captured as HIP graphs and replayed (FGMRES(10): every
+    graph is replayed once per restart cycle, and the whole solve twice with
+    direct-launch solves in between) give the same bits as direct launches."""
+    A, G, D, W = small
+    nr = 4
+    B = np.stack([pg.rng_normal(90 + c, A.n) for c in range(nr)], axis=1)
+    outs = []
+    for use_graph in (False, True, False, True):
+        Xd = _dev(np.zeros((A.n, nr)), gpu)
+        res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=10, maxits=500, tol=1e-8, prec=prec, use_graph=use_graph)
+        outs.append((_host(Xd), [r[:2] for r in res]))
+    for X, res in outs[1:]:
+        assert res == outs[0][1]
+        assert np.array_equal(X, outs[0][0])
+    assert all(st == 0 for st, _ in outs[0][1]) and max(it for _, it in outs[0][1]) > 10

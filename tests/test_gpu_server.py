@@ -32,8 +32,9 @@ def _alone(A, D, b):
     return st, it, xd.numpy()
 
 
-def test_server_mix_equals_alone(operators, gpu):
-    S = pg.Server(max_batch=8, slots=2, mem_reserve=1 << 30, kdim=50, tol=TOL)
+@pytest.mark.parametrize("use_graph", [False, True], ids=["launches", "graphs"])
+def test_server_mix_equals_alone(use_graph, operators, gpu):
+    S = pg.Server(max_batch=8, slots=2, mem_reserve=1 << 30, kdim=50, tol=TOL, use_graph=use_graph)
     ids = [S.add_operator(D) for _, _, D in operators]
     rng = np.random.default_rng(0)
     plan = []  # (op, B)
