@@ -34,6 +34,7 @@ CONFIGS = {
     "m64": dict(grid=(64, 64, 64), nlev=3, ncomp=64, droptol=1e-3, rownnz=20, rank=16, kdim=50, tol=1e-8),
 }
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_LEVEL0_16RHS = 1161.7e6  # bytes read + written per level-0 ring-solve launch, 16 RHS FP32
 
 
 def log(*a):
@@ -52,9 +53,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve", action="store_true",
                     help="run the serving stream (resident operators, Poisson arrivals) instead")
-    ap.add_argument("--requests", type=int, default=48, help="requests of the serving stream")
-    ap.add_argument("--rate", type=float, default=20.0, help="mean request arrivals per second")
+    ap.add_argument("--requests", type=int, default=32, help="requests of the serving stream")
+    ap.add_argument("--rate", type=float, default=2.0, help="mean request arrivals per second")
     ap.add_argument("--graphs", type=int, default=1, help="serving: replay captured FGMRES steps")
+    ap.add_argument("--probe", action="store_true",
+                    help="counter probe: 16-RHS FP32 applies of the 128^3 operator and a 1 GiB copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not built",
@@ -62,6 +65,9 @@ def main():
         sys.exit(2)
     if args.serve:
         serve(args)
+        return
+    if args.probe:
+        probe(args)
         return
 
     import pargemslr_amd as pg
@@ -160,9 +166,13 @@ def main():
         f"max true rel res {max(true_res):.2e}, ok={ok}")
 
     ach = prof["total_bytes"] / (prof["total_ms"] * 1e-3) / 1e9 if prof["total_ms"] > 0 else 0.0
+    # HBM bytes per level-0 launch of this workload, measured with rocprofv3
+    # FETCH_SIZE and WRITE_SIZE passes over `bench.py --probe`, each scaled by
+    # what the counter reads for a 1 GiB device copy (summary under profiles/)
+    traffic = TRAFFIC_LEVEL0_16RHS if (args.config == "c2" and nr == 16) else None
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_trsv (level-0 block ILU solve, L+U sweeps)",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_trsv_ring (level-0 block ILU solve, L+U sweeps)",
                 "launches": prof["launches"],
                 "avg_launch_ms": round(prof["total_ms"] / max(1, prof["launches"]), 4),
                 "bytes_per_launch": round(prof["total_bytes"] / max(1, prof["launches"]))}
@@ -219,6 +229,33 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def probe(args):
+    """Workload for hardware-counter passes (rocprofv3 --pmc): one warm-up
+    apply of the 128^3 operator (FP32, 16 right-hand sides), a 1 GiB
+    device-to-device copy whose known size calibrates FETCH_SIZE and
+    WRITE_SIZE, then three applies."""
+    import pargemslr_amd as pg
+
+    cfg = CONFIGS["c2"]
+    A = pg.laplacian(*cfg["grid"])
+    G = pg.Gemslr(A, pg.config_params(cfg["nlev"], cfg["ncomp"], cfg["droptol"], cfg["rownnz"], cfg["rank"]), seed=0)
+    D = pg.DevOp(G, prec_mask=pg.PGS_PREC_FP32)
+    nr = 16
+    W = pg.Workspace(D, max_nrhs=nr, kdim=2)
+    V = pg.DeviceArray.from_numpy(np.stack([pg.rng_normal(c, A.n) for c in range(nr)], axis=1))
+    Z = pg.DeviceArray(V.shape)
+    W.apply(V, Z, nrhs=nr, prec=pg.PGS_PREC_FP32)
+    pg.synchronize()
+    nb = 1 << 30
+    src, dst = pg.DeviceArray(nb // 8), pg.DeviceArray(nb // 8)
+    pg.capi._chk(pg.lib().pgs_memcpy_dtod(0, dst.ptr, src.ptr, nb, None), "pgs_memcpy_dtod")
+    log(f"[probe] copied {nb} bytes")
+    for _ in range(3):
+        W.apply(V, Z, nrhs=nr, prec=pg.PGS_PREC_FP32)
+    pg.synchronize()
+    log("[probe] applies done")
 
 
 def serve(args):
