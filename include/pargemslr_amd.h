@@ -163,8 +163,9 @@ int pgs_dev_op_create(pgs_gemslr_t op, int device, int prec_mask, pgs_dev_op_t* 
 int pgs_dev_op_destroy(pgs_dev_op_t dop);
 /* device bytes held by the operator */
 int pgs_dev_op_bytes(pgs_dev_op_t dop, int64_t* bytes);
-/* order n and device of the operator */
-int pgs_dev_op_dims(pgs_dev_op_t dop, int* n, int* device);
+/* order n, device and value type (is_complex: 1 for complex128 operators,
+ * whose vectors are (re, im) double pairs) of the operator */
+int pgs_dev_op_dims(pgs_dev_op_t dop, int* n, int* device, int* is_complex);
 
 /* Workspace for up to max_nrhs right-hand sides and Krylov dimension kdim. */
 int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out);
@@ -261,7 +262,8 @@ typedef struct {
 int pgs_server_create(int device, const pgs_server_opts_t* opts, pgs_server_t* out);
 /* the operator must outlive the server */
 int pgs_server_add_operator(pgs_server_t srv, pgs_dev_op_t dop, int* op_id);
-/* b, x: host arrays n x nrhs, column c at [c * n]; iters: nrhs ints or NULL.
+/* b, x: host arrays n x nrhs, column c at [c * n] (complex operators: (re, im)
+ * double pairs, column c at [2 * c * n]); iters: nrhs ints or NULL.
  * All three stay owned by the caller and must stay valid until
  * pgs_server_wait returns for the request; x and iters are written then. */
 int pgs_server_submit(pgs_server_t srv, int op_id, int nrhs, const double* b, double* x, int* iters, int64_t* req);

@@ -1,5 +1,8 @@
 """ctypes wrapper of the oracle (oracle/oracle.c) -- TEST INFRASTRUCTURE ONLY.
 
+Every function takes real (float64) or complex (complex128) data and calls
+the matching C instantiation (orc_* or orc_*_z).
+
 Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg;
 never by the product path.
 """
@@ -39,16 +42,18 @@ def lib():
             build()
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
-        L.orc_csr_matvec.argtypes = [C.c_int, vp, vp, vp, C.c_double, vp, C.c_double, vp]
-        L.orc_ilu_solve.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp]
-        L.orc_lowrank_apply.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp]
         L.orc_gemslr_work_size.argtypes = [C.POINTER(OrcGemslr)]
         L.orc_gemslr_work_size.restype = C.c_long
-        L.orc_gemslr_apply.argtypes = [C.POINTER(OrcGemslr), vp, vp, vp]
-        L.orc_mgs.argtypes = [C.c_int, vp, C.c_int, vp, C.c_int, C.POINTER(C.c_double)]
-        L.orc_cgs2.argtypes = [C.c_int, vp, C.c_int, vp, C.c_int, C.POINTER(C.c_double)]
-        L.orc_fgmres.argtypes = [C.c_int, vp, vp, vp, C.POINTER(OrcGemslr), vp, vp, C.c_int, C.c_int, C.c_double,
-                                 C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), vp]
+        for sfx in ("", "_z"):
+            getattr(L, "orc_csr_matvec" + sfx).argtypes = [C.c_int, vp, vp, vp, C.c_double, vp, C.c_double, vp]
+            getattr(L, "orc_ilu_solve" + sfx).argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+            getattr(L, "orc_lowrank_apply" + sfx).argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp]
+            getattr(L, "orc_gemslr_apply" + sfx).argtypes = [C.POINTER(OrcGemslr), vp, vp, vp]
+            getattr(L, "orc_mgs" + sfx).argtypes = [C.c_int, vp, C.c_int, vp, C.c_int, C.POINTER(C.c_double)]
+            getattr(L, "orc_cgs2" + sfx).argtypes = [C.c_int, vp, C.c_int, vp, C.c_int, C.POINTER(C.c_double)]
+            getattr(L, "orc_fgmres" + sfx).argtypes = [C.c_int, vp, vp, vp, C.POINTER(OrcGemslr), vp, vp, C.c_int,
+                                                       C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                                       C.POINTER(C.c_double), vp]
         _lib = L
     return _lib
 
@@ -65,31 +70,47 @@ def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
+def _cplx(*arrs):
+    return any(np.iscomplexobj(a) for a in arrs if a is not None)
+
+
+def _val(a, cplx):
+    return np.ascontiguousarray(a, dtype=np.complex128 if cplx else np.float64)
+
+
+def _fn(name, cplx):
+    return getattr(lib(), name + ("_z" if cplx else ""))
+
+
 def csr_matvec(ia, ja, a, x, alpha=1.0, beta=0.0, y=None):
-    ia, ja, a, x = _i32(ia), _i32(ja), _f64(a), _f64(x)
+    cz = _cplx(a, x, y)
+    ia, ja, a, x = _i32(ia), _i32(ja), _val(a, cz), _val(x, cz)
     n = len(ia) - 1
-    y = np.zeros(n) if y is None else _f64(y).copy()
-    lib().orc_csr_matvec(n, _p(ia), _p(ja), _p(a), alpha, _p(x), beta, _p(y))
+    y = np.zeros(n, dtype=a.dtype) if y is None else _val(y, cz).copy()
+    _fn("orc_csr_matvec", cz)(n, _p(ia), _p(ja), _p(a), alpha, _p(x), beta, _p(y))
     return y
 
 
 def ilu_solve(L, U, Dinv, rhs):
     (Li, Lj, La), (Ui, Uj, Ua) = L, U
+    cz = _cplx(La, Ua, Dinv, rhs)
     n = len(Li) - 1
-    arrs = [_i32(Li), _i32(Lj), _f64(La), _i32(Ui), _i32(Uj), _f64(Ua), _f64(Dinv), _f64(rhs)]
-    x = np.zeros(n)
-    lib().orc_ilu_solve(n, *[_p(a) for a in arrs], _p(x))
+    arrs = [_i32(Li), _i32(Lj), _val(La, cz), _i32(Ui), _i32(Uj), _val(Ua, cz), _val(Dinv, cz), _val(rhs, cz)]
+    x = np.zeros(n, dtype=arrs[-1].dtype)
+    _fn("orc_ilu_solve", cz)(n, *[_p(a) for a in arrs], _p(x))
     return x
 
 
 def lowrank_apply(W, WH, z):
+    cz = _cplx(W, WH, z)
+    dt = np.complex128 if cz else np.float64
     m, k = W.shape
-    Wc = np.asfortranarray(W, dtype=np.float64)
-    WHc = np.asfortranarray(WH, dtype=np.float64)
-    z = _f64(z)
-    x = np.zeros(m)
-    t = np.zeros(max(k, 1))
-    lib().orc_lowrank_apply(m, k, C.c_void_p(Wc.ctypes.data), C.c_void_p(WHc.ctypes.data), _p(z), _p(x), _p(t))
+    Wc = np.asfortranarray(W, dtype=dt)
+    WHc = np.asfortranarray(WH, dtype=dt)
+    z = _val(z, cz)
+    x = np.zeros(m, dtype=dt)
+    t = np.zeros(max(k, 1), dtype=dt)
+    _fn("orc_lowrank_apply", cz)(m, k, C.c_void_p(Wc.ctypes.data), C.c_void_p(WHc.ctypes.data), _p(z), _p(x), _p(t))
     return x
 
 
@@ -100,6 +121,8 @@ class GemslrOracle:
     def __init__(self, n, perm, levels):
         self.n = n
         self._keep = []
+        self.is_complex = cz = _cplx(*[d["Dinv"] for d in levels])
+        self.dtype = np.dtype(np.complex128 if cz else np.float64)
         self.perm = self._k(_i32(perm))
         lv = (OrcLevel * len(levels))()
         for i, d in enumerate(levels):
@@ -111,37 +134,41 @@ class GemslrOracle:
                 ia, ja, a = d[nm]
                 setattr(o, nm + "_ia", _p(self._k(_i32(ia))))
                 setattr(o, nm + "_ja", _p(self._k(_i32(ja))))
-                setattr(o, nm + "_a", _p(self._k(_f64(a))))
-            o.Dinv = _p(self._k(_f64(d["Dinv"])))
+                setattr(o, nm + "_a", _p(self._k(_val(a, cz))))
+            o.Dinv = _p(self._k(_val(d["Dinv"], cz)))
             if d["rank"] > 0:
-                o.W = C.c_void_p(self._k(np.asfortranarray(d["W"], dtype=np.float64)).ctypes.data)
-                o.WH = C.c_void_p(self._k(np.asfortranarray(d["WH"], dtype=np.float64)).ctypes.data)
+                o.W = C.c_void_p(self._k(np.asfortranarray(d["W"], dtype=self.dtype)).ctypes.data)
+                o.WH = C.c_void_p(self._k(np.asfortranarray(d["WH"], dtype=self.dtype)).ctypes.data)
         self._levels = lv
         self.s = OrcGemslr(n, len(levels), _p(self.perm), lv)
-        self.work = np.zeros(lib().orc_gemslr_work_size(C.byref(self.s)))
+        self.work = np.zeros(lib().orc_gemslr_work_size(C.byref(self.s)), dtype=self.dtype)
 
     def _k(self, a):
         self._keep.append(a)
         return a
 
     def apply(self, v):
-        v = _f64(v)
-        z = np.zeros(self.n)
-        lib().orc_gemslr_apply(C.byref(self.s), _p(v), _p(z), _p(self.work))
+        v = _val(v, self.is_complex)
+        z = np.zeros(self.n, dtype=self.dtype)
+        _fn("orc_gemslr_apply", self.is_complex)(C.byref(self.s), _p(v), _p(z), _p(self.work))
         return z
 
 
 def fgmres(A, b, x0=None, prec=None, kdim=50, maxits=1000, tol=1e-8, absolute=False, orth=0):
-    """Oracle FGMRES; A is a pargemslr_amd.CsrHost (real), prec a GemslrOracle or None.
-    Returns (x, status, iters, rel_res, hist)."""
-    ia, ja, a = _i32(A.ia), _i32(A.ja), _f64(A.a)
-    b = _f64(b)
-    x = np.zeros(A.n) if x0 is None else _f64(x0).copy()
+    """Oracle FGMRES; A is a pargemslr_amd.CsrHost (real or complex), prec a
+    GemslrOracle of the same value type or None.  Returns (x, status, iters,
+    rel_res, hist)."""
+    cz = _cplx(A.a)
+    if prec is not None and prec.is_complex != cz:
+        raise ValueError("operator and preconditioner value types differ")
+    ia, ja, a = _i32(A.ia), _i32(A.ja), _val(A.a, cz)
+    b = _val(b, cz)
+    x = np.zeros(A.n, dtype=b.dtype) if x0 is None else _val(x0, cz).copy()
     it = C.c_int()
     rr = C.c_double()
     hist = np.full(maxits + 1, np.nan)
-    st = lib().orc_fgmres(A.n, _p(ia), _p(ja), _p(a), C.byref(prec.s) if prec is not None else None, _p(b), _p(x),
-                          kdim, maxits, tol, int(absolute), orth, C.byref(it), C.byref(rr), _p(hist))
+    st = _fn("orc_fgmres", cz)(A.n, _p(ia), _p(ja), _p(a), C.byref(prec.s) if prec is not None else None, _p(b),
+                               _p(x), kdim, maxits, tol, int(absolute), orth, C.byref(it), C.byref(rr), _p(hist))
     return x, st, it.value, rr.value, hist[: it.value + 1]
 
 
@@ -149,6 +176,8 @@ def mgs(V, k, H, cgs2=False):
     """Orthogonalize column k+1 of V (n x (k+2), Fortran order) in place; fills H[:, k]."""
     n = V.shape[0]
     t = C.c_double()
-    f = lib().orc_cgs2 if cgs2 else lib().orc_mgs
+    cz = np.iscomplexobj(V)
+    assert np.iscomplexobj(H) == cz
+    f = _fn("orc_cgs2" if cgs2 else "orc_mgs", cz)
     f(n, C.c_void_p(V.ctypes.data), k, C.c_void_p(H.ctypes.data), H.shape[0], C.byref(t))
     return t.value

@@ -145,7 +145,7 @@ def lib():
     L.pgs_stream_destroy.argtypes = [C.c_int, vp]
     L.pgs_stream_synchronize.argtypes = [C.c_int, vp]
     L.pgs_device_synchronize.argtypes = [C.c_int]
-    L.pgs_dev_op_dims.argtypes = [vp, ip, ip]
+    L.pgs_dev_op_dims.argtypes = [vp, ip, ip, ip]
     L.pgs_ws_estimate.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int64)]
     L.pgs_server_create.argtypes = [C.c_int, C.POINTER(ServerOpts), C.POINTER(vp)]
     L.pgs_server_add_operator.argtypes = [vp, vp, ip]
@@ -304,8 +304,16 @@ class Gemslr:
         return np.ctypeslib.as_array(p, shape=(self.A.n,)).copy()
 
     def levels(self):
-        """Level structure as numpy arrays (copies)."""
+        """Level structure as numpy arrays (copies; complex128 values for a
+        complex operator)."""
         out = []
+        cplx = bool(self.A.is_complex)
+        nd = 2 if cplx else 1
+
+        def vals(p, count):
+            a = np.ctypeslib.as_array(p, shape=(max(count, 1) * nd,)).copy()[: count * nd]
+            return a.view(np.complex128) if cplx else a
+
         nlev = self.info().nlev
         for l in range(nlev):
             v = self.level_view(l)
@@ -319,19 +327,21 @@ class Gemslr:
                 ia_np = np.ctypeslib.as_array(ia, shape=(nr + 1,)).copy()
                 nnz = int(ia_np[-1])
                 ja_np = np.ctypeslib.as_array(getattr(v, nm + "_ja"), shape=(max(nnz, 1),)).copy()[:nnz] if nnz else np.zeros(0, np.int32)
-                a_np = np.ctypeslib.as_array(getattr(v, nm + "_a"), shape=(max(nnz, 1),)).copy()[:nnz] if nnz else np.zeros(0)
+                a_np = vals(getattr(v, nm + "_a"), nnz) if nnz else np.zeros(0, np.complex128 if cplx else np.float64)
                 d[nm] = (ia_np, ja_np, a_np)
-            d["Dinv"] = np.ctypeslib.as_array(v.Dinv, shape=(v.nloc,)).copy()
+            d["Dinv"] = vals(v.Dinv, v.nloc)
             if v.rank > 0:
-                d["W"] = np.ctypeslib.as_array(v.W, shape=(v.rank, v.nrem)).T.copy()
-                d["WH"] = np.ctypeslib.as_array(v.WH, shape=(v.rank, v.nrem)).T.copy()
+                d["W"] = vals(v.W, v.rank * v.nrem).reshape(v.rank, v.nrem).T.copy()
+                d["WH"] = vals(v.WH, v.rank * v.nrem).reshape(v.rank, v.nrem).T.copy()
             out.append(d)
         return out
 
     def host_apply(self, v):
-        v = np.ascontiguousarray(v, dtype=np.float64)
+        dt = np.complex128 if self.A.is_complex else np.float64
+        v = np.ascontiguousarray(v, dtype=dt)
         z = np.empty_like(v)
-        _chk(lib().pgs_gemslr_host_apply(self._h, _dptr(v), _dptr(z)), "pgs_gemslr_host_apply")
+        _chk(lib().pgs_gemslr_host_apply(self._h, _dptr(v.view(np.float64)), _dptr(z.view(np.float64))),
+             "pgs_gemslr_host_apply")
         return z
 
 
@@ -440,6 +450,8 @@ class DevOp:
     def __init__(self, gem, device=0, prec_mask=PGS_PREC_FP64 | PGS_PREC_FP32):
         self.gem = gem
         self.n = gem.A.n
+        self.is_complex = bool(gem.A.is_complex)
+        self.dtype = np.dtype(np.complex128 if self.is_complex else np.float64)
         self.device = device
         self._h = C.c_void_p()
         _chk(lib().pgs_dev_op_create(gem.handle, device, prec_mask, C.byref(self._h)), "pgs_dev_op_create")
@@ -487,8 +499,10 @@ class Workspace:
         return dict(launches=p.launches, total_ms=p.total_ms, total_bytes=p.total_bytes)
 
     def spmv(self, x, y, alpha=1.0, beta=0.0, nrhs=1, stream=None):
-        a = (C.c_double * 2)(alpha, 0.0)
-        b = (C.c_double * 2)(beta, 0.0)
+        """y = alpha A x + beta y (alpha, beta complex for a complex operator)."""
+        alpha, beta = complex(alpha), complex(beta)
+        a = (C.c_double * 2)(alpha.real, alpha.imag)
+        b = (C.c_double * 2)(beta.real, beta.imag)
         _chk(lib().pgs_spmv(self.dop._h, self._h, a, _tptr(x), b, _tptr(y), nrhs, _sh(stream)), "pgs_spmv")
 
     def apply(self, v, z, nrhs=1, prec=PGS_PREC_FP64, stream=None):
@@ -523,13 +537,15 @@ class Server:
         return i.value
 
     def submit(self, op_id, B):
-        """B: n x nrhs (or n) host array; returns a request id."""
-        B = np.asarray(B, dtype=np.float64)
+        """B: n x nrhs (or n) host array (complex128 for a complex operator);
+        returns a request id."""
+        dt = self._ops[op_id].dtype if 0 <= op_id < len(self._ops) else np.dtype(np.float64)
+        B = np.asarray(B, dtype=dt)
         if B.ndim == 1:
             B = B[:, None]
         n, nr = B.shape
         Bc = np.asfortranarray(B)
-        X = np.zeros((n, nr), dtype=np.float64, order="F")
+        X = np.zeros((n, nr), dtype=dt, order="F")
         it = np.zeros(nr, dtype=np.int32)
         r = C.c_int64()
         _chk(lib().pgs_server_submit(self._h, op_id, nr, Bc.ctypes.data_as(C.POINTER(C.c_double)),

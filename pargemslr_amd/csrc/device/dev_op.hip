@@ -63,19 +63,31 @@ void upload(DevMem& m, const std::vector<T>& v, int64_t* counter) {
   if (counter) *counter += (int64_t)(v.size() * sizeof(T));
 }
 
+// host value (double or std::complex<double>) -> device value
+template <typename To, typename From>
+struct HVal {
+  static To get(const From& v) { return (To)v; }
+};
+template <typename R>
+struct HVal<cplx<R>, zdouble> {
+  static cplx<R> get(const zdouble& v) { return cplx<R>((R)v.real(), (R)v.imag()); }
+};
+template <typename To, typename From>
+To hval(const From& v) { return HVal<To, From>::get(v); }
+
 template <typename To, typename From>
 std::vector<To> convert_vec(const std::vector<From>& v) {
   std::vector<To> o(v.size());
-  for (size_t i = 0; i < v.size(); ++i) o[i] = (To)v[i];
+  for (size_t i = 0; i < v.size(); ++i) o[i] = hval<To>(v[i]);
   return o;
 }
 
 // W (column-major m x k) -> row-major
-template <typename To>
-std::vector<To> to_row_major(const std::vector<double>& W, int m, int k) {
+template <typename To, typename From>
+std::vector<To> to_row_major(const std::vector<From>& W, int m, int k) {
   std::vector<To> o((size_t)m * k);
   for (int j = 0; j < k; ++j)
-    for (int i = 0; i < m; ++i) o[(size_t)i * k + j] = (To)W[(size_t)j * m + i];
+    for (int i = 0; i < m; ++i) o[(size_t)i * k + j] = hval<To>(W[(size_t)j * m + i]);
   return o;
 }
 
@@ -86,8 +98,8 @@ struct DevCsr {
   DevMem ia, ja, a;
 };
 
-template <typename V>
-void upload_csr(DevCsr& d, const Csr<double>& A, int64_t* counter) {
+template <typename V, typename T>
+void upload_csr(DevCsr& d, const Csr<T>& A, int64_t* counter) {
   d.nrows = A.nrows;
   d.ncols = A.ncols;
   d.nnz = A.nnz();
@@ -106,11 +118,12 @@ struct DevTrsv {
   int nblocks = 0, nloc = 0, max_rows = 0, max_steps = 0, max_width = 0;
   int window = 0;  // ring size when the packed columns are ring slots, else 0
   int64_t nnz_padded = 0, nnz_real = 0, nrows_total = 0;
+  int64_t max_block_ent = 0, max_block_steps = 0;  // largest entry and step counts of one block
   DevMem row0, step, stepU, ent0, ri0, desc, ent, ri;
 };
 
-template <typename V>
-void upload_trsv(DevTrsv& d, const TrsvPack<double>& P, int64_t* counter) {
+template <typename V, typename T>
+void upload_trsv(DevTrsv& d, const TrsvPack<T>& P, int64_t* counter) {
   d.nblocks = P.nblocks;
   d.nloc = P.nloc;
   d.max_rows = P.max_block_rows;
@@ -120,6 +133,11 @@ void upload_trsv(DevTrsv& d, const TrsvPack<double>& P, int64_t* counter) {
   d.nnz_padded = (int64_t)P.col.size();
   d.nnz_real = P.nnz_real;
   d.nrows_total = (int64_t)P.rows.size();
+  d.max_block_ent = d.max_block_steps = 0;
+  for (int b = 0; b < P.nblocks; ++b) {
+    d.max_block_ent = std::max<int64_t>(d.max_block_ent, P.blk_ent0[b + 1] - P.blk_ent0[b]);
+    d.max_block_steps = std::max<int64_t>(d.max_block_steps, P.blk_step[b + 1] - P.blk_step[b]);
+  }
   upload(d.row0, P.blk_row0, counter);
   upload(d.step, P.blk_step, counter);
   upload(d.stepU, P.blk_stepU, counter);
@@ -130,12 +148,12 @@ void upload_trsv(DevTrsv& d, const TrsvPack<double>& P, int64_t* counter) {
   for (size_t i = 0; i < P.col.size(); ++i) {
     ent[i] = TPair<V>();
     ent[i].col = P.col[i];
-    ent[i].val = (V)P.val[i];
+    ent[i].val = hval<V>(P.val[i]);
   }
   for (size_t i = 0; i < P.rows.size(); ++i) {
     ri[i] = TPair<V>();
     ri[i].col = P.rows[i];
-    ri[i].val = (V)P.dinv[i];
+    ri[i].val = hval<V>(P.dinv[i]);
   }
   ent.back() = TPair<V>();
   ri.back() = TPair<V>();
@@ -169,11 +187,13 @@ struct pgs_dev_op_s {
   int trsv_nt = 128;     // threads of the pipelined kernel
   int trsv_lpr = 4;      // lanes per row (steps hold at most trsv_nt / trsv_lpr rows)
   int64_t bytes = 0;
-  DevCsr A;  // FP64
+  DevCsr A;  // double or complex<double> values
   DevMem perm;
   std::vector<int> lev_ptr;
   DevPrec<double> p64;
   DevPrec<float> p32;
+  DevPrec<cplx<double>> pz64;  // complex operators
+  DevPrec<cplx<float>> pz32;
 };
 
 struct pgs_ws_s {
@@ -226,6 +246,13 @@ template <>
 DevPrec<double>& prec_of<double>(pgs_dev_op_s* op) { return op->p64; }
 template <>
 DevPrec<float>& prec_of<float>(pgs_dev_op_s* op) { return op->p32; }
+template <>
+DevPrec<cplx<double>>& prec_of<cplx<double>>(pgs_dev_op_s* op) { return op->pz64; }
+template <>
+DevPrec<cplx<float>>& prec_of<cplx<float>>(pgs_dev_op_s* op) { return op->pz32; }
+
+// element size of the operator's vectors (double or complex<double>)
+inline size_t vec_es(const pgs_dev_op_s* op) { return op->is_complex ? 16 : 8; }
 
 template <typename VA, typename VX, typename VY, typename VACC>
 void launch_spmv(const DevCsr& M, const VX* x, VY* y, const VY* r, VACC alpha, VACC beta, int mode, int nrhs,
@@ -325,9 +352,15 @@ void launch_trsv_pipe(int nt, int lpr, const DevTrsv& T, const V* rhs, V* x, int
 // rows), 5 slots per lane (rows of at most 20 entries).
 constexpr int kRingNT = 128, kRingLPR = 4, kRingEPL = 5;
 
+// stage depth of the ring kernel's register ring: 8 for float, 6 for double
+// and complex<float>, 4 for complex<double> (registers per stage grow with
+// the value size)
+template <typename V>
+constexpr int ring_depth() { return sizeof(V) == 4 ? 8 : sizeof(V) == 8 ? 6 : 4; }
+
 template <typename V, int CG>
 void launch_trsv_ring_cg(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
-  constexpr int D = sizeof(V) == 4 ? 8 : 6;
+  constexpr int D = ring_depth<V>();
   auto kern = k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, CG>;  // LDS limit raised in set_kernel_attrs
   const size_t lds = (size_t)(T.window + 1) * CG * sizeof(V);  // ring + one dummy slot
   const unsigned grid = (unsigned)T.nblocks * (unsigned)(nrhs / CG);
@@ -338,8 +371,11 @@ void launch_trsv_ring_cg(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x,
 
 template <typename V>
 void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
-  // the kernel addresses a block's rows of all columns with 32-bit offsets
-  if ((int64_t)T.max_rows * nrhs * (int64_t)sizeof(V) >= (int64_t)INT32_MAX)
+  // the kernel addresses a block's rows of all columns, its entries and its
+  // step descriptors with 32-bit byte offsets
+  if ((int64_t)T.max_rows * nrhs * (int64_t)sizeof(V) >= (int64_t)INT32_MAX ||
+      (T.max_block_ent + 1) * (int64_t)sizeof(TPair<V>) >= (int64_t)INT32_MAX ||
+      (T.max_block_steps + 64) * 16 >= (int64_t)INT32_MAX)
     throw std::runtime_error("diagonal block too large for 32-bit offsets");
   int cg = std::min(nrhs, op->trsv_cg);
   while (cg > 1 && (size_t)(T.window + 1) * cg * sizeof(V) > (size_t)op->lds_max) cg >>= 1;
@@ -357,9 +393,11 @@ void launch_trsv_impl(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
     return;
   }
   const size_t lds = (size_t)T.max_rows * sizeof(V);
-  if (op->trsv_variant == 0 && T.max_width <= kTrsvMaxWidth) {
-    if (lds <= (size_t)op->lds_max) launch_trsv_pipe<V, true>(op->trsv_nt, op->trsv_lpr, T, rhs, x, nrhs, lds, s);
-    else launch_trsv_pipe<V, false>(op->trsv_nt, op->trsv_lpr, T, rhs, x, nrhs, lds, s);
+  if (!scalar_traits<V>::is_complex && op->trsv_variant == 0 && T.max_width <= kTrsvMaxWidth) {
+    if constexpr (!scalar_traits<V>::is_complex) {
+      if (lds <= (size_t)op->lds_max) launch_trsv_pipe<V, true>(op->trsv_nt, op->trsv_lpr, T, rhs, x, nrhs, lds, s);
+      else launch_trsv_pipe<V, false>(op->trsv_nt, op->trsv_lpr, T, rhs, x, nrhs, lds, s);
+    }
   } else {
     k_trsv_direct<V, kTrsvNT><<<T.nblocks, kTrsvNT, 0, s>>>(T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
                                                             T.ent0.as<int64_t>(), T.ri0.as<int64_t>(),
@@ -379,10 +417,11 @@ void set_lds_limit(K kern, int lds_max) {
 
 template <typename V>
 void set_trsv_attrs(int lds_max) {
-  constexpr int D = sizeof(V) == 4 ? 8 : 6;
+  constexpr int D = ring_depth<V>();
   set_lds_limit(k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, 1>, lds_max);
   set_lds_limit(k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, 2>, lds_max);
   set_lds_limit(k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, 4>, lds_max);
+  if constexpr (scalar_traits<V>::is_complex) return;  // no pipelined kernel for complex values
   constexpr int D2 = sizeof(V) == 4 ? 5 : 4;
   set_lds_limit(k_trsv_pipe<V, true, 64, 2, 10, D2>, lds_max);
   set_lds_limit(k_trsv_pipe<V, true, 128, 2, 10, D2>, lds_max);
@@ -402,12 +441,21 @@ void set_kernel_attrs(int lds_max) {
   std::call_once(once, [lds_max] {
     set_trsv_attrs<float>(lds_max);
     set_trsv_attrs<double>(lds_max);
+    set_trsv_attrs<cplx<float>>(lds_max);
+    set_trsv_attrs<cplx<double>>(lds_max);
     set_basis_attrs<8>(lds_max);
     set_basis_attrs<16>(lds_max);
     set_basis_attrs<32>(lds_max);
     set_basis_attrs<64>(lds_max);
     set_lds_limit(k_basis_reg<kBasisNT, 32, 0, 2>, lds_max);
     set_lds_limit(k_basis_reg<kBasisNT, 32, 1, 2>, lds_max);
+    // coefficient tables of up to 127 x 64 values in LDS
+    set_lds_limit(k_multi_axpy_norm<double, kNT>, lds_max);
+    set_lds_limit(k_multi_axpy_norm<cplx<double>, kNT>, lds_max);
+    set_lds_limit(k_update_x<double, double>, lds_max);
+    set_lds_limit(k_update_x<float, double>, lds_max);
+    set_lds_limit(k_update_x<cplx<double>, cplx<double>>, lds_max);
+    set_lds_limit(k_update_x<cplx<float>, cplx<double>>, lds_max);
   });
 }
 
@@ -460,19 +508,69 @@ void solve_level(pgs_dev_op_s* op, pgs_ws_s* ws, DevPrec<V>& P, int l, const V* 
   launch_trsv<V>(op, ws, l, L.B, z, x, nrhs, s);
 }
 
-// z = M^{-1} v with v in double, z in VZ, apply arithmetic V.
-template <typename V, typename VZ>
-void apply_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* v, VZ* z, int nrhs, hipStream_t s) {
+// z = M^{-1} v with v in VI (the operator's double or complex<double>), z in
+// VZ, apply arithmetic V.
+template <typename V, typename VI, typename VZ>
+void apply_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const VI* v, VZ* z, int nrhs, hipStream_t s) {
   DevPrec<V>& P = prec_of<V>(op);
   if (!P.present) throw std::runtime_error("operator was uploaded without this precision");
   const int64_t n = op->n;
   V* r0 = ws->r0.as<V>();
   V* x0 = ws->x0.as<V>();
-  k_gather<double, V><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
+  k_gather<VI, V><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
   HIPCHK(hipGetLastError());
   solve_level<V>(op, ws, P, 0, r0, x0, nrhs, s);
   k_scatter<V, VZ><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), x0, z);
   HIPCHK(hipGetLastError());
+}
+
+// Uploads A, the permutation and every level (factors packed for the level
+// solves, E, F, W, WH) in the precisions of prec_mask: V64 for PGS_PREC_FP64,
+// V32 for PGS_PREC_FP32.
+template <typename V64, typename V32, typename T>
+void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask) {
+  op->n = H.n;
+  op->nlev = H.nlev;
+  op->lev_ptr = H.lev_ptr;
+  upload_csr<V64>(op->A, H.A, &op->bytes);
+  upload(op->perm, H.perm, &op->bytes);
+  // ring packing (ring kernel) unless another kernel geometry was asked for;
+  // the ring holds at least one column of the operator's values
+  const bool ring = op->trsv_variant == 0 && op->trsv_nt == kRingNT && op->trsv_lpr == kRingLPR;
+  int max_window = 0;
+  if (ring) {
+    max_window = 1;
+    while ((2 * max_window + 1) * (int)sizeof(V64) <= op->lds_max) max_window *= 2;  // + dummy slot
+  }
+  std::vector<TrsvPack<T>> packs;
+  for (int l = 0; l < H.nlev; ++l)
+    packs.push_back(trsv_pack(H.levels[l], H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr, max_window,
+                              kRingLPR * kRingEPL));
+  auto build = [&](auto& P, auto tag) {
+    using V = decltype(tag);
+    P.present = true;
+    P.levels.clear();
+    for (int l = 0; l < H.nlev; ++l) {
+      const HostLevel<T>& HL = H.levels[l];
+      std::unique_ptr<DevLevel> L(new DevLevel());
+      L->lev_start = H.lev_ptr[l];
+      L->nloc = H.nloc(l);
+      L->nrem = H.nrem(l);
+      upload_trsv<V>(L->B, packs[l], &op->bytes);
+      if (l + 1 < H.nlev) {
+        upload_csr<V>(L->E, HL.E, &op->bytes);
+        upload_csr<V>(L->F, HL.F, &op->bytes);
+        L->rank = HL.rank;
+        if (HL.rank > 0) {
+          upload(L->W, to_row_major<V>(HL.W, L->nrem, HL.rank), &op->bytes);
+          upload(L->WH, to_row_major<V>(HL.WH, L->nrem, HL.rank), &op->bytes);
+        }
+      }
+      P.levels.push_back(std::move(L));
+    }
+  };
+  if (prec_mask & PGS_PREC_FP64) build(prec_of<V64>(op), V64(0));
+  if (prec_mask & PGS_PREC_FP32) build(prec_of<V32>(op), V32(0));
 }
 
 int set_err(const HipError& e) {
@@ -500,7 +598,6 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
   if (!hop || !out) return PGS_ERROR_INVALID_PARAM;
   *out = nullptr;
   pgs_gemslr_s* h = reinterpret_cast<pgs_gemslr_s*>(hop);
-  if (h->is_complex) return PGS_ERR_UNSUPPORTED;
   if ((prec_mask & (PGS_PREC_FP64 | PGS_PREC_FP32)) == 0) return PGS_ERROR_INVALID_PARAM;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PGS_ERR_NO_DEVICE;
@@ -524,54 +621,15 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
     }
     if (op->trsv_lpr == 2 && op->trsv_nt == 256) op->trsv_nt = 128;
     if (op->trsv_lpr == 4 && op->trsv_nt == 64) op->trsv_nt = 128;
-    const HostOp<double>& H = *h->op_d;
-    op->n = H.n;
-    op->nlev = H.nlev;
-    op->lev_ptr = H.lev_ptr;
-    upload_csr<double>(op->A, H.A, &op->bytes);
-    upload(op->perm, H.perm, &op->bytes);
     if (const char* v = std::getenv("PGS_TRSV_CG")) {
       const int cg = std::atoi(v);
       if (cg == 1 || cg == 2 || cg == 4) op->trsv_cg = cg;
     }
-    // ring packing (ring kernel) unless another kernel geometry was asked for;
-    // the ring holds at least one FP64 column
-    const bool ring = op->trsv_variant == 0 && op->trsv_nt == kRingNT && op->trsv_lpr == kRingLPR;
-    int max_window = 0;
-    if (ring) {
-      max_window = 1;
-      while ((2 * max_window + 1) * (int)sizeof(double) <= op->lds_max) max_window *= 2;  // + dummy slot
-    }
-    std::vector<TrsvPack<double>> packs;
-    for (int l = 0; l < H.nlev; ++l) {
-      packs.push_back(trsv_pack(H.levels[l], H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr, max_window,
-                                kRingLPR * kRingEPL));
-    }
-    auto build = [&](auto& P, auto tag) {
-      using V = decltype(tag);
-      P.present = true;
-      P.levels.clear();
-      for (int l = 0; l < H.nlev; ++l) {
-        const HostLevel<double>& HL = H.levels[l];
-        std::unique_ptr<DevLevel> L(new DevLevel());
-        L->lev_start = H.lev_ptr[l];
-        L->nloc = H.nloc(l);
-        L->nrem = H.nrem(l);
-        upload_trsv<V>(L->B, packs[l], &op->bytes);
-        if (l + 1 < H.nlev) {
-          upload_csr<V>(L->E, HL.E, &op->bytes);
-          upload_csr<V>(L->F, HL.F, &op->bytes);
-          L->rank = HL.rank;
-          if (HL.rank > 0) {
-            upload(L->W, to_row_major<V>(HL.W, L->nrem, HL.rank), &op->bytes);
-            upload(L->WH, to_row_major<V>(HL.WH, L->nrem, HL.rank), &op->bytes);
-          }
-        }
-        P.levels.push_back(std::move(L));
-      }
-    };
-    if (prec_mask & PGS_PREC_FP64) build(op->p64, double(0));
-    if (prec_mask & PGS_PREC_FP32) build(op->p32, float(0));
+    op->is_complex = h->is_complex;
+    if (h->is_complex)
+      upload_op<cplx<double>, cplx<float>>(op.get(), *h->op_z, prec_mask);
+    else
+      upload_op<double, float>(op.get(), *h->op_d, prec_mask);
     HIPCHK(hipDeviceSynchronize());
   } catch (const HipError& e) {
     return set_err(e);
@@ -596,10 +654,11 @@ int pgs_dev_op_bytes(pgs_dev_op_t dop, int64_t* bytes) {
   return 0;
 }
 
-int pgs_dev_op_dims(pgs_dev_op_t dop, int* n, int* device) {
-  if (!dop || !n || !device) return PGS_ERROR_INVALID_PARAM;
+int pgs_dev_op_dims(pgs_dev_op_t dop, int* n, int* device, int* is_complex) {
+  if (!dop || !n || !device || !is_complex) return PGS_ERROR_INVALID_PARAM;
   *n = reinterpret_cast<pgs_dev_op_s*>(dop)->n;
   *device = reinterpret_cast<pgs_dev_op_s*>(dop)->device;
+  *is_complex = reinterpret_cast<pgs_dev_op_s*>(dop)->is_complex;
   return 0;
 }
 
@@ -608,12 +667,13 @@ int pgs_ws_estimate(pgs_dev_op_t dop, int max_nrhs, int kdim, int64_t* bytes) {
   const pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
   int nr = 1;
   while (nr < max_nrhs) nr <<= 1;
-  const int64_t n = op->n, vb = n * nr * 8;
+  const int64_t es = (int64_t)vec_es(op);
+  const int64_t n = op->n, vb = n * nr * es;
   int64_t b = 2 * vb + (2 * (int64_t)kdim + 1) * vb;  // r0, x0, V, Z
-  for (int l = 0; l < op->nlev; ++l) b += (n - op->lev_ptr[l]) * nr * 8;
-  b += (int64_t)1025 * (kdim + 1) * nr * 8;                // reduction partials
-  b += (int64_t)(kLrBlocks + 2) * 64 * nr * 8;             // low-rank partials
-  b += (int64_t)(kdim + 4) * (kdim + 1) * nr * 8 + 4096;   // Hessenberg, rotations, state
+  for (int l = 0; l < op->nlev; ++l) b += (n - op->lev_ptr[l]) * nr * es;
+  b += (int64_t)1025 * (kdim + 1) * nr * es;                // reduction partials
+  b += (int64_t)(kLrBlocks + 2) * 64 * nr * es;             // low-rank partials
+  b += (int64_t)(kdim + 4) * (kdim + 1) * nr * es + 4096;   // Hessenberg, rotations, state
   *bytes = b;
   return 0;
 }
@@ -631,26 +691,25 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
   try {
     HIPCHK(hipSetDevice(op->device));
     const int64_t n = op->n;
-    const size_t vb = (size_t)n * nr * sizeof(double);
+    const size_t es = vec_es(op);  // bytes of one value of the operator (double or complex<double>)
+    const size_t vb = (size_t)n * nr * es;
     ws->r0.alloc(vb);
     ws->x0.alloc(vb);
     ws->zlev.clear();
     for (int l = 0; l < op->nlev; ++l) {
       std::unique_ptr<DevMem> m(new DevMem());
-      m->alloc((size_t)(n - op->lev_ptr[l]) * nr * sizeof(double));
+      m->alloc((size_t)(n - op->lev_ptr[l]) * nr * es);
       ws->zlev.push_back(std::move(m));
     }
-    int maxrank = 0, maxm = 0;
+    int maxrank = 0;
     for (int l = 0; l + 1 < op->nlev; ++l) {
-      const DevPrec<double>& P64 = op->p64;
-      const DevPrec<float>& P32 = op->p32;
-      int rk = P64.present ? P64.levels[l]->rank : P32.levels[l]->rank;
-      maxrank = std::max(maxrank, rk);
-      maxm = std::max(maxm, n > 0 ? (int)(n - op->lev_ptr[l + 1]) : 0);
+      const auto& lv = op->p64.present ? op->p64.levels : op->p32.present ? op->p32.levels
+                       : op->pz64.present ? op->pz64.levels : op->pz32.levels;
+      maxrank = std::max(maxrank, lv[l]->rank);
     }
     const int lr_blocks = kLrBlocks + 1;
-    ws->lrpart.alloc((size_t)std::max(1, lr_blocks * maxrank * nr) * sizeof(double));
-    ws->lrt.alloc((size_t)std::max(1, maxrank * nr) * sizeof(double));
+    ws->lrpart.alloc((size_t)std::max(1, lr_blocks * maxrank * nr) * es);
+    ws->lrt.alloc((size_t)std::max(1, maxrank * nr) * es);
     // Krylov
     ws->V.alloc((size_t)(kdim + 1) * vb);
     ws->Z.alloc((size_t)kdim * vb);
@@ -660,15 +719,15 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
     const int64_t rows_per_blk = std::max<int64_t>(256, (n + 1023) / 1024);
     ws->part_blocks = (int)((n + rows_per_blk - 1) / rows_per_blk);
     if (const char* v = std::getenv("PGS_FUSED_CGS2")) ws->fused_cgs2 = std::atoi(v) != 0;
-    ws->part.alloc((size_t)ws->part_blocks * (kdim + 1) * nr * sizeof(double));
-    ws->hbuf.alloc((size_t)(kdim + 1) * nr * sizeof(double));
-    ws->h2buf.alloc((size_t)(kdim + 1) * nr * sizeof(double));
+    ws->part.alloc((size_t)ws->part_blocks * (kdim + 1) * nr * es);
+    ws->hbuf.alloc((size_t)(kdim + 1) * nr * es);
+    ws->h2buf.alloc((size_t)(kdim + 1) * nr * es);
     ws->t2.alloc((size_t)nr * sizeof(double));
-    ws->H.alloc((size_t)nr * (kdim + 1) * kdim * sizeof(double));
-    ws->cs.alloc((size_t)nr * kdim * sizeof(double));
-    ws->sn.alloc((size_t)nr * kdim * sizeof(double));
-    ws->rs.alloc((size_t)nr * (kdim + 1) * sizeof(double));
-    ws->coef.alloc((size_t)(kdim + 1) * nr * sizeof(double));
+    ws->H.alloc((size_t)nr * (kdim + 1) * kdim * es);
+    ws->cs.alloc((size_t)nr * kdim * es);
+    ws->sn.alloc((size_t)nr * kdim * es);
+    ws->rs.alloc((size_t)nr * (kdim + 1) * es);
+    ws->coef.alloc((size_t)(kdim + 1) * nr * es);
     ws->upd.alloc((size_t)nr * sizeof(int));
     ws->st.alloc((size_t)nr * sizeof(KState));
     ws->zflag.alloc((size_t)nr * sizeof(double));
@@ -699,8 +758,14 @@ int pgs_spmv(pgs_dev_op_t dop, pgs_ws_t w, const double* alpha, const double* x,
   (void)w;
   try {
     HIPCHK(hipSetDevice(op->device));
-    launch_spmv<double, double, double, double>(op->A, x, y, (const double*)nullptr, alpha[0], beta[0], 0, nrhs,
-                                                (hipStream_t)stream);
+    if (op->is_complex) {
+      using Z = cplx<double>;
+      launch_spmv<Z, Z, Z, Z>(op->A, reinterpret_cast<const Z*>(x), reinterpret_cast<Z*>(y), (const Z*)nullptr,
+                              Z(alpha[0], alpha[1]), Z(beta[0], beta[1]), 0, nrhs, (hipStream_t)stream);
+    } else {
+      launch_spmv<double, double, double, double>(op->A, x, y, (const double*)nullptr, alpha[0], beta[0], 0, nrhs,
+                                                  (hipStream_t)stream);
+    }
   } catch (const HipError& e) {
     return set_err(e);
   }
@@ -715,12 +780,25 @@ int pgs_apply(pgs_dev_op_t dop, pgs_ws_t w, const double* v, double* z, int nrhs
   try {
     HIPCHK(hipSetDevice(op->device));
     hipStream_t s = (hipStream_t)stream;
+    using Z = cplx<double>;
+    const Z* vz = reinterpret_cast<const Z*>(v);
+    Z* zz = reinterpret_cast<Z*>(z);
     if (prec == PGS_PREC_FP64) {
-      if (!op->p64.present) return PGS_ERROR_INVALID_OPTION;
-      apply_impl<double, double>(op, ws, v, z, nrhs, s);
+      if (op->is_complex) {
+        if (!op->pz64.present) return PGS_ERROR_INVALID_OPTION;
+        apply_impl<Z, Z, Z>(op, ws, vz, zz, nrhs, s);
+      } else {
+        if (!op->p64.present) return PGS_ERROR_INVALID_OPTION;
+        apply_impl<double, double, double>(op, ws, v, z, nrhs, s);
+      }
     } else if (prec == PGS_PREC_FP32) {
-      if (!op->p32.present) return PGS_ERROR_INVALID_OPTION;
-      apply_impl<float, double>(op, ws, v, z, nrhs, s);
+      if (op->is_complex) {
+        if (!op->pz32.present) return PGS_ERROR_INVALID_OPTION;
+        apply_impl<cplx<float>, Z, Z>(op, ws, vz, zz, nrhs, s);
+      } else {
+        if (!op->p32.present) return PGS_ERROR_INVALID_OPTION;
+        apply_impl<float, double, double>(op, ws, v, z, nrhs, s);
+      }
     } else {
       return PGS_ERROR_INVALID_OPTION;
     }
@@ -736,28 +814,47 @@ int pgs_apply(pgs_dev_op_t dop, pgs_ws_t w, const double* v, double* z, int nrhs
 
 namespace {
 
-// multi-dot over the first p vectors of V against w: out[j*nrhs+c]
-template <typename VV, typename VW>
-void multidot(pgs_ws_s* ws, int64_t n, int nrhs, int p, const VV* V, int64_t ldv, const VW* w, double* out,
+// out[g] (+)= sum of nblk partials of p*nrhs values of S (a complex value is
+// summed as its two doubles)
+template <typename S>
+void dot_final(int nblk, int p, int nrhs, const S* part, S* out, int accumulate, hipStream_t s) {
+  const int pp = p * (int)(sizeof(S) / sizeof(double));
+  k_dot_final_wave<<<(pp * nrhs + 3) / 4, 256, 0, s>>>(nblk, pp, nrhs, reinterpret_cast<const double*>(part),
+                                                       reinterpret_cast<double*>(out), accumulate);
+}
+
+// multi-dot over the first p vectors of V against w: out[j*nrhs+c] = V_j^H w
+template <typename VV, typename VW, typename S>
+void multidot(pgs_ws_s* ws, int64_t n, int nrhs, int p, const VV* V, int64_t ldv, const VW* w, S* out,
               int accumulate, hipStream_t s) {
   const int64_t rows_per_blk = (n + ws->part_blocks - 1) / ws->part_blocks;
   const int nblk = (int)((n + rows_per_blk - 1) / rows_per_blk);
-  k_multidot<VV, VW, kNT><<<nblk, kNT, 0, s>>>(n, nrhs, p, V, ldv, w, ws->part.as<double>(), rows_per_blk);
-  k_dot_final_wave<<<(p * nrhs + 3) / 4, 256, 0, s>>>(nblk, p, nrhs, ws->part.as<double>(), out, accumulate);
+  k_multidot<VV, VW, S, kNT><<<nblk, kNT, 0, s>>>(n, nrhs, p, V, ldv, w, ws->part.as<S>(), rows_per_blk);
+  dot_final<S>(nblk, p, nrhs, ws->part.as<S>(), out, accumulate, s);
   HIPCHK(hipGetLastError());
 }
 
-// w -= V(:, 0:p) h; with want_norm, t2[c] = ||w(:, c)||^2 of the result
-void multi_axpy(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64_t ldv, const double* h, double* w,
-                double* t2, hipStream_t s) {
+// out[c] = ||x(:, c)||^2
+template <typename S>
+void colnorm2(pgs_ws_s* ws, int64_t n, int nrhs, const S* x, double* out, hipStream_t s) {
+  const int64_t rows_per_blk = (n + ws->part_blocks - 1) / ws->part_blocks;
+  const int nblk = (int)((n + rows_per_blk - 1) / rows_per_blk);
+  k_colnorm2<S, kNT><<<nblk, kNT, 0, s>>>(n, nrhs, x, ws->part.as<double>(), rows_per_blk);
+  k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, ws->part.as<double>(), out, 0);
+  HIPCHK(hipGetLastError());
+}
+
+// w -= V(:, 0:p) h; with t2, t2[c] = ||w(:, c)||^2 of the result
+template <typename S>
+void multi_axpy(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ldv, const S* h, S* w, double* t2,
+                hipStream_t s) {
   // enough workgroups to keep every CU busy on the streaming update
   const int nblk = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n * nrhs + kNT - 1) / kNT));
-  const size_t shm = sizeof(double) * ((size_t)p * nrhs + (kNT / 64) * nrhs);
-  k_multi_axpy_norm<kNT><<<nblk, kNT, shm, s>>>(n, nrhs, p, V, ldv, h, w, t2 ? ws->part.as<double>() : nullptr);
+  const size_t shm = sizeof(S) * (size_t)p * nrhs + sizeof(double) * (kNT / 64) * nrhs;
+  k_multi_axpy_norm<S, kNT><<<nblk, kNT, shm, s>>>(n, nrhs, p, V, ldv, h, w, t2 ? ws->part.as<double>() : nullptr);
   if (t2) k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, ws->part.as<double>(), t2, 0);
   HIPCHK(hipGetLastError());
 }
-
 
 template <int PM, int MODE, int LPE>
 void launch_basis_mode(int lds_max, int nblk, size_t lds, int64_t n, int nrhs, int p, const double* V, int64_t ldv,
@@ -806,27 +903,41 @@ void basis_reg(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64_
   HIPCHK(hipGetLastError());
 }
 
-template <typename VZ>
-int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int nrhs, const pgs_fgmres_opts_t& o,
+// apply arithmetic of the two preconditioner precisions for Krylov scalar S
+template <typename S>
+struct PrecTypes {
+  using P64 = double;
+  using P32 = float;
+};
+template <>
+struct PrecTypes<cplx<double>> {
+  using P64 = cplx<double>;
+  using P32 = cplx<float>;
+};
+
+// FGMRES (FlexGmresClass::Solve, fgmres.hpp:300-540) on nrhs columns at once:
+// Krylov scalar S (double or complex<double>), preconditioned basis Z in VZ.
+template <typename S, typename VZ>
+int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, const pgs_fgmres_opts_t& o,
                 pgs_fgmres_result_t* res, hipStream_t s) {
   const int64_t n = op->n;
   const int m = o.kdim;
   const int64_t ldv = n * nrhs;
-  double* V = ws->V.as<double>();
+  S* V = ws->V.as<S>();
   VZ* Z = ws->Z.as<VZ>();
   KState* st = ws->st.as<KState>();
-  double* H = ws->H.as<double>();
-  double* rs = ws->rs.as<double>();
+  S* H = ws->H.as<S>();
+  S* rs = ws->rs.as<S>();
   const int nb = nblocks_for(n * nrhs, kNT);
   const int check = o.check_every > 0 ? o.check_every : 1;
   const int maxits = o.maxits;
   // ||b||, v0 = b - A x, ||v0||
-  multidot<double, double>(ws, n, nrhs, 1, b, ldv, b, ws->nb2.as<double>(), 0, s);
-  launch_spmv<double, double, double, double>(op->A, x, V, b, 1.0, 0.0, 1, nrhs, s);
-  multidot<double, double>(ws, n, nrhs, 1, V, ldv, V, ws->nr2.as<double>(), 0, s);
+  colnorm2<S>(ws, n, nrhs, b, ws->nb2.as<double>(), s);
+  launch_spmv<S, S, S, S>(op->A, x, V, b, S(1.0), S(0.0), 1, nrhs, s);
+  colnorm2<S>(ws, n, nrhs, V, ws->nr2.as<double>(), s);
   k_fgmres_init<<<1, 64, 0, s>>>(nrhs, ws->nb2.as<double>(), ws->nr2.as<double>(), o.tol, o.absolute_tol, st,
                                  ws->zflag.as<double>());
-  k_zero_cols<<<nb, kNT, 0, s>>>(n, nrhs, ws->zflag.as<double>(), x);
+  k_zero_cols<S><<<nb, kNT, 0, s>>>(n, nrhs, ws->zflag.as<double>(), x);
   HIPCHK(hipGetLastError());
   int* hflag = ws->flag_host;
   // number of running (restartable) or active columns, read back through pinned memory
@@ -843,47 +954,52 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int 
   // against v_0..v_{i-1}, Givens update, v_i = w / ||w||; every launch on
   // stream `ss`, nothing else (it may be captured into a graph)
   auto inner_step = [&](int i, hipStream_t ss) {
-    double* vprev = V + (int64_t)(i - 1) * ldv;
-    double* w = V + (int64_t)i * ldv;
+    S* vprev = V + (int64_t)(i - 1) * ldv;
+    S* w = V + (int64_t)i * ldv;
     VZ* zc = Z + (int64_t)(i - 1) * ldv;
     if (o.use_precond) {
       if (o.prec == PGS_PREC_FP64)
-        apply_impl<double, VZ>(op, ws, vprev, zc, nrhs, ss);
+        apply_impl<typename PrecTypes<S>::P64, S, VZ>(op, ws, vprev, zc, nrhs, ss);
       else
-        apply_impl<float, VZ>(op, ws, vprev, zc, nrhs, ss);
+        apply_impl<typename PrecTypes<S>::P32, S, VZ>(op, ws, vprev, zc, nrhs, ss);
     } else {
-      k_copy_conv<VZ><<<nb, kNT, 0, ss>>>(n * nrhs, vprev, zc);
+      k_copy_conv<S, VZ><<<nb, kNT, 0, ss>>>(n * nrhs, vprev, zc);
     }
-    launch_spmv<double, VZ, double, double>(op->A, zc, w, (const double*)nullptr, 1.0, 0.0, 0, nrhs, ss);
-    double* h = ws->hbuf.as<double>();
-    double* h2 = ws->h2buf.as<double>();
+    launch_spmv<S, VZ, S, S>(op->A, zc, w, (const S*)nullptr, S(1.0), S(0.0), 0, nrhs, ss);
+    S* h = ws->hbuf.as<S>();
+    S* h2 = ws->h2buf.as<S>();
     double* t2 = ws->t2.as<double>();
     if (o.orth == PGS_ORTH_CGS2) {
       // PargemslrCgs2 (matrixops.cpp:10588-10709): two projection passes, the
       // norm of the result accumulated by the second update.  Three sweeps
       // over the basis: h = V^T w; w -= V h fused with h2 = V^T w; w -= V h2.
-      if (ws->fused_cgs2 && basis_fits(ws, i, nrhs)) {
-        basis_reg(ws, n, nrhs, i, V, ldv, nullptr, w, h, 0, ss);
-        basis_reg(ws, n, nrhs, i, V, ldv, h, w, h2, 1, ss);
-        basis_reg(ws, n, nrhs, i, V, ldv, h2, w, t2, 2, ss);
-      } else {
-        multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h, 0, ss);
-        multi_axpy(ws, n, nrhs, i, V, ldv, h, w, nullptr, ss);
-        multidot<double, double>(ws, n, nrhs, i, V, ldv, w, h2, 0, ss);
-        multi_axpy(ws, n, nrhs, i, V, ldv, h2, w, t2, ss);
+      bool fused = false;
+      if constexpr (!scalar_traits<S>::is_complex) {
+        if (ws->fused_cgs2 && basis_fits(ws, i, nrhs)) {
+          basis_reg(ws, n, nrhs, i, V, ldv, nullptr, w, h, 0, ss);
+          basis_reg(ws, n, nrhs, i, V, ldv, h, w, h2, 1, ss);
+          basis_reg(ws, n, nrhs, i, V, ldv, h2, w, t2, 2, ss);
+          fused = true;
+        }
       }
-      k_dot_final_wave<<<(i * nrhs + 3) / 4, 256, 0, ss>>>(1, i, nrhs, h2, h, 1);
+      if (!fused) {
+        multidot<S, S, S>(ws, n, nrhs, i, V, ldv, w, h, 0, ss);
+        multi_axpy<S>(ws, n, nrhs, i, V, ldv, h, w, nullptr, ss);
+        multidot<S, S, S>(ws, n, nrhs, i, V, ldv, w, h2, 0, ss);
+        multi_axpy<S>(ws, n, nrhs, i, V, ldv, h2, w, t2, ss);
+      }
+      dot_final<S>(1, i, nrhs, h2, h, 1, ss);
     } else {
       // PargemslrMgs (matrixops.cpp:10728-10807): one (dot, axpy) pair per basis vector
       for (int j = 0; j < i; ++j) {
-        multidot<double, double>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, w, h + (int64_t)j * nrhs, 0, ss);
-        multi_axpy(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w, j == i - 1 ? t2 : nullptr,
-                   ss);
+        multidot<S, S, S>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, w, h + (int64_t)j * nrhs, 0, ss);
+        multi_axpy<S>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w,
+                      j == i - 1 ? t2 : nullptr, ss);
       }
     }
-    k_givens<<<1, 64, 0, ss>>>(nrhs, m, i, maxits, h, ws->t2.as<double>(), H, ws->cs.as<double>(),
-                               ws->sn.as<double>(), rs, st);
-    k_scale_cols<<<nb, kNT, 0, ss>>>(n, nrhs, st, w);
+    k_givens<S><<<1, 64, 0, ss>>>(nrhs, m, i, maxits, h, ws->t2.as<double>(), H, ws->cs.as<S>(), ws->sn.as<S>(),
+                                  rs, st);
+    k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w);
     HIPCHK(hipGetLastError());
   };
   // graphs of the inner steps, kept per workspace while the settings that
@@ -902,8 +1018,8 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int 
   int iters_host = 0;
   while (true) {
     if (read_flag(true) == 0) break;
-    k_cycle_start<<<1, 64, 0, s>>>(nrhs, m, st, rs);
-    k_scale_cols<<<nb, kNT, 0, s>>>(n, nrhs, st, V);
+    k_cycle_start<S><<<1, 64, 0, s>>>(nrhs, m, st, rs);
+    k_scale_cols<S><<<nb, kNT, 0, s>>>(n, nrhs, st, V);
     int i = 0;
     while (i < m && iters_host < maxits) {
       ++i;
@@ -938,13 +1054,14 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const double* b, double* x, int 
       }
     }
     // x += Z y
-    k_backsolve<<<1, 64, 0, s>>>(nrhs, m, H, rs, st, ws->coef.as<double>(), ws->upd.as<int>());
-    k_update_x<VZ><<<nb, kNT, sizeof(double) * std::max(1, i) * nrhs, s>>>(n, nrhs, i, Z, ldv, ws->coef.as<double>(), ws->upd.as<int>(), x);
+    k_backsolve<S><<<1, 64, 0, s>>>(nrhs, m, H, rs, st, ws->coef.as<S>(), ws->upd.as<int>());
+    k_update_x<VZ, S><<<nb, kNT, sizeof(S) * std::max(1, i) * nrhs, s>>>(n, nrhs, i, Z, ldv, ws->coef.as<S>(),
+                                                                        ws->upd.as<int>(), x);
     HIPCHK(hipGetLastError());
     if (iters_host >= maxits) break;
     if (read_flag(true) == 0) break;
     // restart: v0 = b - A x (fgmres.hpp:500-514); normr keeps the Givens estimate
-    launch_spmv<double, double, double, double>(op->A, x, V, b, 1.0, 0.0, 1, nrhs, s);
+    launch_spmv<S, S, S, S>(op->A, x, V, b, S(1.0), S(0.0), 1, nrhs, s);
   }
   std::vector<KState> hs(nrhs);
   HIPCHK(hipMemcpyAsync(hs.data(), st, sizeof(KState) * nrhs, hipMemcpyDeviceToHost, s));
@@ -971,14 +1088,23 @@ extern "C" int pgs_fgmres(pgs_dev_op_t dop, pgs_ws_t w, const double* b, double*
   if (opts->kdim < 1 || opts->kdim > ws->kdim || opts->maxits < 0) return PGS_ERROR_INVALID_PARAM;
   if (opts->orth != PGS_ORTH_MGS && opts->orth != PGS_ORTH_CGS2) return PGS_ERROR_INVALID_OPTION;
   if (opts->use_precond && opts->prec != PGS_PREC_FP64 && opts->prec != PGS_PREC_FP32) return PGS_ERROR_INVALID_OPTION;
-  if (opts->use_precond && opts->prec == PGS_PREC_FP64 && !op->p64.present) return PGS_ERROR_INVALID_OPTION;
-  if (opts->use_precond && opts->prec == PGS_PREC_FP32 && !op->p32.present) return PGS_ERROR_INVALID_OPTION;
+  const bool has64 = op->is_complex ? op->pz64.present : op->p64.present;
+  const bool has32 = op->is_complex ? op->pz32.present : op->p32.present;
+  if (opts->use_precond && opts->prec == PGS_PREC_FP64 && !has64) return PGS_ERROR_INVALID_OPTION;
+  if (opts->use_precond && opts->prec == PGS_PREC_FP32 && !has32) return PGS_ERROR_INVALID_OPTION;
   try {
     HIPCHK(hipSetDevice(op->device));
     hipStream_t s = (hipStream_t)stream;
-    if (opts->use_precond && opts->prec == PGS_PREC_FP32)
-      return fgmres_impl<float>(op, ws, b, x, nrhs, *opts, res, s);
-    return fgmres_impl<double>(op, ws, b, x, nrhs, *opts, res, s);
+    const bool low = opts->use_precond && opts->prec == PGS_PREC_FP32;
+    if (op->is_complex) {
+      using Z = cplx<double>;
+      const Z* bz = reinterpret_cast<const Z*>(b);
+      Z* xz = reinterpret_cast<Z*>(x);
+      if (low) return fgmres_impl<Z, cplx<float>>(op, ws, bz, xz, nrhs, *opts, res, s);
+      return fgmres_impl<Z, Z>(op, ws, bz, xz, nrhs, *opts, res, s);
+    }
+    if (low) return fgmres_impl<double, float>(op, ws, b, x, nrhs, *opts, res, s);
+    return fgmres_impl<double, double>(op, ws, b, x, nrhs, *opts, res, s);
   } catch (const HipError& e) {
     return set_err(e);
   } catch (const std::exception&) {

@@ -16,6 +16,14 @@ namespace dev {
 
 constexpr int kWave = 64;
 
+// lane shuffle of a real or complex value
+template <typename V>
+__device__ __forceinline__ V shfl_xor_v(V v, int m, int width = kWave) { return __shfl_xor(v, m, width); }
+template <typename R>
+__device__ __forceinline__ cplx<R> shfl_xor_v(cplx<R> v, int m, int width = kWave) {
+  return cplx<R>(__shfl_xor(v.re, m, width), __shfl_xor(v.im, m, width));
+}
+
 template <typename To, typename From>
 __device__ __forceinline__ To cvt(From v) { return (To)v; }
 
@@ -67,7 +75,7 @@ __global__ void k_spmv1(int nrows, const int* __restrict__ ia, const int* __rest
     for (int k = ia[row] + q; k < k1; k += LPR) acc += (VACC)a[k] * (VACC)x[ja[k]];
   }
 #pragma unroll
-  for (int m = LPR / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m, LPR);
+  for (int m = LPR / 2; m > 0; m >>= 1) acc += shfl_xor_v(acc, m, LPR);
   if (row < nrows && q == 0) {
     if (mode == 1) y[row] = cvt<VY>((VACC)r[row] - acc);
     else if (beta == (VACC)0) y[row] = cvt<VY>(alpha * acc);
@@ -113,7 +121,7 @@ __global__ __launch_bounds__(NT) void k_lr_dot(int m, int k, int nrhs, const V* 
   const int i1 = min(m, i0 + rows_per_blk);
   for (int j = 0; j < k; ++j) {
     V acc = 0;
-    for (int i = i0 + rl; i < i1; i += lanes) acc += W[(int64_t)i * k + j] * z[(int64_t)i * nrhs + c];
+    for (int i = i0 + rl; i < i1; i += lanes) acc += sconj(W[(int64_t)i * k + j]) * z[(int64_t)i * nrhs + c];
     sh[tid] = acc;
     __syncthreads();
     for (int s = NT / 2; s >= nrhs; s >>= 1) {
@@ -127,7 +135,7 @@ __global__ __launch_bounds__(NT) void k_lr_dot(int m, int k, int nrhs, const V* 
 
 // Stage 1 with the k partial sums of an element in registers (KM >= k,
 // loops unrolled to KM with the uniform guard j < k): every lane walks the
-// elements g = (i, c) of z with a grid stride and adds W[i, j] * z[i, c] to
+// elements g = (i, c) of z with a grid stride and adds conj(W[i, j]) * z[i, c] to
 // sum j; the sums are then reduced over the lanes of a column and over the
 // waves into part[blk * k * nrhs + j * nrhs + c].  Dynamic LDS:
 // (NT / 64) * k * nrhs values of V.
@@ -146,14 +154,14 @@ __global__ __launch_bounds__(NT) void k_lr_dot_reg(int m, int k, int nrhs, const
     const V* wr = W + (g / nrhs) * k;
 #pragma unroll
     for (int j = 0; j < KM; ++j)
-      if (j < k) acc[j] += wr[j] * zv;
+      if (j < k) acc[j] += sconj(wr[j]) * zv;
   }
   const int wid = tid / 64, lane = tid % 64;
 #pragma unroll
   for (int j = 0; j < KM; ++j) {
     if (j < k) {
       V s = acc[j];
-      for (int o = 32; o >= nrhs; o >>= 1) s += __shfl_xor(s, o);
+      for (int o = 32; o >= nrhs; o >>= 1) s += shfl_xor_v(s, o);
       if (lane < nrhs) red[(wid * k + j) * nrhs + lane] = s;
     }
   }
@@ -174,7 +182,7 @@ __global__ void k_lr_final(int nblk, int k, int nrhs, const V* __restrict__ part
   V s = 0;
   for (int b = lane; b < nblk; b += 64) s += part[(int64_t)b * k * nrhs + g];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  for (int o = 32; o > 0; o >>= 1) s += shfl_xor_v(s, o);
   if (lane == 0) t[g] = s;
 }
 
@@ -201,13 +209,13 @@ struct KState {
   int icur;     // inner steps of the current cycle
 };
 
-// partial dot products d[j, c] = sum_rows V_j[r, c] * w[r, c] for j < p
-// (w may be one of the V_j).  One partial per workgroup.
-template <typename VV, typename VW, int NT>
+// partial dot products d[j, c] = sum_rows conj(V_j[r, c]) * w[r, c] for j < p
+// (w may be one of the V_j), accumulated in S.  One partial per workgroup.
+template <typename VV, typename VW, typename S, int NT>
 __global__ __launch_bounds__(NT) void k_multidot(int64_t n, int nrhs, int p, const VV* __restrict__ V, int64_t ldv,
-                                                 const VW* __restrict__ w, double* __restrict__ part,
+                                                 const VW* __restrict__ w, S* __restrict__ part,
                                                  int64_t rows_per_blk) {
-  __shared__ double sh[NT];
+  __shared__ S sh[NT];
   const int tid = threadIdx.x;
   const int c = tid % nrhs;
   const int lanes = NT / nrhs;
@@ -216,19 +224,42 @@ __global__ __launch_bounds__(NT) void k_multidot(int64_t n, int nrhs, int p, con
   const int64_t i1 = min(n, i0 + rows_per_blk);
   for (int j = 0; j < p; ++j) {
     const VV* vj = V + j * ldv;
-    double acc = 0;
-    for (int64_t i = i0 + rl; i < i1; i += lanes) acc += (double)vj[i * nrhs + c] * (double)w[i * nrhs + c];
+    S acc = 0;
+    for (int64_t i = i0 + rl; i < i1; i += lanes) acc += sconj((S)vj[i * nrhs + c]) * (S)w[i * nrhs + c];
     // reduce across lanes that share column c
-    for (int m = kWave / 2; m >= nrhs; m >>= 1) acc += __shfl_xor(acc, m);
+    for (int m = kWave / 2; m >= nrhs; m >>= 1) acc += shfl_xor_v(acc, m);
     const int wid = tid / kWave, lane = tid % kWave;
     if (lane < nrhs) sh[wid * nrhs + lane] = acc;
     __syncthreads();
     if (tid < nrhs) {
-      double s = 0;
+      S s = 0;
       for (int q = 0; q < NT / kWave; ++q) s += sh[q * nrhs + tid];
       part[((int64_t)blockIdx.x * p + j) * nrhs + tid] = s;
     }
     __syncthreads();
+  }
+}
+
+// per-workgroup partial sums of |x[r, c]|^2
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void k_colnorm2(int64_t n, int nrhs, const S* __restrict__ x,
+                                                 double* __restrict__ part, int64_t rows_per_blk) {
+  __shared__ double sh[NT];
+  const int tid = threadIdx.x;
+  const int c = tid % nrhs;
+  const int lanes = NT / nrhs;
+  const int64_t i0 = (int64_t)blockIdx.x * rows_per_blk;
+  const int64_t i1 = min(n, i0 + rows_per_blk);
+  double acc = 0;
+  for (int64_t i = i0 + tid / nrhs; i < i1; i += lanes) acc += (double)sabs2(x[i * nrhs + c]);
+  for (int m = kWave / 2; m >= nrhs; m >>= 1) acc += __shfl_xor(acc, m);
+  const int wid = tid / kWave, lane = tid % kWave;
+  if (lane < nrhs) sh[wid * nrhs + lane] = acc;
+  __syncthreads();
+  if (tid < nrhs) {
+    double s = 0;
+    for (int q = 0; q < NT / kWave; ++q) s += sh[q * nrhs + tid];
+    part[(int64_t)blockIdx.x * nrhs + tid] = s;
   }
 }
 
@@ -259,14 +290,15 @@ __global__ __launch_bounds__(256) void k_dot_final_wave(int nblk, int p, int nrh
 }
 
 // w[r, c] -= sum_{j<p} V_j[r, c] * h[j, c]  and, when part != nullptr, the
-// per-workgroup partial sums of w[r, c]^2 (part[blk * nrhs + c]).
-// Dynamic shared memory: p*nrhs + (NT/64)*nrhs doubles.
-template <int NT>
-__global__ __launch_bounds__(NT) void k_multi_axpy_norm(int64_t n, int nrhs, int p, const double* __restrict__ V,
-                                                        int64_t ldv, const double* __restrict__ h,
-                                                        double* __restrict__ w, double* __restrict__ part) {
-  extern __shared__ double hs[];
-  double* red = hs + p * nrhs;
+// per-workgroup partial sums of |w[r, c]|^2 (part[blk * nrhs + c]).
+// Dynamic shared memory: p*nrhs values of S, then (NT/64)*nrhs doubles.
+template <typename S, int NT>
+__global__ __launch_bounds__(NT) void k_multi_axpy_norm(int64_t n, int nrhs, int p, const S* __restrict__ V,
+                                                        int64_t ldv, const S* __restrict__ h,
+                                                        S* __restrict__ w, double* __restrict__ part) {
+  extern __shared__ unsigned char ma_smem[];
+  S* hs = reinterpret_cast<S*>(ma_smem);
+  double* red = reinterpret_cast<double*>(hs + p * nrhs);
   for (int t = threadIdx.x; t < p * nrhs; t += NT) hs[t] = h[t];
   __syncthreads();
   const int tid = threadIdx.x;
@@ -278,10 +310,10 @@ __global__ __launch_bounds__(NT) void k_multi_axpy_norm(int64_t n, int nrhs, int
   double sq = 0;
   for (int64_t i = i0 + tid / nrhs; i < i1; i += lanes) {
     const int64_t g = i * nrhs + c;
-    double acc = w[g];
+    S acc = w[g];
     for (int j = 0; j < p; ++j) acc -= V[j * ldv + g] * hs[j * nrhs + c];
     w[g] = acc;
-    sq += acc * acc;
+    sq += (double)sabs2(acc);
   }
   if (part == nullptr) return;  // uniform
   for (int m = kWave / 2; m >= nrhs; m >>= 1) sq += __shfl_xor(sq, m);
@@ -395,10 +427,11 @@ __global__ void k_multi_axpy(int64_t n, int nrhs, int p, const double* __restric
 }
 
 // v[r, c] *= s[c]   (s from KState.scale)
-__global__ void k_scale_cols(int64_t n, int nrhs, const KState* __restrict__ st, double* __restrict__ v) {
+template <typename S>
+__global__ void k_scale_cols(int64_t n, int nrhs, const KState* __restrict__ st, S* __restrict__ v) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * nrhs) return;
-  v[g] *= st[g % nrhs].scale;
+  v[g] *= S(st[g % nrhs].scale);
 }
 
 // FGMRES start (fgmres.hpp:318-359): normb, normr from squared norms.
@@ -429,21 +462,23 @@ __global__ void k_fgmres_init(int nrhs, const double* __restrict__ nb2, const do
 }
 
 // x[r, c] = 0 where flag[c] != 0
-__global__ void k_zero_cols(int64_t n, int nrhs, const double* __restrict__ flag, double* __restrict__ x) {
+template <typename S>
+__global__ void k_zero_cols(int64_t n, int nrhs, const double* __restrict__ flag, S* __restrict__ x) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * nrhs) return;
-  if (flag[g % nrhs] != 0.0) x[g] = 0.0;
+  if (flag[g % nrhs] != 0.0) x[g] = S(0.0);
 }
 
 // Start of a restart cycle (fgmres.hpp:382-386): rs[0] = normr, v0 /= normr.
-__global__ void k_cycle_start(int nrhs, int m, KState* __restrict__ st, double* __restrict__ rs) {
+template <typename S>
+__global__ void k_cycle_start(int nrhs, int m, KState* __restrict__ st, S* __restrict__ rs) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
   KState s = st[c];
   if (s.status == 0) {
     s.active = 1;
     s.icur = 0;
-    rs[(int64_t)c * (m + 1)] = s.normr;
+    rs[(int64_t)c * (m + 1)] = S(s.normr);
     s.scale = 1.0 / s.normr;
   } else {
     s.active = 0;
@@ -452,11 +487,14 @@ __global__ void k_cycle_start(int nrhs, int m, KState* __restrict__ st, double* 
   st[c] = s;
 }
 
-// Hessenberg column + Givens update for inner step i (1-based), fgmres.hpp:424-461.
-// hcol[j*nrhs + c], j <= i-1 : orthogonalization coefficients; t2[c] = ||w||^2.
-__global__ void k_givens(int nrhs, int m, int i, int maxits, const double* __restrict__ hcol,
-                         const double* __restrict__ t2, double* __restrict__ H, double* __restrict__ cs,
-                         double* __restrict__ sn, double* __restrict__ rs, KState* __restrict__ st) {
+// Hessenberg column + Givens update for inner step i (1-based), fgmres.hpp:424-461
+// (complex: c = h_ii / gamma and s = h_i+1,i / gamma, rotations applied with
+// conj(c) as the reference does).  hcol[j*nrhs + c], j <= i-1 : orthogonalization
+// coefficients; t2[c] = ||w||^2.
+template <typename S>
+__global__ void k_givens(int nrhs, int m, int i, int maxits, const S* __restrict__ hcol,
+                         const double* __restrict__ t2, S* __restrict__ H, S* __restrict__ cs,
+                         S* __restrict__ sn, S* __restrict__ rs, KState* __restrict__ st) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
   KState s = st[c];
@@ -466,25 +504,25 @@ __global__ void k_givens(int nrhs, int m, int i, int maxits, const double* __res
     return;
   }
   const double eps = 2.220446049250313e-16;
-  double* Hc = H + (int64_t)c * (m + 1) * m;
-  double* csc = cs + (int64_t)c * m;
-  double* snc = sn + (int64_t)c * m;
-  double* rsc = rs + (int64_t)c * (m + 1);
+  S* Hc = H + (int64_t)c * (m + 1) * m;
+  S* csc = cs + (int64_t)c * m;
+  S* snc = sn + (int64_t)c * m;
+  S* rsc = rs + (int64_t)c * (m + 1);
   const int col = i - 1;
   for (int j = 0; j < i; ++j) Hc[j + col * (m + 1)] = hcol[j * nrhs + c];
   const double t = sqrt(t2[c]);
-  Hc[i + col * (m + 1)] = t;
+  Hc[i + col * (m + 1)] = S(t);
   s.iters += 1;
   s.icur = i;
   for (int j = 1; j < i; ++j) {
-    double hii = Hc[(j - 1) + col * (m + 1)];
-    double hj = Hc[j + col * (m + 1)];
-    Hc[(j - 1) + col * (m + 1)] = csc[j - 1] * hii + snc[j - 1] * hj;
+    const S hii = Hc[(j - 1) + col * (m + 1)];
+    const S hj = Hc[j + col * (m + 1)];
+    Hc[(j - 1) + col * (m + 1)] = sconj(csc[j - 1]) * hii + snc[j - 1] * hj;
     Hc[j + col * (m + 1)] = -snc[j - 1] * hii + csc[j - 1] * hj;
   }
-  double hii = Hc[(i - 1) + col * (m + 1)];
-  double hii1 = Hc[i + col * (m + 1)];
-  double gam = sqrt(hii * hii + hii1 * hii1);
+  const S hii = Hc[(i - 1) + col * (m + 1)];
+  const S hii1 = Hc[i + col * (m + 1)];
+  const double gam = sqrt((double)sabs2(hii) + (double)sabs2(hii1));
   if (fabs(gam) < eps) {  // breakdown: the reference leaves x untouched (fgmres.hpp:440-443)
     s.status = 2;
     s.active = 0;
@@ -492,12 +530,12 @@ __global__ void k_givens(int nrhs, int m, int i, int maxits, const double* __res
     st[c] = s;
     return;
   }
-  csc[i - 1] = hii / gam;
-  snc[i - 1] = hii1 / gam;
+  csc[i - 1] = hii / S(gam);
+  snc[i - 1] = hii1 / S(gam);
   rsc[i] = -snc[i - 1] * rsc[i - 1];
-  rsc[i - 1] = csc[i - 1] * rsc[i - 1];
-  Hc[(i - 1) + col * (m + 1)] = csc[i - 1] * hii + snc[i - 1] * hii1;
-  s.normr = fabs(rsc[i]);
+  rsc[i - 1] = sconj(csc[i - 1]) * rsc[i - 1];
+  Hc[(i - 1) + col * (m + 1)] = sconj(csc[i - 1]) * hii + snc[i - 1] * hii1;
+  s.normr = sabs_d(rsc[i]);
   s.scale = (t > 0.0) ? 1.0 / t : 0.0;
   if (s.normr <= s.tolr) {
     s.status = 1;
@@ -509,16 +547,17 @@ __global__ void k_givens(int nrhs, int m, int i, int maxits, const double* __res
 }
 
 // Back substitution on the triangularized H (fgmres.hpp:471-480); coef[j*nrhs+c].
-__global__ void k_backsolve(int nrhs, int m, const double* __restrict__ H, double* __restrict__ rs,
-                            const KState* __restrict__ st, double* __restrict__ coef, int* __restrict__ upd) {
+template <typename S>
+__global__ void k_backsolve(int nrhs, int m, const S* __restrict__ H, S* __restrict__ rs,
+                            const KState* __restrict__ st, S* __restrict__ coef, int* __restrict__ upd) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
   KState s = st[c];
-  for (int j = 0; j < m; ++j) coef[j * nrhs + c] = 0.0;
+  for (int j = 0; j < m; ++j) coef[j * nrhs + c] = S(0.0);
   upd[c] = 0;
   if (s.status == 2 || s.status == 4 || s.icur == 0) return;
-  const double* Hc = H + (int64_t)c * (m + 1) * m;
-  double* rsc = rs + (int64_t)c * (m + 1);
+  const S* Hc = H + (int64_t)c * (m + 1) * m;
+  S* rsc = rs + (int64_t)c * (m + 1);
   const int i = s.icur;
   rsc[i - 1] /= Hc[(i - 1) + (i - 1) * (m + 1)];
   for (int k = i - 2; k >= 0; --k) {
@@ -531,18 +570,19 @@ __global__ void k_backsolve(int nrhs, int m, const double* __restrict__ H, doubl
 
 // x[r, c] += sum_j coef[j, c] * Z_j[r, c] for columns with upd[c]
 // (dynamic shared memory: p*nrhs doubles)
-template <typename VZ>
+template <typename VZ, typename S>
 __global__ void k_update_x(int64_t n, int nrhs, int p, const VZ* __restrict__ Z, int64_t ldz,
-                           const double* __restrict__ coef, const int* __restrict__ upd, double* __restrict__ x) {
-  extern __shared__ double cs[];
+                           const S* __restrict__ coef, const int* __restrict__ upd, S* __restrict__ x) {
+  extern __shared__ unsigned char ux_smem[];
+  S* cs = reinterpret_cast<S*>(ux_smem);
   for (int t = threadIdx.x; t < p * nrhs; t += blockDim.x) cs[t] = coef[t];
   __syncthreads();
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * nrhs) return;
   const int c = (int)(g % nrhs);
   if (!upd[c]) return;
-  double acc = x[g];
-  for (int j = 0; j < p; ++j) acc += cs[j * nrhs + c] * (double)Z[j * ldz + g];
+  S acc = x[g];
+  for (int j = 0; j < p; ++j) acc += cs[j * nrhs + c] * (S)Z[j * ldz + g];
   x[g] = acc;
 }
 
@@ -563,8 +603,8 @@ __global__ void k_count_active(int nrhs, const KState* __restrict__ st, int* __r
 }
 
 // copy v (double) into z (VZ) : identity preconditioner
-template <typename VZ>
-__global__ void k_copy_conv(int64_t m, const double* __restrict__ in, VZ* __restrict__ out) {
+template <typename S, typename VZ>
+__global__ void k_copy_conv(int64_t m, const S* __restrict__ in, VZ* __restrict__ out) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m) out[t] = (VZ)in[t];
 }

@@ -22,6 +22,54 @@
 namespace pgs {
 namespace dev {
 
+// Complex value of the device path: (re, im), arithmetic as in std::complex.
+template <typename R>
+struct alignas(2 * sizeof(R)) cplx {
+  R re, im;
+  cplx() = default;
+  __host__ __device__ constexpr cplx(R r) : re(r), im(0) {}
+  __host__ __device__ constexpr cplx(R r, R i) : re(r), im(i) {}
+  template <typename Q>
+  __host__ __device__ explicit constexpr cplx(const cplx<Q>& o) : re((R)o.re), im((R)o.im) {}
+  __host__ __device__ cplx& operator+=(const cplx& o) { re += o.re; im += o.im; return *this; }
+  __host__ __device__ cplx& operator-=(const cplx& o) { re -= o.re; im -= o.im; return *this; }
+  __host__ __device__ cplx& operator*=(const cplx& o) {
+    const R r = re * o.re - im * o.im;
+    im = re * o.im + im * o.re;
+    re = r;
+    return *this;
+  }
+  __host__ __device__ cplx& operator/=(const cplx& o) {
+    const R d = o.re * o.re + o.im * o.im;
+    const R r = (re * o.re + im * o.im) / d;
+    im = (im * o.re - re * o.im) / d;
+    re = r;
+    return *this;
+  }
+};
+template <typename R> __host__ __device__ inline cplx<R> operator+(cplx<R> a, const cplx<R>& b) { return a += b; }
+template <typename R> __host__ __device__ inline cplx<R> operator-(cplx<R> a, const cplx<R>& b) { return a -= b; }
+template <typename R> __host__ __device__ inline cplx<R> operator*(cplx<R> a, const cplx<R>& b) { return a *= b; }
+template <typename R> __host__ __device__ inline cplx<R> operator/(cplx<R> a, const cplx<R>& b) { return a /= b; }
+template <typename R> __host__ __device__ inline cplx<R> operator-(const cplx<R>& a) { return cplx<R>(-a.re, -a.im); }
+template <typename R> __host__ __device__ inline bool operator==(const cplx<R>& a, const cplx<R>& b) {
+  return a.re == b.re && a.im == b.im;
+}
+
+// Scalar traits: real type, conjugate, squared magnitude.
+template <typename V> struct scalar_traits { using real = V; static constexpr bool is_complex = false; };
+template <typename R> struct scalar_traits<cplx<R>> { using real = R; static constexpr bool is_complex = true; };
+template <typename V> __host__ __device__ inline V sconj(const V& v) { return v; }
+template <typename R> __host__ __device__ inline cplx<R> sconj(const cplx<R>& v) { return cplx<R>(v.re, -v.im); }
+template <typename V> __host__ __device__ inline V sabs2(const V& v) { return v * v; }
+template <typename R> __host__ __device__ inline R sabs2(const cplx<R>& v) { return v.re * v.re + v.im * v.im; }
+template <typename V> __host__ __device__ inline double sabs_d(const V& v) { return fabs((double)v); }
+template <typename R> __host__ __device__ inline double sabs_d(const cplx<R>& v) {
+  return sqrt((double)v.re * v.re + (double)v.im * v.im);
+}
+template <typename V> __host__ __device__ inline V sreal(const V& v) { return v; }
+template <typename R> __host__ __device__ inline R sreal(const cplx<R>& v) { return v.re; }
+
 template <typename V>
 struct alignas(8) TPair;
 template <>
@@ -34,6 +82,18 @@ struct alignas(16) TPair<double> {  // value first: a 12-byte load takes (value,
   double val;
   int col;
   int pad;
+};
+template <>
+struct alignas(16) TPair<cplx<float>> {  // one 12-byte load takes (value, column)
+  cplx<float> val;
+  int col;
+  int pad;
+};
+template <>
+struct alignas(16) TPair<cplx<double>> {  // a 16-byte load takes the value, a 4-byte load the column
+  cplx<double> val;
+  int col;
+  int pad[3];
 };
 
 constexpr int kBufferRsrcWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit dwords, no swizzle
@@ -60,6 +120,21 @@ __device__ __forceinline__ TPair<double> bload_pair(__amdgpu_buffer_rsrc_t r, in
   p.val = __hiloint2double((int)v[1], (int)v[0]);
   p.col = (int)v[2];
   p.pad = 0;
+  return p;
+}
+__device__ __forceinline__ TPair<cplx<float>> bload_pair(__amdgpu_buffer_rsrc_t r, int off, cplx<float>) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
+  TPair<cplx<float>> p;
+  p.val = cplx<float>(__uint_as_float(v[0]), __uint_as_float(v[1]));
+  p.col = (int)v[2];
+  p.pad = 0;
+  return p;
+}
+__device__ __forceinline__ TPair<cplx<double>> bload_pair(__amdgpu_buffer_rsrc_t r, int off, cplx<double>) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  TPair<cplx<double>> p;
+  p.val = cplx<double>(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
+  p.col = (int)__builtin_amdgcn_raw_buffer_load_b32(r, off + 16, 0, 0);
   return p;
 }
 // The first three fields of a step descriptor (row offset, entry offset, rows).
@@ -104,6 +179,14 @@ __device__ __forceinline__ float dpp_xor2(float v) {
 }
 __device__ __forceinline__ double dpp_xor1(double v) { return quad_perm_f64(v, 1); }
 __device__ __forceinline__ double dpp_xor2(double v) { return quad_perm_f64(v, 2); }
+template <typename R>
+__device__ __forceinline__ cplx<R> dpp_xor1(cplx<R> v) { return cplx<R>(dpp_xor1(v.re), dpp_xor1(v.im)); }
+template <typename R>
+__device__ __forceinline__ cplx<R> dpp_xor2(cplx<R> v) { return cplx<R>(dpp_xor2(v.re), dpp_xor2(v.im)); }
+template <typename R>
+__device__ __forceinline__ cplx<R> quad_sum(cplx<R> v) { return cplx<R>(quad_sum(v.re), quad_sum(v.im)); }
+template <typename R>
+__device__ __forceinline__ cplx<R> pair_sum(cplx<R> v) { return cplx<R>(pair_sum(v.re), pair_sum(v.im)); }
 
 // Direct kernel: any width, nrhs columns (row-interleaved); one thread per
 // (row of the step, column), plain loads.  Used for levels whose rows are too
@@ -293,6 +376,20 @@ __device__ __forceinline__ void bstore_val(__amdgpu_buffer_rsrc_t r, int off, fl
 }
 __device__ __forceinline__ void bstore_val(__amdgpu_buffer_rsrc_t r, int off, double v) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, off, 0, 0);
+}
+__device__ __forceinline__ cplx<float> bload_val(__amdgpu_buffer_rsrc_t r, int off, cplx<float>) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  return cplx<float>(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+__device__ __forceinline__ cplx<double> bload_val(__amdgpu_buffer_rsrc_t r, int off, cplx<double>) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return cplx<double>(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
+}
+__device__ __forceinline__ void bstore_val(__amdgpu_buffer_rsrc_t r, int off, cplx<float> v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void bstore_val(__amdgpu_buffer_rsrc_t r, int off, cplx<double> v) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, off, 0, 0);
 }
 
 template <typename V, int NT, int LPR, int EPL, int D, int CG, bool UPPER>

@@ -50,6 +50,7 @@ struct Request {
 struct Operator {
   pgs_dev_op_t dop = nullptr;
   int n = 0;
+  int dpv = 1;  // doubles per value: 1 (double) or 2 (complex<double>)
 };
 
 // a workspace of one worker for one operator, with its batch buffers
@@ -90,9 +91,10 @@ void free_slot(int device, Slot& s) {
 int alloc_slot(pgs_server_s* S, const Operator& op, Slot& s) {
   const int nb = S->opts.max_batch;
   int rc = pgs_ws_create(op.dop, nb, S->opts.kdim, &s.ws);
-  if (!rc) rc = pgs_dev_malloc(S->device, (int64_t)op.n * nb * 8, (void**)&s.B);
-  if (!rc) rc = pgs_dev_malloc(S->device, (int64_t)op.n * nb * 8, (void**)&s.X);
-  if (!rc && hipHostMalloc((void**)&s.stage, (size_t)op.n * nb * 8, hipHostMallocDefault) != hipSuccess)
+  const int64_t vb = (int64_t)op.n * nb * op.dpv * 8;
+  if (!rc) rc = pgs_dev_malloc(S->device, vb, (void**)&s.B);
+  if (!rc) rc = pgs_dev_malloc(S->device, vb, (void**)&s.X);
+  if (!rc && hipHostMalloc((void**)&s.stage, (size_t)vb, hipHostMallocDefault) != hipSuccess)
     rc = PGS_ERR_OOM;
   if (rc) free_slot(S->device, s);
   return rc;
@@ -101,7 +103,7 @@ int alloc_slot(pgs_server_s* S, const Operator& op, Slot& s) {
 int64_t slot_bytes(pgs_server_s* S, const Operator& op) {
   int64_t ws = 0;
   if (pgs_ws_estimate(op.dop, S->opts.max_batch, S->opts.kdim, &ws)) return -1;
-  return ws + 2 * (int64_t)op.n * S->opts.max_batch * 8;
+  return ws + 2 * (int64_t)op.n * S->opts.max_batch * op.dpv * 8;
 }
 
 bool memory_fits(pgs_server_s* S, int64_t need) {
@@ -115,6 +117,7 @@ void run_batch(pgs_server_s* S, void* stream, std::map<int, Slot>& cache, std::v
   const int op_id = batch[0]->op;
   const Operator& op = S->ops[op_id];
   const int n = op.n;
+  const int dpv = op.dpv;
   int rc = 0;
   Slot* sl = nullptr;
   auto it = cache.find(op_id);
@@ -144,18 +147,20 @@ void run_batch(pgs_server_s* S, void* stream, std::map<int, Slot>& cache, std::v
   std::vector<pgs_fgmres_result_t> res(nb);
   if (!rc) {
     // pack the columns row-interleaved; unused columns are zero (no iterations)
-    std::memset(sl->stage, 0, sizeof(double) * (size_t)n * nb);
+    const int64_t vb = (int64_t)n * nb * dpv * 8;
+    std::memset(sl->stage, 0, (size_t)vb);
     int co = 0;
     for (Request* r : batch) {
       for (int c = 0; c < r->nrhs; ++c) {
-        const double* bc = r->b + (int64_t)c * n;
-        double* dst = sl->stage + co + c;
-        for (int i = 0; i < n; ++i) dst[(int64_t)i * nb] = bc[i];
+        const double* bc = r->b + (int64_t)c * n * dpv;
+        double* dst = sl->stage + (int64_t)(co + c) * dpv;
+        for (int i = 0; i < n; ++i)
+          for (int q = 0; q < dpv; ++q) dst[(int64_t)i * nb * dpv + q] = bc[(int64_t)i * dpv + q];
       }
       co += r->nrhs;
     }
-    rc = pgs_memcpy_htod(S->device, sl->B, sl->stage, (int64_t)n * nb * 8, stream);
-    if (!rc) rc = pgs_memset_zero(S->device, sl->X, (int64_t)n * nb * 8, stream);
+    rc = pgs_memcpy_htod(S->device, sl->B, sl->stage, vb, stream);
+    if (!rc) rc = pgs_memset_zero(S->device, sl->X, vb, stream);
     pgs_fgmres_opts_t o;
     o.kdim = S->opts.kdim;
     o.maxits = S->opts.maxits;
@@ -167,7 +172,7 @@ void run_batch(pgs_server_s* S, void* stream, std::map<int, Slot>& cache, std::v
     o.check_every = S->opts.check_every;
     o.use_graph = S->opts.use_graph;
     if (!rc) rc = pgs_fgmres(op.dop, sl->ws, sl->B, sl->X, nb, &o, res.data(), stream);
-    if (!rc) rc = pgs_memcpy_dtoh(S->device, sl->stage, sl->X, (int64_t)n * nb * 8, stream);
+    if (!rc) rc = pgs_memcpy_dtoh(S->device, sl->stage, sl->X, (int64_t)n * nb * dpv * 8, stream);
   }
   int co = 0;
   for (Request* r : batch) {
@@ -175,9 +180,10 @@ void run_batch(pgs_server_s* S, void* stream, std::map<int, Slot>& cache, std::v
     r->batch_nrhs = nb;
     for (int c = 0; c < r->nrhs; ++c) {
       if (!rc) {
-        double* xc = r->x + (int64_t)c * n;
-        const double* src = sl->stage + co + c;
-        for (int i = 0; i < n; ++i) xc[i] = src[(int64_t)i * nb];
+        double* xc = r->x + (int64_t)c * n * dpv;
+        const double* src = sl->stage + (int64_t)(co + c) * dpv;
+        for (int i = 0; i < n; ++i)
+          for (int q = 0; q < dpv; ++q) xc[(int64_t)i * dpv + q] = src[(int64_t)i * nb * dpv + q];
         if (r->status == 0 && res[co + c].status != 0) r->status = res[co + c].status;
       }
       if (r->iters) r->iters[c] = rc ? 0 : res[co + c].iters;
@@ -261,9 +267,10 @@ int pgs_server_add_operator(pgs_server_t S, pgs_dev_op_t dop, int* op_id) {
   if (!S || !dop || !op_id) return PGS_ERROR_INVALID_PARAM;
   Operator o;
   o.dop = dop;
-  int dev = 0;
-  int rc = pgs_dev_op_dims(dop, &o.n, &dev);
+  int dev = 0, cplx = 0;
+  int rc = pgs_dev_op_dims(dop, &o.n, &dev, &cplx);
   if (rc) return rc;
+  o.dpv = cplx ? 2 : 1;
   if (dev != S->device) return PGS_ERROR_INVALID_PARAM;
   std::lock_guard<std::mutex> lk(S->m);
   S->ops.push_back(o);

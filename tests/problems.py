@@ -9,16 +9,22 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def read_mtx(name):
-    """MatrixMarket fixture from the reference's test data (tests/golden)."""
+    """MatrixMarket fixture from the reference's test data (tests/golden;
+    a .gz name is read through gzip)."""
+    import gzip
     import scipy.io as sio
-    M = sio.mmread(os.path.join(GOLDEN, name))
-    return M
+    path = os.path.join(GOLDEN, name)
+    if name.endswith(".gz"):
+        with gzip.open(path) as f:
+            return sio.mmread(f)
+    return sio.mmread(path)
 
 
 def csr_from_scipy(M):
     M = M.tocsr()
     M.sort_indices()
-    return pg.CsrHost(M.shape[0], M.indptr, M.indices, M.data.astype(np.float64))
+    dt = np.complex128 if np.iscomplexobj(M.data) else np.float64
+    return pg.CsrHost(M.shape[0], M.indptr, M.indices, M.data.astype(dt))
 
 
 def dense(A):
