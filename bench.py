@@ -259,18 +259,28 @@ def probe(args):
 
 
 def serve(args):
-    """Serving stream: resident operators (2D Laplacian 1024^2, 3D Laplacian
-    64^3, upwind convection-diffusion 96^3 with c = 0.3; GeMSLR 3 levels, 64
+    """Serving stream: four resident operators (2D Laplacian 1024^2, 3D
+    Laplacian 64^3, upwind convection-diffusion 96^3 with c = 0.3, complex-
+    shifted 3D Laplacian 80^3 with shift 0.2+0.02i; GeMSLR 3 levels, 64
     subdomains, ILUT 1e-3/20, rank 16), requests with Poisson arrivals, each
-    on a uniformly chosen operator with 1-16 N(0,1) right-hand sides, solved
-    to 1e-8 by the request server (mixed form, continuous batching)."""
+    on a uniformly chosen operator with 1-16 N(0,1) right-hand sides (complex
+    N(0,1) for the complex operator), solved to 1e-8 by the request server
+    (FP32 / complex64 apply inside FP64 / complex128 FGMRES, continuous
+    batching)."""
     import threading
 
     import pargemslr_amd as pg
 
     rng = np.random.default_rng(0)
     mats = [("laplacian2d_1024^2", pg.laplacian(1024, 1024, 1)), ("laplacian3d_64^3", pg.laplacian(64, 64, 64)),
-            ("convdiff3d_96^3_c0.3", pg.convdiff(96, 96, 96, 0.3))]
+            ("convdiff3d_96^3_c0.3", pg.convdiff(96, 96, 96, 0.3)),
+            ("complex_laplacian3d_80^3_shift0.2+0.02i", pg.laplacian(80, 80, 80, shift=0.2 + 0.02j, complex_=True))]
+
+    def rhs(seed, A):
+        if A.is_complex:
+            return pg.rng_normal(seed, 2 * A.n).view(np.complex128)
+        return pg.rng_normal(seed, A.n)
+
     params = pg.config_params(3, 64, 1e-3, 20, 16, kdim=50)
     ops = []
     for name, A in mats:
@@ -283,12 +293,12 @@ def serve(args):
     ids = [S.add_operator(D) for _, _, _, D in ops]
     # warm-up: one request per operator (workspaces, graphs)
     for i, (_, A, _, _) in enumerate(ops):
-        S.wait(S.submit(ids[i], pg.rng_normal(7, A.n)))
+        S.wait(S.submit(ids[i], rhs(7, A)))
     plan = []
     for k in range(args.requests):
         op = int(rng.integers(0, len(ops)))
         nr = int(rng.integers(1, 17))
-        B = np.stack([pg.rng_normal(100000 + 64 * k + c, ops[op][1].n) for c in range(nr)], axis=1)
+        B = np.stack([rhs(100000 + 64 * k + c, ops[op][1]) for c in range(nr)], axis=1)
         plan.append((op, B, float(rng.exponential(1.0 / args.rate))))
     reqs = [None] * len(plan)
 
@@ -312,17 +322,27 @@ def serve(args):
         for c in range(B.shape[1]):
             worst = max(worst, float(np.linalg.norm(B[:, c] - A.matvec(X[:, c])) / np.linalg.norm(B[:, c])))
     ok = all(info["status"] == 0 for _, _, info in out) and worst <= 1e-8
+    per_op = {}
+    for i, (name, _, _, _) in enumerate(ops):
+        sel = [(B, r) for (op, B, _), r in zip(plan, out) if op == i]
+        if not sel:
+            continue
+        l_i = np.array([r[2]["total_ms"] for _, r in sel])
+        per_op[name] = {"requests": len(sel), "rhs": int(sum(B.shape[1] for B, _ in sel)),
+                        "iters_mean": round(float(np.mean(np.concatenate([r[1] for _, r in sel]))), 1),
+                        "p50_ms": round(float(np.percentile(l_i, 50)), 2),
+                        "p99_ms": round(float(np.percentile(l_i, 99)), 2)}
     line = {
         "metric": "serving: FGMRES solves/s and request latency at 1e-8 true relative residual (mixed, batched)",
         "value": round(cols / T, 3), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f32-apply/f64-krylov", "data": "synthetic",
+        "dtype": "f32-apply/f64-krylov (complex64/complex128 for the complex operator)", "data": "synthetic",
         "config": {"workload": "serving stream: " + ", ".join(n for n, _, _, _ in ops),
                    "requests": len(plan), "rhs_total": cols, "arrival_rate_per_s": args.rate,
                    "max_batch": 16, "slots": 3, "graphs": bool(args.graphs)},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
                        "mean_queue": round(float(np.mean([i["queue_ms"] for _, _, i in out])), 2)},
         "batch_nrhs_mean": round(float(np.mean([i["batch_nrhs"] for _, _, i in out])), 2),
-        "max_true_rel_res": worst, "converged": ok, "wall_s": round(T, 2),
+        "per_operator": per_op, "max_true_rel_res": worst, "converged": ok, "wall_s": round(T, 2),
     }
     print(json.dumps(line), flush=True)
 

@@ -186,6 +186,8 @@ struct pgs_dev_op_s {
   int trsv_cg = 4;       // most right-hand-side columns per workgroup of the ring kernel
   int trsv_nt = 128;     // threads of the pipelined kernel
   int trsv_lpr = 4;      // lanes per row (steps hold at most trsv_nt / trsv_lpr rows)
+  int ring_nt = 128;     // ring kernel: threads (64: one wave, no barrier per step)
+  int ring_lpr = 4;      // ring kernel: lanes per row (steps hold at most ring_nt / ring_lpr rows)
   int64_t bytes = 0;
   DevCsr A;  // double or complex<double> values
   DevMem perm;
@@ -348,9 +350,11 @@ void launch_trsv_pipe(int nt, int lpr, const DevTrsv& T, const V* rhs, V* x, int
   }
 }
 
-// Ring kernel geometry: 128 threads, 4 lanes per row (steps of at most 32
-// rows), 5 slots per lane (rows of at most 20 entries).
-constexpr int kRingNT = 128, kRingLPR = 4, kRingEPL = 5;
+// Ring kernel geometries: rows of at most kRingWidth entries, LPR lanes per
+// row with kRingWidth / LPR slots each; 128 threads and 4 lanes per row (steps
+// of at most 32 rows), or one wave of 64 threads with 4 lanes (16 rows) or 2
+// lanes (32 rows) per row.
+constexpr int kRingWidth = 20;
 
 // stage depth of the ring kernel's register ring: 8 for float, 6 for double
 // and complex<float>, 4 for complex<double> (registers per stage grow with
@@ -358,15 +362,15 @@ constexpr int kRingNT = 128, kRingLPR = 4, kRingEPL = 5;
 template <typename V>
 constexpr int ring_depth() { return sizeof(V) == 4 ? 8 : sizeof(V) == 8 ? 6 : 4; }
 
-template <typename V, int CG>
-void launch_trsv_ring_cg(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
+template <typename V, int NT, int LPR, int CG>
+void launch_trsv_ring_cg(const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
   constexpr int D = ring_depth<V>();
-  auto kern = k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, CG>;  // LDS limit raised in set_kernel_attrs
+  auto kern = k_trsv_ring<V, NT, LPR, kRingWidth / LPR, D, CG>;  // LDS limit raised in set_kernel_attrs
   const size_t lds = (size_t)(T.window + 1) * CG * sizeof(V);  // ring + one dummy slot
   const unsigned grid = (unsigned)T.nblocks * (unsigned)(nrhs / CG);
-  kern<<<grid, kRingNT, lds, s>>>(T.nblocks, T.window - 1, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
-                                  T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), T.desc.as<int4>(),
-                                  T.ent.as<TPair<V>>(), T.ri.as<TPair<V>>(), rhs, x, nrhs);
+  kern<<<grid, NT, lds, s>>>(T.nblocks, T.window - 1, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
+                             T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), T.desc.as<int4>(), T.ent.as<TPair<V>>(),
+                             T.ri.as<TPair<V>>(), rhs, x, nrhs);
 }
 
 template <typename V>
@@ -378,10 +382,20 @@ void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
       (T.max_block_steps + 64) * 16 >= (int64_t)INT32_MAX)
     throw std::runtime_error("diagonal block too large for 32-bit offsets");
   int cg = std::min(nrhs, op->trsv_cg);
+  if (op->ring_lpr == 2) cg = std::min(cg, 2);  // a lane pair finalizes at most two columns
   while (cg > 1 && (size_t)(T.window + 1) * cg * sizeof(V) > (size_t)op->lds_max) cg >>= 1;
-  if (cg >= 4) launch_trsv_ring_cg<V, 4>(op, T, rhs, x, nrhs, s);
-  else if (cg == 2) launch_trsv_ring_cg<V, 2>(op, T, rhs, x, nrhs, s);
-  else launch_trsv_ring_cg<V, 1>(op, T, rhs, x, nrhs, s);
+  if (op->ring_nt == 64 && op->ring_lpr == 2) {
+    if (cg >= 2) launch_trsv_ring_cg<V, 64, 2, 2>(T, rhs, x, nrhs, s);
+    else launch_trsv_ring_cg<V, 64, 2, 1>(T, rhs, x, nrhs, s);
+  } else if (op->ring_nt == 64) {
+    if (cg >= 4) launch_trsv_ring_cg<V, 64, 4, 4>(T, rhs, x, nrhs, s);
+    else if (cg == 2) launch_trsv_ring_cg<V, 64, 4, 2>(T, rhs, x, nrhs, s);
+    else launch_trsv_ring_cg<V, 64, 4, 1>(T, rhs, x, nrhs, s);
+  } else {
+    if (cg >= 4) launch_trsv_ring_cg<V, 128, 4, 4>(T, rhs, x, nrhs, s);
+    else if (cg == 2) launch_trsv_ring_cg<V, 128, 4, 2>(T, rhs, x, nrhs, s);
+    else launch_trsv_ring_cg<V, 128, 4, 1>(T, rhs, x, nrhs, s);
+  }
 }
 
 template <typename V>
@@ -418,9 +432,14 @@ void set_lds_limit(K kern, int lds_max) {
 template <typename V>
 void set_trsv_attrs(int lds_max) {
   constexpr int D = ring_depth<V>();
-  set_lds_limit(k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, 2>, lds_max);
-  set_lds_limit(k_trsv_ring<V, kRingNT, kRingLPR, kRingEPL, D, 4>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D, 2>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D, 4>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D, 2>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D, 4>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, D, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, D, 2>, lds_max);
   if constexpr (scalar_traits<V>::is_complex) return;  // no pipelined kernel for complex values
   constexpr int D2 = sizeof(V) == 4 ? 5 : 4;
   set_lds_limit(k_trsv_pipe<V, true, 64, 2, 10, D2>, lds_max);
@@ -536,7 +555,8 @@ void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask) {
   upload(op->perm, H.perm, &op->bytes);
   // ring packing (ring kernel) unless another kernel geometry was asked for;
   // the ring holds at least one column of the operator's values
-  const bool ring = op->trsv_variant == 0 && op->trsv_nt == kRingNT && op->trsv_lpr == kRingLPR;
+  const bool ring = op->trsv_variant == 0 && op->trsv_nt == 128 && op->trsv_lpr == 4;
+  const int max_rows = ring ? op->ring_nt / op->ring_lpr : op->trsv_nt / op->trsv_lpr;
   int max_window = 0;
   if (ring) {
     max_window = 1;
@@ -544,8 +564,7 @@ void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask) {
   }
   std::vector<TrsvPack<T>> packs;
   for (int l = 0; l < H.nlev; ++l)
-    packs.push_back(trsv_pack(H.levels[l], H.lev_ptr[l], H.nloc(l), op->trsv_nt / op->trsv_lpr, max_window,
-                              kRingLPR * kRingEPL));
+    packs.push_back(trsv_pack(H.levels[l], H.lev_ptr[l], H.nloc(l), max_rows, max_window, kRingWidth));
   auto build = [&](auto& P, auto tag) {
     using V = decltype(tag);
     P.present = true;
@@ -621,6 +640,12 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
     }
     if (op->trsv_lpr == 2 && op->trsv_nt == 256) op->trsv_nt = 128;
     if (op->trsv_lpr == 4 && op->trsv_nt == 64) op->trsv_nt = 128;
+    if (const char* v = std::getenv("PGS_RING_GEOM")) {
+      // threads x lanes per row of the ring kernel: 128x4, 64x4 or 64x2
+      if (!std::strcmp(v, "64x4")) { op->ring_nt = 64; op->ring_lpr = 4; }
+      else if (!std::strcmp(v, "64x2")) { op->ring_nt = 64; op->ring_lpr = 2; }
+      else if (!std::strcmp(v, "128x4")) { op->ring_nt = 128; op->ring_lpr = 4; }
+    }
     if (const char* v = std::getenv("PGS_TRSV_CG")) {
       const int cg = std::atoi(v);
       if (cg == 1 || cg == 2 || cg == 4) op->trsv_cg = cg;

@@ -401,7 +401,8 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   constexpr int VS = (int)sizeof(V);
   constexpr int RW = LPR * EPL;  // slab width of every ring-packed step
   static_assert(D % 2 == 0, "stage depth must be even");
-  static_assert(LPR == 4 && (CG == 1 || CG == 2 || CG == 4), "quad lanes, 1, 2 or 4 columns");
+  static_assert((LPR == 4 && (CG == 1 || CG == 2 || CG == 4)) || (LPR == 2 && (CG == 1 || CG == 2)),
+                "quad lanes with 1, 2 or 4 columns, or lane pairs with 1 or 2 columns");
   constexpr int NCL = 1;  // columns finalized per lane
   const int tid = threadIdx.x;
   const int q = tid % LPR;
@@ -479,9 +480,16 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
 #pragma unroll
         for (int c = 0; c < CG; ++c) acc[c] += wgt[e] * xv[e].v[c];
       }
-      // sum over the quad, scattered so that lane q ends with column q % CG
+      // sum over the row's lanes, scattered so that lane q ends with column q % CG
       V a;
-      if constexpr (CG == 4) {
+      if constexpr (LPR == 2) {
+        if constexpr (CG == 2) {
+          const bool b0 = q & 1;
+          a = (b0 ? acc[1] : acc[0]) + dpp_xor1(b0 ? acc[0] : acc[1]);
+        } else {
+          a = pair_sum(acc[0]);
+        }
+      } else if constexpr (CG == 4) {
         // xor 1: keep columns with bit 0 = q bit 0, send the other two
         const bool b0 = q & 1, b1 = q & 2;
         const V k0 = (b0 ? acc[1] : acc[0]) + dpp_xor1(b0 ? acc[0] : acc[1]);
@@ -501,7 +509,15 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
       // the buffer's range (dropped): no branch
       xs[wr ? slot * CG + cl[0] : dummy] = v;
       bstore_val(rs_dst, wr ? (row * nrhs + c0 + cl[0]) * VS : 0x7ffffff0, v);
-      __syncthreads();
+      if constexpr (NT == 64) {
+        // one wave: its LDS operations complete in issue order, so the next
+        // step's gathers see this step's writes without a wait or barrier;
+        // only the compiler must keep the order
+        __builtin_amdgcn_wave_barrier();
+        __asm__ __volatile__("" ::: "memory");
+      } else {
+        __syncthreads();
+      }
     }
   }
 #undef PGS_RING_STAGE

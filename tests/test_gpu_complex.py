@@ -71,6 +71,24 @@ def test_complex_apply_parity(nr, cplx, gpu):
             assert np.linalg.norm(Z[:, c] - refs[c]) <= tol * np.linalg.norm(refs[c]), (prec, c)
 
 
+@pytest.mark.parametrize("geom", ["64x4", "64x2"])
+def test_complex_apply_single_wave_ring(geom, cplx, gpu, monkeypatch):
+    """The single-wave ring geometries on complex values (4 right-hand sides)."""
+    A, G, _, _ = cplx
+    monkeypatch.setenv("PGS_RING_GEOM", geom)
+    D = pg.DevOp(G)
+    W = pg.Workspace(D, max_nrhs=4, kdim=2)
+    O = P.oracle_of(G)
+    V = np.stack([_crandn(90 + c, A.n) for c in range(4)], axis=1)
+    for prec, tol in ((pg.PGS_PREC_FP64, 1e-12), (pg.PGS_PREC_FP32, 1e-4)):
+        Zd = _dev(np.zeros((A.n, 4)), gpu)
+        W.apply(_dev(V, gpu), Zd, nrhs=4, prec=prec)
+        Z = _host(Zd)
+        for c in range(4):
+            ref = O.apply(V[:, c])
+            assert np.linalg.norm(Z[:, c] - ref) <= tol * np.linalg.norm(ref), (geom, prec, c)
+
+
 def test_complex_apply_host_agrees(cplx, gpu):
     """The library's host apply and the device apply agree (the same factors
     run on the CPU and on the GPU)."""
