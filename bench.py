@@ -34,7 +34,11 @@ CONFIGS = {
     "m64": dict(grid=(64, 64, 64), nlev=3, ncomp=64, droptol=1e-3, rownnz=20, rank=16, kdim=50, tol=1e-8),
 }
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_LEVEL0_16RHS = 1161.7e6  # bytes read + written per level-0 ring-solve launch, 16 RHS FP32
+# HBM bytes read + written per level-0 ring-solve launch of the 128^3 operator
+# (FP32 apply), by right-hand sides per launch, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes over `bench.py --probe --nrhs N`, each
+# scaled by what the counter reads for a 1 GiB device copy
+TRAFFIC_LEVEL0 = {16: 1161.7e6, 32: 2265.5e6}
 
 
 def log(*a):
@@ -47,7 +51,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--nrhs", type=int, default=32)
     ap.add_argument("--fp64-solves", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU time budget of the oracle baseline")
     ap.add_argument("--no-cpu", action="store_true")
@@ -57,7 +61,7 @@ def main():
     ap.add_argument("--rate", type=float, default=2.0, help="mean request arrivals per second")
     ap.add_argument("--graphs", type=int, default=1, help="serving: replay captured FGMRES steps")
     ap.add_argument("--probe", action="store_true",
-                    help="counter probe: 16-RHS FP32 applies of the 128^3 operator and a 1 GiB copy")
+                    help="counter probe: --nrhs FP32 applies of the 128^3 operator and a 1 GiB copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not built",
@@ -166,10 +170,8 @@ def main():
         f"max true rel res {max(true_res):.2e}, ok={ok}")
 
     ach = prof["total_bytes"] / (prof["total_ms"] * 1e-3) / 1e9 if prof["total_ms"] > 0 else 0.0
-    # HBM bytes per level-0 launch of this workload, measured with rocprofv3
-    # FETCH_SIZE and WRITE_SIZE passes over `bench.py --probe`, each scaled by
-    # what the counter reads for a 1 GiB device copy (summary under profiles/)
-    traffic = TRAFFIC_LEVEL0_16RHS if (args.config == "c2" and nr == 16) else None
+    # HBM bytes per level-0 launch of this workload (summary under profiles/)
+    traffic = TRAFFIC_LEVEL0.get(nr) if args.config == "c2" else None
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_trsv_ring (level-0 block ILU solve, L+U sweeps)",
@@ -233,7 +235,7 @@ def main():
 
 def probe(args):
     """Workload for hardware-counter passes (rocprofv3 --pmc): one warm-up
-    apply of the 128^3 operator (FP32, 16 right-hand sides), a 1 GiB
+    apply of the 128^3 operator (FP32, --nrhs right-hand sides), a 1 GiB
     device-to-device copy whose known size calibrates FETCH_SIZE and
     WRITE_SIZE, then three applies."""
     import pargemslr_amd as pg
@@ -242,7 +244,7 @@ def probe(args):
     A = pg.laplacian(*cfg["grid"])
     G = pg.Gemslr(A, pg.config_params(cfg["nlev"], cfg["ncomp"], cfg["droptol"], cfg["rownnz"], cfg["rank"]), seed=0)
     D = pg.DevOp(G, prec_mask=pg.PGS_PREC_FP32)
-    nr = 16
+    nr = args.nrhs
     W = pg.Workspace(D, max_nrhs=nr, kdim=2)
     V = pg.DeviceArray.from_numpy(np.stack([pg.rng_normal(c, A.n) for c in range(nr)], axis=1))
     Z = pg.DeviceArray(V.shape)
