@@ -1,34 +1,50 @@
 // C++ class surface of pargemslr_amd, header-only over the C ABI of
 // pargemslr_amd.h.  Mirrors the reference classes a sequential (np = 1)
-// driver uses, so that such a driver switches by changing its include and
-// namespace:
-//   SequentialVectorClass  ParGeMSLR/SRC/vectors/sequential_vector.hpp
-//   CsrMatrixClass         ParGeMSLR/SRC/matrices/csr_matrix.hpp:26-716 (MatVec 562-575)
-//   GemslrClass            ParGeMSLR/SRC/preconditioners/gemslr.hpp:1511-3190
-//   IluClass               ParGeMSLR/SRC/preconditioners/ilu.hpp:118-1206
-//   FlexGmresClass         ParGeMSLR/SRC/solvers/fgmres.hpp:27-688
+// driver uses, with the same template parameters and typedef names, so that
+// such a driver switches by changing its include and namespace:
+//   SequentialVectorClass<T>  ParGeMSLR/SRC/vectors/sequential_vector.hpp (typedefs :575-579)
+//   CsrMatrixClass<T>         ParGeMSLR/SRC/matrices/csr_matrix.hpp:26-716 (MatVec 562-575, typedefs 718-721)
+//   GemslrClass<M, V, T>      ParGeMSLR/SRC/preconditioners/gemslr.hpp:1511-3190 (typedefs 3193-3196)
+//   IluClass<M, V, T>         ParGeMSLR/SRC/preconditioners/ilu.hpp:118-1206 (typedefs 1208-1211)
+//   FlexGmresClass<M, V, T>   ParGeMSLR/SRC/solvers/fgmres.hpp:27-688 (typedefs 690-693)
 // with the SolverClass contract (ParGeMSLR/SRC/solvers/solver.hpp:45-534):
 // Setup(x, rhs) / Solve(x, rhs) / SetSolveLocation / SetWithParameterArray,
 // int return codes, non-owning matrix and preconditioner pointers.
 //
-// Values are double (real operators).  A vector lives on the host
-// (kMemoryHost) or on the device (kMemoryDevice); the solver's location
-// decides where the solve runs: kMemoryDevice runs the HIP path,
+// DataType is double or complexd (std::complex<double>, the layout of the
+// reference's complexd: real part, then imaginary part).  A vector lives on
+// the host (kMemoryHost) or on the device (kMemoryDevice); the solver's
+// location decides where the solve runs: kMemoryDevice runs the HIP path,
 // kMemoryHost runs the library's host apply (GemslrClass::Solve only).
 #ifndef PARGEMSLR_AMD_HPP
 #define PARGEMSLR_AMD_HPP
 
+#include <complex>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "pargemslr_amd.h"
 
 namespace pargemslr_amd {
 
+typedef std::complex<double> complexd;
+
 // memory locations (ParGeMSLR/SRC/utils/memory.hpp:21-27)
 enum MemoryLocationEnum { kMemoryHost = 0, kMemoryDevice = 1 };
 
+template <typename T>
+struct ValueTraits {
+  static_assert(std::is_same<T, double>::value || std::is_same<T, complexd>::value,
+                "values are double or complexd");
+  static constexpr int is_complex = std::is_same<T, complexd>::value ? 1 : 0;
+  // the C ABI passes (re, im) pairs of doubles
+  static double* raw(T* p) { return reinterpret_cast<double*>(p); }
+  static const double* raw(const T* p) { return reinterpret_cast<const double*>(p); }
+};
+
+template <typename T>
 class SequentialVectorClass {
  public:
   SequentialVectorClass() = default;
@@ -42,18 +58,18 @@ class SequentialVectorClass {
     location_ = location;
     own_ = true;
     if (location == kMemoryHost) {
-      host_.assign((size_t)n, 0.0);
+      host_.assign((size_t)n, T(0.0));
       data_ = host_.data();
       return 0;
     }
     void* p = nullptr;
-    int err = pgs_dev_malloc(device_, (int64_t)n * 8, &p);
+    int err = pgs_dev_malloc(device_, (int64_t)n * (int64_t)sizeof(T), &p);
     if (err) return err;
-    data_ = static_cast<double*>(p);
-    return setzero ? pgs_memset_zero(device_, data_, (int64_t)n * 8, nullptr) : 0;
+    data_ = static_cast<T*>(p);
+    return setzero ? pgs_memset_zero(device_, data_, (int64_t)n * (int64_t)sizeof(T), nullptr) : 0;
   }
   // non-owning view (SequentialVectorClass::SetupPtr, sequential_vector.cpp:337-395)
-  int SetupPtr(double* data, int n, int location) {
+  int SetupPtr(T* data, int n, int location) {
     Clear();
     data_ = data;
     n_ = n;
@@ -68,36 +84,36 @@ class SequentialVectorClass {
     n_ = 0;
     own_ = false;
   }
-  double* GetData() const { return data_; }
+  T* GetData() const { return data_; }
   int GetLengthLocal() const { return n_; }
   int GetDataLocation() const { return location_; }
-  int Fill(double v) {
+  int Fill(T v) {
     if (location_ == kMemoryHost) {
       for (int i = 0; i < n_; ++i) data_[i] = v;
       return 0;
     }
-    std::vector<double> tmp((size_t)n_, v);
-    return pgs_memcpy_htod(device_, data_, tmp.data(), (int64_t)n_ * 8, nullptr);
+    std::vector<T> tmp((size_t)n_, v);
+    return pgs_memcpy_htod(device_, data_, tmp.data(), (int64_t)n_ * (int64_t)sizeof(T), nullptr);
   }
   // copies to / from host arrays regardless of location
-  int CopyFromHost(const double* src) {
+  int CopyFromHost(const T* src) {
     if (location_ == kMemoryHost) {
-      std::memcpy(data_, src, sizeof(double) * n_);
+      std::memcpy(static_cast<void*>(data_), src, sizeof(T) * n_);
       return 0;
     }
-    return pgs_memcpy_htod(device_, data_, src, (int64_t)n_ * 8, nullptr);
+    return pgs_memcpy_htod(device_, data_, src, (int64_t)n_ * (int64_t)sizeof(T), nullptr);
   }
-  int CopyToHost(double* dst) const {
+  int CopyToHost(T* dst) const {
     if (location_ == kMemoryHost) {
-      std::memcpy(dst, data_, sizeof(double) * n_);
+      std::memcpy(static_cast<void*>(dst), data_, sizeof(T) * n_);
       return 0;
     }
-    return pgs_memcpy_dtoh(device_, dst, data_, (int64_t)n_ * 8, nullptr);
+    return pgs_memcpy_dtoh(device_, dst, data_, (int64_t)n_ * (int64_t)sizeof(T), nullptr);
   }
   // SequentialVectorClass::MoveData
   int MoveData(int location) {
     if (location == location_) return 0;
-    std::vector<double> tmp((size_t)n_);
+    std::vector<T> tmp((size_t)n_);
     int err = CopyToHost(tmp.data());
     if (err) return err;
     const int n = n_;
@@ -107,35 +123,42 @@ class SequentialVectorClass {
   }
 
  private:
-  double* data_ = nullptr;
+  T* data_ = nullptr;
   int n_ = 0;
   int location_ = kMemoryHost;
   int device_ = 0;
   bool own_ = false;
-  std::vector<double> host_;
+  std::vector<T> host_;
 };
 
+template <class MatrixType, class VectorType, typename DataType>
 class GemslrClass;
 
+template <typename T>
 class CsrMatrixClass {
  public:
   // copies a 0-based CSR (host arrays)
-  int Setup(int n, const int* ia, const int* ja, const double* a) {
+  int Setup(int n, const int* ia, const int* ja, const T* a) {
     n_ = n;
     ia_.assign(ia, ia + n + 1);
     ja_.assign(ja, ja + ia[n]);
     a_.assign(a, a + ia[n]);
     return 0;
   }
-  // CooMatrixClass::Laplacian (coo_matrix.cpp:519-640), argument order of the reference
-  int Laplacian(int nx, int ny, int nz, double alphax, double alphay, double alphaz, double shift) {
+  // CooMatrixClass::Laplacian (coo_matrix.cpp:519-640), argument order of the
+  // reference; shift and the convection coefficients are T
+  int Laplacian(int nx, int ny, int nz, T alphax, T alphay, T alphaz, T shift) {
     int n = 0;
     int64_t nnz = 0;
     int *ia = nullptr, *ja = nullptr;
     double* a = nullptr;
-    int err = pgs_gen_laplacian(nx, ny, nz, shift, 0.0, alphax, alphay, alphaz, 0, &n, &nnz, &ia, &ja, &a);
+    const complexd s(shift), ax(alphax), ay(alphay), az(alphaz);
+    // the generator takes real convection coefficients
+    if (ax.imag() != 0.0 || ay.imag() != 0.0 || az.imag() != 0.0) return PGS_ERROR_INVALID_PARAM;
+    int err = pgs_gen_laplacian(nx, ny, nz, s.real(), s.imag(), ax.real(), ay.real(), az.real(),
+                                ValueTraits<T>::is_complex, &n, &nnz, &ia, &ja, &a);
     if (err) return err;
-    err = Setup(n, ia, ja, a);
+    err = Setup(n, ia, ja, reinterpret_cast<const T*>(a));
     pgs_free(ia);
     pgs_free(ja);
     pgs_free(a);
@@ -145,25 +168,47 @@ class CsrMatrixClass {
   int64_t GetNumNonzeros() const { return (int64_t)ja_.size(); }
   const int* GetI() const { return ia_.data(); }
   const int* GetJ() const { return ja_.data(); }
-  const double* GetData() const { return a_.data(); }
+  const T* GetData() const { return a_.data(); }
   // y = alpha * A * x + beta * y, trans 'N' (CsrMatrixClass::MatVec,
   // csr_matrix.cpp:1340-1365).  Host vectors run on the host; device vectors
   // need a device operator from a GemslrClass set up on this matrix.
-  int MatVec(char trans, double alpha, const SequentialVectorClass& x, double beta, SequentialVectorClass& y) const;
+  int MatVec(char trans, T alpha, const SequentialVectorClass<T>& x, T beta, SequentialVectorClass<T>& y) const {
+    if (trans != 'N' && trans != 'n') return PGS_ERROR_INVALID_OPTION;
+    if (x.GetLengthLocal() != n_ || y.GetLengthLocal() != n_) return PGS_ERROR_INVALID_PARAM;
+    if (x.GetDataLocation() == kMemoryDevice && y.GetDataLocation() == kMemoryDevice) {
+      if (!dop_) return PGS_ERROR_MEMORY_LOCATION;
+      const complexd a(alpha), b(beta);
+      const double ab[2] = {a.real(), a.imag()}, bb[2] = {b.real(), b.imag()};
+      return pgs_spmv(dop_, ws_, ab, ValueTraits<T>::raw(x.GetData()), bb, ValueTraits<T>::raw(y.GetData()), 1,
+                      nullptr);
+    }
+    if (x.GetDataLocation() != kMemoryHost || y.GetDataLocation() != kMemoryHost) return PGS_ERROR_MEMORY_LOCATION;
+    const T* xv = x.GetData();
+    T* yv = y.GetData();
+    for (int i = 0; i < n_; ++i) {
+      T s(0.0);
+      for (int k = ia_[i]; k < ia_[i + 1]; ++k) s += a_[k] * xv[ja_[k]];
+      yv[i] = (beta == T(0.0) ? T(0.0) : beta * yv[i]) + alpha * s;
+    }
+    return 0;
+  }
 
  private:
+  template <class, class, typename>
   friend class GemslrClass;
   int n_ = 0;
   std::vector<int> ia_, ja_;
-  std::vector<double> a_;
+  std::vector<T> a_;
   mutable pgs_dev_op_t dop_ = nullptr;  // set by GemslrClass::Setup on the device
   mutable pgs_ws_t ws_ = nullptr;
 };
 
 // Preconditioner slot (SolverClass): Setup builds the operator on the host
 // and uploads it; Solve applies z = M^{-1} rhs.
+template <class MatrixType, class VectorType, typename DataType>
 class GemslrClass {
  public:
+  using VT = ValueTraits<DataType>;
   GemslrClass() { pgs_params_default(params_); }
   GemslrClass(const GemslrClass&) = delete;
   GemslrClass& operator=(const GemslrClass&) = delete;
@@ -181,7 +226,7 @@ class GemslrClass {
     host_ = nullptr;
     ready_ = false;
   }
-  int SetMatrix(CsrMatrixClass& A) {
+  int SetMatrix(MatrixType& A) {
     Clear();
     matrix_ = &A;
     return 0;
@@ -195,17 +240,18 @@ class GemslrClass {
     location_ = location;
     return 0;
   }
-  // PGS_PREC_FP64 (default) or PGS_PREC_FP32 arithmetic for the device apply
+  // PGS_PREC_FP64 (default: double / complex128) or PGS_PREC_FP32 (float /
+  // complex64) arithmetic for the device apply
   int SetApplyPrecision(int prec) {
     prec_ = prec;
     return 0;
   }
   void SetSeed(uint64_t seed) { seed_ = seed; }
-  virtual int Setup(SequentialVectorClass&, SequentialVectorClass&) {
+  virtual int Setup(VectorType&, VectorType&) {
     if (ready_) return 0;
     if (!matrix_) return PGS_ERROR_INVALID_PARAM;
-    int err = pgs_gemslr_setup(matrix_->n_, matrix_->ia_.data(), matrix_->ja_.data(), matrix_->a_.data(), 0, params_,
-                               seed_, &host_);
+    int err = pgs_gemslr_setup(matrix_->n_, matrix_->ia_.data(), matrix_->ja_.data(), VT::raw(matrix_->a_.data()),
+                               VT::is_complex, params_, seed_, &host_);
     if (err) return err;
     if (location_ == kMemoryDevice) {
       err = pgs_dev_op_create(host_, 0, PGS_PREC_FP64 | PGS_PREC_FP32, &dop_);
@@ -219,15 +265,15 @@ class GemslrClass {
     return 0;
   }
   // z = M^{-1} rhs (GemslrClass::Solve, gemslr.cpp:3431-3467)
-  virtual int Solve(SequentialVectorClass& x, SequentialVectorClass& rhs) {
+  virtual int Solve(VectorType& x, VectorType& rhs) {
     if (!ready_) return PGS_ERROR_FUNCTION_CALL;
     if (x.GetLengthLocal() != matrix_->n_ || rhs.GetLengthLocal() != matrix_->n_) return PGS_ERROR_INVALID_PARAM;
     if (x.GetDataLocation() == kMemoryDevice && rhs.GetDataLocation() == kMemoryDevice) {
       if (!dop_) return PGS_ERROR_MEMORY_LOCATION;
-      return pgs_apply(dop_, ws_, rhs.GetData(), x.GetData(), 1, prec_, nullptr);
+      return pgs_apply(dop_, ws_, VT::raw(rhs.GetData()), VT::raw(x.GetData()), 1, prec_, nullptr);
     }
     if (x.GetDataLocation() == kMemoryHost && rhs.GetDataLocation() == kMemoryHost)
-      return pgs_gemslr_host_apply(host_, rhs.GetData(), x.GetData());
+      return pgs_gemslr_host_apply(host_, VT::raw(rhs.GetData()), VT::raw(x.GetData()));
     return PGS_ERROR_MEMORY_LOCATION;
   }
   // nnz of the ILU factors and of the low-rank terms (GemslrClass::GetNumNonzeros)
@@ -244,7 +290,7 @@ class GemslrClass {
   void SetWorkspaceKrylovDimension(int kdim) { kdim_ws_ = kdim; }
 
  protected:
-  CsrMatrixClass* matrix_ = nullptr;
+  MatrixType* matrix_ = nullptr;
   double params_[PGS_IO_SIZE];
   int location_ = kMemoryDevice;
   int prec_ = PGS_PREC_FP64;
@@ -260,61 +306,48 @@ class GemslrClass {
 // a one-level GeMSLR operator, i.e. a single ILUT of P A P^T with the RCM
 // ordering P.  Droptol / max row nnz follow PGS_IO_ILU_DROPTOL_B_GLOBAL /
 // PGS_IO_ILU_ROWNNZ_B_GLOBAL.
-class IluClass : public GemslrClass {
+template <class MatrixType, class VectorType, typename DataType>
+class IluClass : public GemslrClass<MatrixType, VectorType, DataType> {
+  using Base = GemslrClass<MatrixType, VectorType, DataType>;
+
  public:
   IluClass() {
-    params_[PGS_IO_PREPOSS_NLEV_GLOBAL] = 1;
-    params_[PGS_IO_SCHUR_ENABLE] = 0;
+    this->params_[PGS_IO_PREPOSS_NLEV_GLOBAL] = 1;
+    this->params_[PGS_IO_SCHUR_ENABLE] = 0;
   }
   int SetDropTolerance(double tol) {
-    params_[PGS_IO_ILU_DROPTOL_B_GLOBAL] = tol;
-    params_[PGS_IO_ILU_DROPTOL_C_GLOBAL] = tol;
+    this->params_[PGS_IO_ILU_DROPTOL_B_GLOBAL] = tol;
+    this->params_[PGS_IO_ILU_DROPTOL_C_GLOBAL] = tol;
     return 0;
   }
   int SetMaxNnzPerRow(int lfil) {
-    params_[PGS_IO_ILU_ROWNNZ_B_GLOBAL] = lfil;
-    params_[PGS_IO_ILU_ROWNNZ_C_GLOBAL] = lfil;
+    this->params_[PGS_IO_ILU_ROWNNZ_B_GLOBAL] = lfil;
+    this->params_[PGS_IO_ILU_ROWNNZ_C_GLOBAL] = lfil;
     return 0;
   }
   int SetWithParameterArray(const double* params) override {
-    GemslrClass::SetWithParameterArray(params);
-    params_[PGS_IO_PREPOSS_NLEV_GLOBAL] = 1;
-    params_[PGS_IO_SCHUR_ENABLE] = 0;
-    params_[PGS_IO_ILU_DROPTOL_C_GLOBAL] = params_[PGS_IO_ILU_DROPTOL_B_GLOBAL];
-    params_[PGS_IO_ILU_ROWNNZ_C_GLOBAL] = params_[PGS_IO_ILU_ROWNNZ_B_GLOBAL];
+    Base::SetWithParameterArray(params);
+    this->params_[PGS_IO_PREPOSS_NLEV_GLOBAL] = 1;
+    this->params_[PGS_IO_SCHUR_ENABLE] = 0;
+    this->params_[PGS_IO_ILU_DROPTOL_C_GLOBAL] = this->params_[PGS_IO_ILU_DROPTOL_B_GLOBAL];
+    this->params_[PGS_IO_ILU_ROWNNZ_C_GLOBAL] = this->params_[PGS_IO_ILU_ROWNNZ_B_GLOBAL];
     return 0;
   }
 };
 
-inline int CsrMatrixClass::MatVec(char trans, double alpha, const SequentialVectorClass& x, double beta,
-                                  SequentialVectorClass& y) const {
-  if (trans != 'N' && trans != 'n') return PGS_ERROR_INVALID_OPTION;
-  if (x.GetLengthLocal() != n_ || y.GetLengthLocal() != n_) return PGS_ERROR_INVALID_PARAM;
-  if (x.GetDataLocation() == kMemoryDevice && y.GetDataLocation() == kMemoryDevice) {
-    if (!dop_) return PGS_ERROR_MEMORY_LOCATION;
-    return pgs_spmv(dop_, ws_, &alpha, x.GetData(), &beta, y.GetData(), 1, nullptr);
-  }
-  if (x.GetDataLocation() != kMemoryHost || y.GetDataLocation() != kMemoryHost) return PGS_ERROR_MEMORY_LOCATION;
-  const double* xv = x.GetData();
-  double* yv = y.GetData();
-  for (int i = 0; i < n_; ++i) {
-    double s = 0.0;
-    for (int k = ia_[i]; k < ia_[i + 1]; ++k) s += a_[k] * xv[ja_[k]];
-    yv[i] = (beta == 0.0 ? 0.0 : beta * yv[i]) + alpha * s;
-  }
-  return 0;
-}
-
 // Flexible GMRES(m) with right preconditioning (FlexGmresClass,
 // fgmres.hpp:230-558).  Runs on the device (x, rhs device vectors).
+template <class MatrixType, class VectorType, typename DataType>
 class FlexGmresClass {
  public:
-  int SetMatrix(CsrMatrixClass& A) {
+  using VT = ValueTraits<DataType>;
+  using PrecondType = GemslrClass<MatrixType, VectorType, DataType>;
+  int SetMatrix(MatrixType& A) {
     matrix_ = &A;
     ready_ = false;
     return 0;
   }
-  int SetPreconditioner(GemslrClass& P) {
+  int SetPreconditioner(PrecondType& P) {
     precond_ = &P;
     ready_ = false;
     return 0;
@@ -352,7 +385,7 @@ class FlexGmresClass {
     orth_ = orth;
     return 0;
   }
-  int Setup(SequentialVectorClass& x, SequentialVectorClass& rhs) {
+  int Setup(VectorType& x, VectorType& rhs) {
     if (ready_) return 0;
     if (!matrix_) return PGS_ERROR_INVALID_PARAM;
     if (precond_) {
@@ -364,7 +397,7 @@ class FlexGmresClass {
     ready_ = true;
     return 0;
   }
-  int Solve(SequentialVectorClass& x, SequentialVectorClass& rhs) {
+  int Solve(VectorType& x, VectorType& rhs) {
     if (!ready_) return PGS_ERROR_FUNCTION_CALL;
     if (location_ != kMemoryDevice || x.GetDataLocation() != kMemoryDevice ||
         rhs.GetDataLocation() != kMemoryDevice)
@@ -373,6 +406,7 @@ class FlexGmresClass {
     if (!dop) return PGS_ERROR_FUNCTION_CALL;
     if (!ws_ || ws_kdim_ < kdim_) {
       if (ws_) pgs_ws_destroy(ws_);
+      ws_ = nullptr;
       int err = pgs_ws_create(dop, 1, kdim_, &ws_);
       if (err) return err;
       ws_kdim_ = kdim_;
@@ -388,7 +422,7 @@ class FlexGmresClass {
     o.check_every = 1;
     o.use_graph = 0;
     pgs_fgmres_result_t r;
-    int err = pgs_fgmres(dop, ws_, rhs.GetData(), x.GetData(), 1, &o, &r, nullptr);
+    int err = pgs_fgmres(dop, ws_, VT::raw(rhs.GetData()), VT::raw(x.GetData()), 1, &o, &r, nullptr);
     if (err) return err;
     iters_ = r.iters;
     rel_res_ = r.rel_res;
@@ -403,8 +437,8 @@ class FlexGmresClass {
   }
 
  private:
-  CsrMatrixClass* matrix_ = nullptr;
-  GemslrClass* precond_ = nullptr;
+  MatrixType* matrix_ = nullptr;
+  PrecondType* precond_ = nullptr;
   int kdim_ = 50, maxits_ = 100;
   double tol_ = 1e-6;
   bool absolute_tol_ = false;
@@ -416,6 +450,19 @@ class FlexGmresClass {
   int iters_ = 0, status_ = 0;
   double rel_res_ = 0.0;
 };
+
+// the reference's typedef names (double and complexd instantiations)
+typedef SequentialVectorClass<double> vector_seq_double;
+typedef SequentialVectorClass<complexd> vector_seq_complexd;
+typedef CsrMatrixClass<double> matrix_csr_double;
+typedef CsrMatrixClass<complexd> matrix_csr_complexd;
+typedef GemslrClass<CsrMatrixClass<double>, SequentialVectorClass<double>, double> precond_gemslr_csr_seq_double;
+typedef GemslrClass<CsrMatrixClass<complexd>, SequentialVectorClass<complexd>, complexd>
+    precond_gemslr_csr_seq_complexd;
+typedef IluClass<CsrMatrixClass<double>, SequentialVectorClass<double>, double> precond_ilu_csr_seq_double;
+typedef IluClass<CsrMatrixClass<complexd>, SequentialVectorClass<complexd>, complexd> precond_ilu_csr_seq_complexd;
+typedef FlexGmresClass<CsrMatrixClass<double>, SequentialVectorClass<double>, double> fgmres_csr_seq_double;
+typedef FlexGmresClass<CsrMatrixClass<complexd>, SequentialVectorClass<complexd>, complexd> fgmres_csr_seq_complexd;
 
 }  // namespace pargemslr_amd
 

@@ -23,7 +23,10 @@ def test_driver_is_built_and_links():
 @pytest.mark.parametrize("args", [
     ["64", "64", "1", "2", "4", "1e-2", "10", "0", "30", "1e-8", "1"],
     ["32", "32", "32", "3", "8", "1e-3", "20", "16", "50", "1e-8", "2"],
-])
+    # complexd classes, complex-shifted Laplacian, complex64 apply (the reference's gemslrz driver)
+    ["24", "24", "24", "3", "8", "1e-3", "20", "16", "50", "1e-8", "2", "0.2", "0.02", "1"],
+    ["24", "24", "24", "3", "8", "1e-3", "20", "16", "50", "1e-8", "1", "0.2", "0.02", "1"],
+], ids=["real-c1", "real-mixed", "complex-mixed", "complex-fp64"])
 def test_driver_solves(args, gpu):
     out = subprocess.run([_driver()] + args, capture_output=True, text=True, timeout=120)
     print(out.stdout)
