@@ -77,6 +77,8 @@ extern "C" {
 #define PGS_IO_SCHUR_ENABLE 130
 #define PGS_IO_SCHUR_ITER_TOL 131
 #define PGS_IO_SCHUR_MAXITS 132
+/* most inner Schur iterations per apply (PGS_IO_SCHUR_MAXITS) this library takes */
+#define PGS_SCHUR_MAXITS_LIMIT 32
 
 /* precision of the preconditioner apply */
 #define PGS_PREC_FP64 1 /* apply in the operator's own precision (double / complex128) */
@@ -119,7 +121,9 @@ int pgs_gemslr_info(pgs_gemslr_t op, pgs_gemslr_info_t* info);
 /* One level of the factored operator, level-local numbering.  The factors of
  * all ncomp diagonal blocks are concatenated into one block-diagonal CSR
  * (L strict lower, U strict upper, Dinv = inverse pivots).  E is nrem x nloc,
- * F is nloc x nrem, W and WH are nrem x rank column-major.  Pointers stay
+ * F is nloc x nrem, W and WH are nrem x rank column-major.  C (nrem x nrem,
+ * the lower-right block of P A P^T) is kept on level 0 when the inner Schur
+ * iteration is on (PGS_IO_SCHUR_ENABLE), else C_ia is NULL.  Pointers stay
  * valid until pgs_gemslr_destroy. */
 typedef struct {
   int lev_start, nloc, nrem, ncomp, rank;
@@ -129,13 +133,16 @@ typedef struct {
   const int *E_ia, *E_ja, *F_ia, *F_ja;
   const double *E_a, *F_a;
   const double *W, *WH;
+  const int *C_ia, *C_ja;
+  const double* C_a;
 } pgs_level_view_t;
 int pgs_gemslr_level_view(pgs_gemslr_t op, int level, pgs_level_view_t* view);
 /* perm[i]: (P b)[i] = b[perm[i]] */
 int pgs_gemslr_perm(pgs_gemslr_t op, const int** perm);
 
 /* Host-resident apply z = M^{-1} v in FP64 (GemslrClass::Solve on kMemoryHost,
- * gemslr.cpp:3431-3467). */
+ * gemslr.cpp:3431-3467), with the inner Schur iteration on level 0 when it is
+ * on (gemslr.cpp:3556-3563). */
 int pgs_gemslr_host_apply(pgs_gemslr_t op, const double* v, double* z);
 
 /* ---------------- device ---------------- */

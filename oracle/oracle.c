@@ -27,6 +27,11 @@
  *   orc_fgmres         unpreconditioned residual history vs a numpy GMRES
  *                      (unique in exact arithmetic); exact preconditioner
  *                      converges in one iteration; Givens LSQ vs lstsq
+ *   inner Schur        (orc_gemslr_apply with inner_maxits > 0) exact ILU +
+ *   iteration          enough inner steps gives A^{-1} even without a low-rank
+ *                      term; one and K inner steps equal a dense numpy GMRES
+ *                      on S = C - E B^{-1} F with the preconditioner of the
+ *                      levels below (tests/test_oracle_schur.py)
  */
 #include <complex.h>
 #include <math.h>
@@ -43,12 +48,16 @@ typedef struct {
   const int *E_ia, *E_ja, *F_ia, *F_ja;
   const void *E_a, *F_a;
   const void *W, *WH; /* nrem x rank, column-major */
+  const int *C_ia, *C_ja; /* level 0 with the inner iteration: nrem x nrem lower-right block */
+  const void* C_a;
 } orc_level;
 
 typedef struct {
   int n, nlev;
   const int* perm;
   const orc_level* levels;
+  int inner_maxits;  /* > 0: inner FGMRES on the level-0 Schur complement */
+  double inner_tol;
 } orc_gemslr;
 
 /* workspace size (values) of orc_gemslr_apply / orc_gemslr_apply_z */

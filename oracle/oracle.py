@@ -21,11 +21,13 @@ class OrcLevel(C.Structure):
                 ("L_ia", C.c_void_p), ("L_ja", C.c_void_p), ("U_ia", C.c_void_p), ("U_ja", C.c_void_p),
                 ("L_a", C.c_void_p), ("U_a", C.c_void_p), ("Dinv", C.c_void_p),
                 ("E_ia", C.c_void_p), ("E_ja", C.c_void_p), ("F_ia", C.c_void_p), ("F_ja", C.c_void_p),
-                ("E_a", C.c_void_p), ("F_a", C.c_void_p), ("W", C.c_void_p), ("WH", C.c_void_p)]
+                ("E_a", C.c_void_p), ("F_a", C.c_void_p), ("W", C.c_void_p), ("WH", C.c_void_p),
+                ("C_ia", C.c_void_p), ("C_ja", C.c_void_p), ("C_a", C.c_void_p)]
 
 
 class OrcGemslr(C.Structure):
-    _fields_ = [("n", C.c_int), ("nlev", C.c_int), ("perm", C.c_void_p), ("levels", C.POINTER(OrcLevel))]
+    _fields_ = [("n", C.c_int), ("nlev", C.c_int), ("perm", C.c_void_p), ("levels", C.POINTER(OrcLevel)),
+                ("inner_maxits", C.c_int), ("inner_tol", C.c_double)]
 
 
 _lib = None
@@ -116,9 +118,11 @@ def lowrank_apply(W, WH, z):
 
 class GemslrOracle:
     """Oracle view of a factored operator given as the level dicts of
-    pargemslr_amd.Gemslr.levels() plus the permutation."""
+    pargemslr_amd.Gemslr.levels() plus the permutation.  schur = (maxits,
+    tol) turns on the inner FGMRES on the level-0 Schur complement (level 0
+    must carry its block "C")."""
 
-    def __init__(self, n, perm, levels):
+    def __init__(self, n, perm, levels, schur=None):
         self.n = n
         self._keep = []
         self.is_complex = cz = _cplx(*[d["Dinv"] for d in levels])
@@ -128,7 +132,7 @@ class GemslrOracle:
         for i, d in enumerate(levels):
             o = lv[i]
             o.lev_start, o.nloc, o.nrem, o.rank = d["lev_start"], d["nloc"], d["nrem"], d["rank"]
-            for nm in ("L", "U", "E", "F"):
+            for nm in ("L", "U", "E", "F", "C"):
                 if d.get(nm) is None:
                     continue
                 ia, ja, a = d[nm]
@@ -140,7 +144,12 @@ class GemslrOracle:
                 o.W = C.c_void_p(self._k(np.asfortranarray(d["W"], dtype=self.dtype)).ctypes.data)
                 o.WH = C.c_void_p(self._k(np.asfortranarray(d["WH"], dtype=self.dtype)).ctypes.data)
         self._levels = lv
-        self.s = OrcGemslr(n, len(levels), _p(self.perm), lv)
+        inner = (0, 0.0)
+        if schur is not None and len(levels) > 1:
+            if levels[0].get("C") is None:
+                raise ValueError("the inner iteration needs level 0's block C")
+            inner = (int(schur[0]), float(schur[1]))
+        self.s = OrcGemslr(n, len(levels), _p(self.perm), lv, inner[0], inner[1])
         self.work = np.zeros(lib().orc_gemslr_work_size(C.byref(self.s)), dtype=self.dtype)
 
     def _k(self, a):

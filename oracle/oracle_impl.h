@@ -56,10 +56,141 @@ void ORC_FN(orc_lowrank_apply)(int m, int rank, const ORC_T* W, const ORC_T* WH,
     for (i = 0; i < m; ++i) x[i] += WH[(size_t)j * m + i] * t[j];
 }
 
-/* GemslrClass::SolveLevel (SRC/preconditioners/gemslr.cpp:3474-3625) without
- * the inner Schur iteration.  rhs, x: length n - lev_start.  work: per level
- * buffers carved from `work` (z of length n - lev_start, xlr of length nrem,
- * t of length rank). */
+static void ORC_FN(solve_level)(const orc_gemslr* op, int l, const ORC_T* rhs, ORC_T* x, ORC_T* work);
+void ORC_FN(orc_mgs)(int n, ORC_T* V, int k, ORC_T* H, int ldh, double* t_out);
+
+/* x^H y (SequentialVectorClass::Dot -> zdotc, SRC/vectors/vectorops.cpp:100-120) */
+static ORC_T ORC_FN(dot)(int n, const ORC_T* x, const ORC_T* y) {
+  ORC_T s = 0.0;
+  int i;
+  for (i = 0; i < n; ++i) s += ORC_CONJ(x[i]) * y[i];
+  return s;
+}
+
+/* Norm2: sqrt(|x^H x|) */
+static double ORC_FN(norm2)(int n, const ORC_T* x) { return sqrt(ORC_ABS(ORC_FN(dot)(n, x, x))); }
+
+/* y = S x on level 0, S = C - E B^{-1} F with B^{-1} the block ILU solve
+ * (GemslrClass::SchurMatVec, SRC/preconditioners/gemslr.cpp:3799-3955 with
+ * alpha = 1, beta = 0: y = -E B^{-1} F x, then y += C x).  u, b: nloc values. */
+static void ORC_FN(schur_matvec)(const orc_gemslr* op, const ORC_T* x, ORC_T* y, ORC_T* u, ORC_T* b) {
+  const orc_level* L = &op->levels[0];
+  const int nl = L->nloc, m = L->nrem;
+  int i, k;
+  ORC_FN(orc_csr_matvec)(nl, L->F_ia, L->F_ja, (const ORC_T*)L->F_a, 1.0, x, 0.0, u);
+  ORC_FN(orc_ilu_solve)(nl, L->L_ia, L->L_ja, (const ORC_T*)L->L_a, L->U_ia, L->U_ja, (const ORC_T*)L->U_a,
+                        (const ORC_T*)L->Dinv, u, b);
+  for (i = 0; i < m; ++i) {
+    ORC_T e = 0.0, c = 0.0;
+    for (k = L->E_ia[i]; k < L->E_ia[i + 1]; ++k) e += ((const ORC_T*)L->E_a)[k] * b[L->E_ja[k]];
+    for (k = L->C_ia[i]; k < L->C_ia[i + 1]; ++k) c += ((const ORC_T*)L->C_a)[k] * x[L->C_ja[k]];
+    y[i] = -e + c;
+  }
+}
+
+/* z = SolveLevel(1, v + W H W^H v): the preconditioner of the inner
+ * iteration (GemslrSchurMatrixClass::Solve, gemslr.cpp:292-320).
+ * t: m values, xlr: m values, c: rank values, work: the level-1 buffers. */
+static void ORC_FN(schur_precond)(const orc_gemslr* op, const ORC_T* v, ORC_T* z, ORC_T* t, ORC_T* xlr, ORC_T* c,
+                                  ORC_T* work) {
+  const orc_level* L = &op->levels[0];
+  const int m = L->nrem;
+  int i;
+  if (L->rank > 0) {
+    ORC_FN(orc_lowrank_apply)(m, L->rank, (const ORC_T*)L->W, (const ORC_T*)L->WH, v, xlr, c);
+    for (i = 0; i < m; ++i) t[i] = xlr[i] + v[i];
+  } else {
+    for (i = 0; i < m; ++i) t[i] = v[i];
+  }
+  ORC_FN(solve_level)(op, 1, t, z, work);
+}
+
+/* Inner FGMRES on the level-0 Schur complement with x0 = 0, the solve phase
+ * of SolveLevel with inner iterations (gemslr.cpp:3556-3563): FlexGmresClass::
+ * Solve (SRC/solvers/fgmres.hpp:263-558) with Krylov dimension = iteration
+ * limit = inner_maxits, relative tolerance inner_tol, PargemslrMgs.  The
+ * restart residual the reference computes after the last cycle does not
+ * change x and is skipped.  b, x: nrem values; work: the level-1 buffers. */
+static void ORC_FN(schur_inner)(const orc_gemslr* op, const ORC_T* b, ORC_T* x, ORC_T* work) {
+  const orc_level* L = &op->levels[0];
+  const int m = L->nrem, nl = L->nloc, K = op->inner_maxits;
+  ORC_T *V, *Z, *H, *c, *s, *rs, *t, *xlr, *lc, *u, *bb;
+  ORC_T hii, hii1;
+  double normb, normr, tolr, tn, gam;
+  int i = 0, j, k, r;
+  for (r = 0; r < m; ++r) x[r] = 0.0;
+  normb = ORC_FN(norm2)(m, b);
+  if (normb < ORC_EPS) return;
+  V = (ORC_T*)calloc((size_t)m * (K + 1), sizeof(ORC_T));
+  Z = (ORC_T*)calloc((size_t)m * K, sizeof(ORC_T));
+  H = (ORC_T*)calloc((size_t)(K + 1) * K, sizeof(ORC_T));
+  c = (ORC_T*)calloc(K, sizeof(ORC_T));
+  s = (ORC_T*)calloc(K, sizeof(ORC_T));
+  rs = (ORC_T*)calloc(K + 1, sizeof(ORC_T));
+  t = (ORC_T*)calloc(m, sizeof(ORC_T));
+  xlr = (ORC_T*)calloc(m, sizeof(ORC_T));
+  lc = (ORC_T*)calloc(L->rank > 0 ? L->rank : 1, sizeof(ORC_T));
+  u = (ORC_T*)calloc(nl > 0 ? nl : 1, sizeof(ORC_T));
+  bb = (ORC_T*)calloc(nl > 0 ? nl : 1, sizeof(ORC_T));
+#define Hi(ii, jj) H[(ii) + (size_t)(jj) * (K + 1)]
+  /* x0 = 0: the residual is b, normr = normb */
+  normr = normb;
+  tolr = op->inner_tol * normb;
+  rs[0] = normr;
+  for (r = 0; r < m; ++r) V[r] = b[r] * (1.0 / normr);
+  while (i < K) {
+    ORC_T* w;
+    i++;
+    w = V + (size_t)i * m;
+    ORC_FN(schur_precond)(op, V + (size_t)(i - 1) * m, Z + (size_t)(i - 1) * m, t, xlr, lc, work);
+    ORC_FN(schur_matvec)(op, Z + (size_t)(i - 1) * m, w, u, bb);
+    ORC_FN(orc_mgs)(m, V, i - 1, H, K + 1, &tn);
+    for (j = 1; j < i; j++) {
+      hii = Hi(j - 1, i - 1);
+      Hi(j - 1, i - 1) = ORC_CONJ(c[j - 1]) * hii + s[j - 1] * Hi(j, i - 1);
+      Hi(j, i - 1) = -s[j - 1] * hii + c[j - 1] * Hi(j, i - 1);
+    }
+    hii = Hi(i - 1, i - 1);
+    hii1 = Hi(i, i - 1);
+    gam = sqrt(ORC_ABS(hii) * ORC_ABS(hii) + ORC_ABS(hii1) * ORC_ABS(hii1));
+    if (fabs(gam) < ORC_EPS) goto out; /* breakdown: x stays at the initial guess */
+    c[i - 1] = hii / gam;
+    s[i - 1] = hii1 / gam;
+    rs[i] = -s[i - 1] * rs[i - 1];
+    rs[i - 1] = ORC_CONJ(c[i - 1]) * rs[i - 1];
+    Hi(i - 1, i - 1) = ORC_CONJ(c[i - 1]) * hii + s[i - 1] * hii1;
+    normr = ORC_ABS(rs[i]);
+    if (normr <= tolr) break;
+  }
+  rs[i - 1] /= Hi(i - 1, i - 1);
+  for (k = i - 2; k >= 0; k--) {
+    for (j = k + 1; j < i; j++) rs[k] -= Hi(k, j) * rs[j];
+    rs[k] /= Hi(k, k);
+  }
+  for (j = 0; j < i; j++) {
+    const ORC_T* z = Z + (size_t)j * m;
+    for (r = 0; r < m; ++r) x[r] += rs[j] * z[r];
+  }
+out:
+#undef Hi
+  free(V);
+  free(Z);
+  free(H);
+  free(c);
+  free(s);
+  free(rs);
+  free(t);
+  free(xlr);
+  free(lc);
+  free(u);
+  free(bb);
+}
+
+/* GemslrClass::SolveLevel (SRC/preconditioners/gemslr.cpp:3474-3625); on
+ * level 0 with inner_maxits > 0 the low-rank correction and the next level
+ * are replaced by the inner FGMRES (3556-3563).  rhs, x: length n -
+ * lev_start.  work: per level buffers carved from `work` (z of length n -
+ * lev_start, xlr of length nrem, t of length rank). */
 static void ORC_FN(solve_level)(const orc_gemslr* op, int l, const ORC_T* rhs, ORC_T* x, ORC_T* work) {
   const orc_level* L = &op->levels[l];
   const int nl = L->nloc, m = L->nrem;
@@ -81,12 +212,16 @@ static void ORC_FN(solve_level)(const orc_gemslr* op, int l, const ORC_T* rhs, O
   /* steps 2-3: z_l = -E z_u; z_l += rhs_l */
   ORC_FN(orc_csr_matvec)(m, L->E_ia, L->E_ja, (const ORC_T*)L->E_a, -1.0, z, 0.0, zl);
   for (i = 0; i < m; ++i) zl[i] += rhs[nl + i];
-  /* step 4: low-rank correction, then the next level */
-  if (L->rank > 0) {
-    ORC_FN(orc_lowrank_apply)(m, L->rank, (const ORC_T*)L->W, (const ORC_T*)L->WH, zl, xlr, t);
-    for (i = 0; i < m; ++i) zl[i] += xlr[i];
+  /* step 4: low-rank correction, then the next level (or the inner iteration) */
+  if (l == 0 && op->inner_maxits > 0) {
+    ORC_FN(schur_inner)(op, zl, x + nl, t + (L->rank > 0 ? L->rank : 0));
+  } else {
+    if (L->rank > 0) {
+      ORC_FN(orc_lowrank_apply)(m, L->rank, (const ORC_T*)L->W, (const ORC_T*)L->WH, zl, xlr, t);
+      for (i = 0; i < m; ++i) zl[i] += xlr[i];
+    }
+    ORC_FN(solve_level)(op, l + 1, zl, x + nl, t + (L->rank > 0 ? L->rank : 0));
   }
-  ORC_FN(solve_level)(op, l + 1, zl, x + nl, t + (L->rank > 0 ? L->rank : 0));
   /* steps 5-6: z_u = -F x_l; z_u += rhs_u */
   ORC_FN(orc_csr_matvec)(nl, L->F_ia, L->F_ja, (const ORC_T*)L->F_a, -1.0, x + nl, 0.0, z);
   for (i = 0; i < nl; ++i) z[i] += rhs[i];
@@ -107,17 +242,6 @@ void ORC_FN(orc_gemslr_apply)(const orc_gemslr* op, const ORC_T* v, ORC_T* z, OR
   ORC_FN(solve_level)(op, 0, r, x, work + 2 * n);
   for (i = 0; i < n; ++i) z[op->perm[i]] = x[i];
 }
-
-/* x^H y (SequentialVectorClass::Dot -> zdotc, SRC/vectors/vectorops.cpp:100-120) */
-static ORC_T ORC_FN(dot)(int n, const ORC_T* x, const ORC_T* y) {
-  ORC_T s = 0.0;
-  int i;
-  for (i = 0; i < n; ++i) s += ORC_CONJ(x[i]) * y[i];
-  return s;
-}
-
-/* Norm2: sqrt(|x^H x|) */
-static double ORC_FN(norm2)(int n, const ORC_T* x) { return sqrt(ORC_ABS(ORC_FN(dot)(n, x, x))); }
 
 /* PargemslrMgs without re-orthogonalization (tol_reorth < 0, as FGMRES calls
  * it: SRC/solvers/fgmres.hpp:426; SRC/matrices/matrixops.cpp:10728-10807).

@@ -52,7 +52,8 @@ class LevelView(C.Structure):
                 ("E_ia", C.POINTER(C.c_int)), ("E_ja", C.POINTER(C.c_int)),
                 ("F_ia", C.POINTER(C.c_int)), ("F_ja", C.POINTER(C.c_int)),
                 ("E_a", C.POINTER(C.c_double)), ("F_a", C.POINTER(C.c_double)),
-                ("W", C.POINTER(C.c_double)), ("WH", C.POINTER(C.c_double))]
+                ("W", C.POINTER(C.c_double)), ("WH", C.POINTER(C.c_double)),
+                ("C_ia", C.POINTER(C.c_int)), ("C_ja", C.POINTER(C.c_int)), ("C_a", C.POINTER(C.c_double))]
 
 
 class FgmresOpts(C.Structure):
@@ -268,10 +269,17 @@ class Gemslr:
         self.A = A
         self._h = C.c_void_p()
         params = np.ascontiguousarray(params, dtype=np.float64)
+        self.params = params.copy()
         a = A.a.view(np.float64) if A.is_complex else A.a
         a = np.ascontiguousarray(a)
         _chk(lib().pgs_gemslr_setup(A.n, _iptr(A.ia), _iptr(A.ja), _dptr(a), int(A.is_complex), _dptr(params),
                                     C.c_uint64(seed), C.byref(self._h)), "pgs_gemslr_setup")
+
+    def schur(self):
+        """(maxits, tol) of the inner Schur iteration, or None when it is off."""
+        if self.params[IO["SCHUR_ENABLE"]] == 0 or self.info().nlev < 2:
+            return None
+        return int(self.params[IO["SCHUR_MAXITS"]]), float(self.params[IO["SCHUR_ITER_TOL"]])
 
     def close(self):
         if self._h:
@@ -330,6 +338,13 @@ class Gemslr:
                 a_np = vals(getattr(v, nm + "_a"), nnz) if nnz else np.zeros(0, np.complex128 if cplx else np.float64)
                 d[nm] = (ia_np, ja_np, a_np)
             d["Dinv"] = vals(v.Dinv, v.nloc)
+            if bool(v.C_ia):
+                cia = np.ctypeslib.as_array(v.C_ia, shape=(v.nrem + 1,)).copy()
+                cnnz = int(cia[-1])
+                cja = np.ctypeslib.as_array(v.C_ja, shape=(max(cnnz, 1),)).copy()[:cnnz]
+                d["C"] = (cia, cja, vals(v.C_a, cnnz))
+            else:
+                d["C"] = None
             if v.rank > 0:
                 d["W"] = vals(v.W, v.rank * v.nrem).reshape(v.rank, v.nrem).T.copy()
                 d["WH"] = vals(v.WH, v.rank * v.nrem).reshape(v.rank, v.nrem).T.copy()
@@ -576,8 +591,10 @@ class Server:
 
 
 def config_params(nlev, ncomp, droptol, rownnz, rank, kdim=50, maxits=1000, tol=1e-8, lr_tol_eig=1e-3,
-                  lr_maxits=10, droptol_c=None, rownnz_c=None, perm=1):
-    """Reference parameter array for a GeMSLR configuration (inner Schur iteration off)."""
+                  lr_maxits=10, droptol_c=None, rownnz_c=None, perm=1, schur=None):
+    """Reference parameter array for a GeMSLR configuration.  schur: None
+    (inner Schur iteration off) or (maxits, tol) of the inner FGMRES on the
+    level-0 Schur complement (the reference's defaults are (5, 1e-2))."""
     p = default_params()
     p[IO["NLEV"]] = nlev
     p[IO["NCOMP"]] = ncomp
@@ -593,7 +610,10 @@ def config_params(nlev, ncomp, droptol, rownnz, rank, kdim=50, maxits=1000, tol=
     p[IO["LR_TOL_EIG2"]] = lr_tol_eig
     p[IO["LR_MAXITS1"]] = lr_maxits
     p[IO["LR_MAXITS2"]] = lr_maxits
-    p[IO["SCHUR_ENABLE"]] = 0
+    p[IO["SCHUR_ENABLE"]] = 0 if schur is None else 1
+    if schur is not None:
+        p[IO["SCHUR_MAXITS"]] = schur[0]
+        p[IO["SCHUR_ITER_TOL"]] = schur[1]
     p[IO["SOLVER_KDIM"]] = kdim
     p[IO["SOLVER_MAXITS"]] = maxits
     p[IO["SOLVER_TOL"]] = tol

@@ -165,6 +165,7 @@ struct DevLevel {
   int lev_start = 0, nloc = 0, nrem = 0, rank = 0;
   DevTrsv B;
   DevCsr E, F;
+  DevCsr C;      // level 0 with the inner Schur iteration: the nrem x nrem lower-right block
   DevMem W, WH;  // row-major nrem x rank
 };
 
@@ -188,6 +189,8 @@ struct pgs_dev_op_s {
   int trsv_lpr = 4;      // lanes per row (steps hold at most trsv_nt / trsv_lpr rows)
   int ring_nt = 128;     // ring kernel: threads (64: one wave, no barrier per step)
   int ring_lpr = 4;      // ring kernel: lanes per row (steps hold at most ring_nt / ring_lpr rows)
+  int inner_maxits = 0;     // > 0: inner FGMRES on the level-0 Schur complement in every apply
+  double inner_tol = 0.0;
   int64_t bytes = 0;
   DevCsr A;  // double or complex<double> values
   DevMem perm;
@@ -206,6 +209,10 @@ struct pgs_ws_s {
   std::vector<std::unique_ptr<DevMem>> zlev;
   // Krylov
   DevMem V, Z, w, part, hbuf, h2buf, t2, H, cs, sn, rs, coef, upd, st, zflag, nb2, nr2, flag;
+  // inner Schur iteration: basis, preconditioned basis, temporaries (vectors
+  // in the apply's precision) and its Hessenberg / Givens state per column
+  DevMem in_V, in_Z, in_T, in_U, in_Y, in_h, in_t2, in_H, in_cs, in_sn, in_rs, in_coef, in_upd, in_st, in_flag,
+      in_nb2;
   int* flag_host = nullptr;
   int part_blocks = 0;
   bool fused_cgs2 = true;  // CGS2 as three k_basis_reg sweeps instead of four
@@ -469,8 +476,8 @@ void set_kernel_attrs(int lds_max) {
     set_lds_limit(k_basis_reg<kBasisNT, 32, 0, 2>, lds_max);
     set_lds_limit(k_basis_reg<kBasisNT, 32, 1, 2>, lds_max);
     // coefficient tables of up to 127 x 64 values in LDS
-    set_lds_limit(k_multi_axpy_norm<double, kNT>, lds_max);
-    set_lds_limit(k_multi_axpy_norm<cplx<double>, kNT>, lds_max);
+    set_lds_limit(k_multi_axpy_norm<double, double, kNT>, lds_max);
+    set_lds_limit(k_multi_axpy_norm<cplx<double>, cplx<double>, kNT>, lds_max);
     set_lds_limit(k_update_x<double, double>, lds_max);
     set_lds_limit(k_update_x<float, double>, lds_max);
     set_lds_limit(k_update_x<cplx<double>, cplx<double>>, lds_max);
@@ -502,6 +509,132 @@ void launch_lowrank(pgs_ws_s* ws, const DevLevel& L, V* zl, int nrhs, hipStream_
   HIPCHK(hipGetLastError());
 }
 
+// out[g] (+)= sum of nblk partials of p*nrhs values of S (a complex value is
+// summed as its two doubles)
+template <typename S>
+void dot_final(int nblk, int p, int nrhs, const S* part, S* out, int accumulate, hipStream_t s) {
+  const int pp = p * (int)(sizeof(S) / sizeof(double));
+  k_dot_final_wave<<<(pp * nrhs + 3) / 4, 256, 0, s>>>(nblk, pp, nrhs, reinterpret_cast<const double*>(part),
+                                                       reinterpret_cast<double*>(out), accumulate);
+}
+
+// multi-dot over the first p vectors of V against w: out[j*nrhs+c] = V_j^H w
+template <typename VV, typename VW, typename S>
+void multidot(pgs_ws_s* ws, int64_t n, int nrhs, int p, const VV* V, int64_t ldv, const VW* w, S* out,
+              int accumulate, hipStream_t s) {
+  const int64_t rows_per_blk = (n + ws->part_blocks - 1) / ws->part_blocks;
+  const int nblk = (int)((n + rows_per_blk - 1) / rows_per_blk);
+  k_multidot<VV, VW, S, kNT><<<nblk, kNT, 0, s>>>(n, nrhs, p, V, ldv, w, ws->part.as<S>(), rows_per_blk);
+  dot_final<S>(nblk, p, nrhs, ws->part.as<S>(), out, accumulate, s);
+  HIPCHK(hipGetLastError());
+}
+
+// out[c] = ||x(:, c)||^2
+template <typename S>
+void colnorm2(pgs_ws_s* ws, int64_t n, int nrhs, const S* x, double* out, hipStream_t s) {
+  const int64_t rows_per_blk = (n + ws->part_blocks - 1) / ws->part_blocks;
+  const int nblk = (int)((n + rows_per_blk - 1) / rows_per_blk);
+  k_colnorm2<S, kNT><<<nblk, kNT, 0, s>>>(n, nrhs, x, ws->part.as<double>(), rows_per_blk);
+  k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, ws->part.as<double>(), out, 0);
+  HIPCHK(hipGetLastError());
+}
+
+// w -= V(:, 0:p) h; with t2, t2[c] = ||w(:, c)||^2 of the result.  V, w in
+// VV, h and the arithmetic in S.
+template <typename VV, typename S = VV>
+void multi_axpy(pgs_ws_s* ws, int64_t n, int nrhs, int p, const VV* V, int64_t ldv, const S* h, VV* w, double* t2,
+                hipStream_t s) {
+  // enough workgroups to keep every CU busy on the streaming update
+  const int nblk = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n * nrhs + kNT - 1) / kNT));
+  const size_t shm = sizeof(S) * (size_t)p * nrhs + sizeof(double) * (kNT / 64) * nrhs;
+  k_multi_axpy_norm<VV, S, kNT><<<nblk, kNT, shm, s>>>(n, nrhs, p, V, ldv, h, w,
+                                                       t2 ? ws->part.as<double>() : nullptr);
+  if (t2) k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, ws->part.as<double>(), t2, 0);
+  HIPCHK(hipGetLastError());
+}
+
+template <typename V>
+void solve_level(pgs_dev_op_s* op, pgs_ws_s* ws, DevPrec<V>& P, int l, const V* rhs, V* x, int nrhs, hipStream_t s);
+
+// scalar of the inner iteration's dots, Hessenberg and rotations: double for
+// real vectors (float or double), complex<double> for complex vectors
+template <typename V>
+struct InnerScalar {
+  using type = double;
+};
+template <typename R>
+struct InnerScalar<cplx<R>> {
+  using type = cplx<double>;
+};
+
+// x = inner FGMRES on the level-0 Schur complement, x0 = 0 (the solve phase of
+// SolveLevel with inner iterations, gemslr.cpp:3556-3563; FlexGmresClass::Solve,
+// fgmres.hpp:263-558, with Krylov dimension = iteration limit = K and relative
+// tolerance inner_tol; PargemslrMgs).  Operator: S z = C z - E B^{-1} F z
+// (GemslrClass::SchurMatVec, gemslr.cpp:3799-3955); preconditioner:
+// SolveLevel(1, v + W H W^H v) (GemslrSchurMatrixClass::Solve, gemslr.cpp:292-320).
+// All K steps are launched for every column; a column that has converged (or
+// broken down) is masked by its state as in the batched outer FGMRES, so the
+// result per column is that of the reference's early exit, and no launch
+// waits for the host (the steps may be captured into graphs).
+template <typename V>
+void inner_schur(pgs_dev_op_s* op, pgs_ws_s* ws, DevPrec<V>& P, const V* b, V* x, int nrhs, hipStream_t s) {
+  using S = typename InnerScalar<V>::type;
+  const DevLevel& L0 = *P.levels[0];
+  const int64_t m = L0.nrem;
+  const int K = op->inner_maxits;
+  const int64_t ldv = m * nrhs;
+  const unsigned nb = nblocks_for(ldv, kNT);
+  V* Vb = ws->in_V.as<V>();
+  V* Zb = ws->in_Z.as<V>();
+  V* T = ws->in_T.as<V>();
+  V* U = ws->in_U.as<V>();
+  V* Y = ws->in_Y.as<V>();
+  S* h = ws->in_h.as<S>();
+  double* t2 = ws->in_t2.as<double>();
+  S* H = ws->in_H.as<S>();
+  S* rs = ws->in_rs.as<S>();
+  KState* st = ws->in_st.as<KState>();
+  // ||b||; x0 = 0, so the residual is b and normr = normb
+  colnorm2<V>(ws, m, nrhs, b, ws->in_nb2.as<double>(), s);
+  k_fgmres_init<<<1, 64, 0, s>>>(nrhs, ws->in_nb2.as<double>(), ws->in_nb2.as<double>(), op->inner_tol, 0, st,
+                                 ws->in_flag.as<double>());
+  k_fill_zero<V><<<nb, kNT, 0, s>>>(ldv, x);
+  // v0 = b / normb
+  k_copy_conv<V, V><<<nb, kNT, 0, s>>>(ldv, b, Vb);
+  k_cycle_start<S><<<1, 64, 0, s>>>(nrhs, K, st, rs);
+  k_scale_cols<V><<<nb, kNT, 0, s>>>(m, nrhs, st, Vb);
+  HIPCHK(hipGetLastError());
+  for (int i = 1; i <= K; ++i) {
+    V* vprev = Vb + (int64_t)(i - 1) * ldv;
+    V* w = Vb + (int64_t)i * ldv;
+    V* zc = Zb + (int64_t)(i - 1) * ldv;
+    // z = SolveLevel(1, v + W H W^H v)
+    k_copy_conv<V, V><<<nb, kNT, 0, s>>>(ldv, vprev, T);
+    launch_lowrank<V>(ws, L0, T, nrhs, s);
+    solve_level<V>(op, ws, P, 1, T, zc, nrhs, s);
+    // w = -E B^{-1} F z, then w += C z (the order of SchurMatVec)
+    launch_spmv<V, V, V, V>(L0.F, zc, U, (const V*)nullptr, V(1.0), V(0.0), 0, nrhs, s);
+    launch_trsv<V>(op, ws, 0, L0.B, U, Y, nrhs, s);
+    launch_spmv<V, V, V, V>(L0.E, Y, w, (const V*)nullptr, V(-1.0), V(0.0), 0, nrhs, s);
+    launch_spmv<V, V, V, V>(L0.C, zc, w, (const V*)nullptr, V(1.0), V(1.0), 0, nrhs, s);
+    // modified Gram-Schmidt against v_0..v_{i-1}; ||w||^2 with the last update
+    for (int j = 0; j < i; ++j) {
+      multidot<V, V, S>(ws, m, nrhs, 1, Vb + (int64_t)j * ldv, ldv, w, h + (int64_t)j * nrhs, 0, s);
+      multi_axpy<V, S>(ws, m, nrhs, 1, Vb + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w,
+                       j == i - 1 ? t2 : nullptr, s);
+    }
+    k_givens<S><<<1, 64, 0, s>>>(nrhs, K, i, K, h, t2, H, ws->in_cs.as<S>(), ws->in_sn.as<S>(), rs, st);
+    k_scale_cols<V><<<nb, kNT, 0, s>>>(m, nrhs, st, w);
+    HIPCHK(hipGetLastError());
+  }
+  // x = Z y over each column's steps
+  k_backsolve<S><<<1, 64, 0, s>>>(nrhs, K, H, rs, st, ws->in_coef.as<S>(), ws->in_upd.as<int>());
+  k_update_x<V, S, V><<<nb, kNT, sizeof(S) * K * nrhs, s>>>(m, nrhs, K, Zb, ldv, ws->in_coef.as<S>(),
+                                                            ws->in_upd.as<int>(), x);
+  HIPCHK(hipGetLastError());
+}
+
 // SolveLevel (GemslrClass::SolveLevel, gemslr.cpp:3474-3625) on the device.
 template <typename V>
 void solve_level(pgs_dev_op_s* op, pgs_ws_s* ws, DevPrec<V>& P, int l, const V* rhs, V* x, int nrhs, hipStream_t s) {
@@ -517,10 +650,15 @@ void solve_level(pgs_dev_op_s* op, pgs_ws_s* ws, DevPrec<V>& P, int l, const V* 
   launch_trsv<V>(op, ws, l, L.B, rhs, z, nrhs, s);
   // z_l = rhs_l - E z_u
   launch_spmv<V, V, V, V>(L.E, z, zl, rhs + (int64_t)nl * nrhs, (V)1, (V)0, 1, nrhs, s);
-  // z_l += W H W^H z_l
-  launch_lowrank<V>(ws, L, zl, nrhs, s);
-  // x_l = SolveLevel(l+1, z_l)
-  solve_level<V>(op, ws, P, l + 1, zl, x + (int64_t)nl * nrhs, nrhs, s);
+  if (l == 0 && op->inner_maxits > 0) {
+    // x_l = inner FGMRES on the Schur complement (gemslr.cpp:3556-3563)
+    inner_schur<V>(op, ws, P, zl, x + (int64_t)nl * nrhs, nrhs, s);
+  } else {
+    // z_l += W H W^H z_l
+    launch_lowrank<V>(ws, L, zl, nrhs, s);
+    // x_l = SolveLevel(l+1, z_l)
+    solve_level<V>(op, ws, P, l + 1, zl, x + (int64_t)nl * nrhs, nrhs, s);
+  }
   // z_u = rhs_u - F x_l
   launch_spmv<V, V, V, V>(L.F, x + (int64_t)nl * nrhs, z, rhs, (V)1, (V)0, 1, nrhs, s);
   // x_u = B^{-1} z_u
@@ -551,6 +689,10 @@ void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask) {
   op->n = H.n;
   op->nlev = H.nlev;
   op->lev_ptr = H.lev_ptr;
+  if (H.prm.schur_enable && H.nlev >= 2) {
+    op->inner_maxits = H.prm.schur_maxits;
+    op->inner_tol = H.prm.schur_tol;
+  }
   upload_csr<V64>(op->A, H.A, &op->bytes);
   upload(op->perm, H.perm, &op->bytes);
   // ring packing (ring kernel) unless another kernel geometry was asked for;
@@ -584,6 +726,7 @@ void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask) {
           upload(L->W, to_row_major<V>(HL.W, L->nrem, HL.rank), &op->bytes);
           upload(L->WH, to_row_major<V>(HL.WH, L->nrem, HL.rank), &op->bytes);
         }
+        if (l == 0 && op->inner_maxits > 0) upload_csr<V>(L->C, HL.C, &op->bytes);
       }
       P.levels.push_back(std::move(L));
     }
@@ -699,6 +842,10 @@ int pgs_ws_estimate(pgs_dev_op_t dop, int max_nrhs, int kdim, int64_t* bytes) {
   b += (int64_t)1025 * (kdim + 1) * nr * es;                // reduction partials
   b += (int64_t)(kLrBlocks + 2) * 64 * nr * es;             // low-rank partials
   b += (int64_t)(kdim + 4) * (kdim + 1) * nr * es + 4096;   // Hessenberg, rotations, state
+  if (op->inner_maxits > 0) {  // inner Schur iteration: 2K+2 vectors of the level-0 remainder, 2 of level 0
+    const int64_t K = op->inner_maxits, m0 = n - op->lev_ptr[1], nl0 = op->lev_ptr[1];
+    b += ((2 * K + 2) * m0 + 2 * nl0) * nr * es + (K + 4) * (K + 1) * nr * 16 + 4096;
+  }
   *bytes = b;
   return 0;
 }
@@ -759,6 +906,27 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
     ws->nb2.alloc((size_t)nr * sizeof(double));
     ws->nr2.alloc((size_t)nr * sizeof(double));
     ws->flag.alloc(16);
+    if (op->inner_maxits > 0) {
+      const int K = op->inner_maxits;
+      const int64_t m0 = n - op->lev_ptr[1], nl0 = op->lev_ptr[1];
+      const size_t ses = es;  // inner scalars: double or complex<double>, as the operator's values
+      ws->in_V.alloc((size_t)(K + 1) * m0 * nr * es);
+      ws->in_Z.alloc((size_t)K * m0 * nr * es);
+      ws->in_T.alloc((size_t)m0 * nr * es);
+      ws->in_U.alloc((size_t)std::max<int64_t>(1, nl0) * nr * es);
+      ws->in_Y.alloc((size_t)std::max<int64_t>(1, nl0) * nr * es);
+      ws->in_h.alloc((size_t)K * nr * ses);
+      ws->in_t2.alloc((size_t)nr * sizeof(double));
+      ws->in_H.alloc((size_t)nr * (K + 1) * K * ses);
+      ws->in_cs.alloc((size_t)nr * K * ses);
+      ws->in_sn.alloc((size_t)nr * K * ses);
+      ws->in_rs.alloc((size_t)nr * (K + 1) * ses);
+      ws->in_coef.alloc((size_t)K * nr * ses);
+      ws->in_upd.alloc((size_t)nr * sizeof(int));
+      ws->in_st.alloc((size_t)nr * sizeof(KState));
+      ws->in_flag.alloc((size_t)nr * sizeof(double));
+      ws->in_nb2.alloc((size_t)nr * sizeof(double));
+    }
     HIPCHK(hipHostMalloc((void**)&ws->flag_host, 16, hipHostMallocDefault));
   } catch (const HipError& e) {
     return set_err(e);
@@ -838,48 +1006,6 @@ int pgs_apply(pgs_dev_op_t dop, pgs_ws_t w, const double* v, double* z, int nrhs
 }  // extern "C"
 
 namespace {
-
-// out[g] (+)= sum of nblk partials of p*nrhs values of S (a complex value is
-// summed as its two doubles)
-template <typename S>
-void dot_final(int nblk, int p, int nrhs, const S* part, S* out, int accumulate, hipStream_t s) {
-  const int pp = p * (int)(sizeof(S) / sizeof(double));
-  k_dot_final_wave<<<(pp * nrhs + 3) / 4, 256, 0, s>>>(nblk, pp, nrhs, reinterpret_cast<const double*>(part),
-                                                       reinterpret_cast<double*>(out), accumulate);
-}
-
-// multi-dot over the first p vectors of V against w: out[j*nrhs+c] = V_j^H w
-template <typename VV, typename VW, typename S>
-void multidot(pgs_ws_s* ws, int64_t n, int nrhs, int p, const VV* V, int64_t ldv, const VW* w, S* out,
-              int accumulate, hipStream_t s) {
-  const int64_t rows_per_blk = (n + ws->part_blocks - 1) / ws->part_blocks;
-  const int nblk = (int)((n + rows_per_blk - 1) / rows_per_blk);
-  k_multidot<VV, VW, S, kNT><<<nblk, kNT, 0, s>>>(n, nrhs, p, V, ldv, w, ws->part.as<S>(), rows_per_blk);
-  dot_final<S>(nblk, p, nrhs, ws->part.as<S>(), out, accumulate, s);
-  HIPCHK(hipGetLastError());
-}
-
-// out[c] = ||x(:, c)||^2
-template <typename S>
-void colnorm2(pgs_ws_s* ws, int64_t n, int nrhs, const S* x, double* out, hipStream_t s) {
-  const int64_t rows_per_blk = (n + ws->part_blocks - 1) / ws->part_blocks;
-  const int nblk = (int)((n + rows_per_blk - 1) / rows_per_blk);
-  k_colnorm2<S, kNT><<<nblk, kNT, 0, s>>>(n, nrhs, x, ws->part.as<double>(), rows_per_blk);
-  k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, ws->part.as<double>(), out, 0);
-  HIPCHK(hipGetLastError());
-}
-
-// w -= V(:, 0:p) h; with t2, t2[c] = ||w(:, c)||^2 of the result
-template <typename S>
-void multi_axpy(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ldv, const S* h, S* w, double* t2,
-                hipStream_t s) {
-  // enough workgroups to keep every CU busy on the streaming update
-  const int nblk = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n * nrhs + kNT - 1) / kNT));
-  const size_t shm = sizeof(S) * (size_t)p * nrhs + sizeof(double) * (kNT / 64) * nrhs;
-  k_multi_axpy_norm<S, kNT><<<nblk, kNT, shm, s>>>(n, nrhs, p, V, ldv, h, w, t2 ? ws->part.as<double>() : nullptr);
-  if (t2) k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, ws->part.as<double>(), t2, 0);
-  HIPCHK(hipGetLastError());
-}
 
 template <int PM, int MODE, int LPE>
 void launch_basis_mode(int lds_max, int nblk, size_t lds, int64_t n, int nrhs, int p, const double* V, int64_t ldv,

@@ -290,12 +290,14 @@ __global__ __launch_bounds__(256) void k_dot_final_wave(int nblk, int p, int nrh
 }
 
 // w[r, c] -= sum_{j<p} V_j[r, c] * h[j, c]  and, when part != nullptr, the
-// per-workgroup partial sums of |w[r, c]|^2 (part[blk * nrhs + c]).
-// Dynamic shared memory: p*nrhs values of S, then (NT/64)*nrhs doubles.
-template <typename S, int NT>
-__global__ __launch_bounds__(NT) void k_multi_axpy_norm(int64_t n, int nrhs, int p, const S* __restrict__ V,
+// per-workgroup partial sums of |w[r, c]|^2 (part[blk * nrhs + c]).  V and w
+// hold VV, h and the arithmetic S (VV = S, or float vectors with double
+// coefficients).  Dynamic shared memory: p*nrhs values of S, then
+// (NT/64)*nrhs doubles.
+template <typename VV, typename S, int NT>
+__global__ __launch_bounds__(NT) void k_multi_axpy_norm(int64_t n, int nrhs, int p, const VV* __restrict__ V,
                                                         int64_t ldv, const S* __restrict__ h,
-                                                        S* __restrict__ w, double* __restrict__ part) {
+                                                        VV* __restrict__ w, double* __restrict__ part) {
   extern __shared__ unsigned char ma_smem[];
   S* hs = reinterpret_cast<S*>(ma_smem);
   double* red = reinterpret_cast<double*>(hs + p * nrhs);
@@ -310,10 +312,11 @@ __global__ __launch_bounds__(NT) void k_multi_axpy_norm(int64_t n, int nrhs, int
   double sq = 0;
   for (int64_t i = i0 + tid / nrhs; i < i1; i += lanes) {
     const int64_t g = i * nrhs + c;
-    S acc = w[g];
-    for (int j = 0; j < p; ++j) acc -= V[j * ldv + g] * hs[j * nrhs + c];
-    w[g] = acc;
-    sq += (double)sabs2(acc);
+    S acc = (S)w[g];
+    for (int j = 0; j < p; ++j) acc -= (S)V[j * ldv + g] * hs[j * nrhs + c];
+    const VV out = (VV)acc;
+    w[g] = out;
+    sq += (double)sabs2(out);
   }
   if (part == nullptr) return;  // uniform
   for (int m = kWave / 2; m >= nrhs; m >>= 1) sq += __shfl_xor(sq, m);
@@ -568,11 +571,11 @@ __global__ void k_backsolve(int nrhs, int m, const S* __restrict__ H, S* __restr
   upd[c] = 1;
 }
 
-// x[r, c] += sum_j coef[j, c] * Z_j[r, c] for columns with upd[c]
-// (dynamic shared memory: p*nrhs doubles)
-template <typename VZ, typename S>
+// x[r, c] += sum_j coef[j, c] * Z_j[r, c] for columns with upd[c], summed
+// in S, x stored as VX (dynamic shared memory: p*nrhs values of S)
+template <typename VZ, typename S, typename VX = S>
 __global__ void k_update_x(int64_t n, int nrhs, int p, const VZ* __restrict__ Z, int64_t ldz,
-                           const S* __restrict__ coef, const int* __restrict__ upd, S* __restrict__ x) {
+                           const S* __restrict__ coef, const int* __restrict__ upd, VX* __restrict__ x) {
   extern __shared__ unsigned char ux_smem[];
   S* cs = reinterpret_cast<S*>(ux_smem);
   for (int t = threadIdx.x; t < p * nrhs; t += blockDim.x) cs[t] = coef[t];
@@ -581,9 +584,17 @@ __global__ void k_update_x(int64_t n, int nrhs, int p, const VZ* __restrict__ Z,
   if (g >= n * nrhs) return;
   const int c = (int)(g % nrhs);
   if (!upd[c]) return;
-  S acc = x[g];
+  S acc = (S)x[g];
   for (int j = 0; j < p; ++j) acc += cs[j * nrhs + c] * (S)Z[j * ldz + g];
-  x[g] = acc;
+  x[g] = (VX)acc;
+}
+
+// x[t] = 0 for t < m (a kernel rather than a memset: launches may be
+// captured into graphs)
+template <typename V>
+__global__ void k_fill_zero(int64_t m, V* __restrict__ x) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) x[t] = V(0.0);
 }
 
 // Restart bookkeeping: any column still running?  flag[0] = number of running columns.

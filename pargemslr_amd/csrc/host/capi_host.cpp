@@ -31,7 +31,11 @@ int pgs_gemslr_setup(int n, const int* ia, const int* ja, const double* a, int i
   GemslrParams prm;
   params_from_array(params, prm);
   prm.seed = seed;
-  if (prm.schur_enable != 0) return PGS_ERROR_INVALID_OPTION;  // inner Schur iteration is not implemented
+  // inner Schur iteration: at least one and at most kSchurMaxIts steps (the
+  // device keeps its Krylov basis in one workspace allocation)
+  if (prm.schur_enable != 0 && (prm.schur_maxits < 1 || prm.schur_maxits > PGS_SCHUR_MAXITS_LIMIT ||
+                                !(prm.schur_tol >= 0.0)))
+    return PGS_ERROR_INVALID_OPTION;
   pgs_gemslr_s* h = new (std::nothrow) pgs_gemslr_s();
   if (!h) return PGS_ERR_OOM;
   h->is_complex = is_complex ? 1 : 0;
@@ -116,6 +120,10 @@ void fill_view(const HostOp<T>& H, int l, pgs_level_view_t* v) {
   v->F_a = reinterpret_cast<const double*>(L.F.a.data());
   v->W = L.rank > 0 ? reinterpret_cast<const double*>(L.W.data()) : nullptr;
   v->WH = L.rank > 0 ? reinterpret_cast<const double*>(L.WH.data()) : nullptr;
+  const bool has_c = l == 0 && H.prm.schur_enable && !L.C.ia.empty();
+  v->C_ia = has_c ? L.C.ia.data() : nullptr;
+  v->C_ja = has_c ? L.C.ja.data() : nullptr;
+  v->C_a = has_c ? reinterpret_cast<const double*>(L.C.a.data()) : nullptr;
 }
 }  // namespace
 

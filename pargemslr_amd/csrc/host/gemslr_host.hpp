@@ -54,7 +54,7 @@ struct HostLevel {
   std::vector<int> dom_ptr;          // ncomp+1 global offsets (permuted numbering)
   std::vector<IluFactor<T>> ilu;     // one per block
   Csr<T> E, F;                       // E: m x nloc, F: nloc x m (levels < nlev-1)
-  Csr<T> C;                          // m x m lower-right block (setup only)
+  Csr<T> C;                          // m x m lower-right block (setup; kept on level 0 for the inner iteration)
   int rank = 0;                      // k
   std::vector<T> W, WH;              // m x k column-major
   std::vector<T> Hk;                 // k x k: (I - H)^{-1} - I
@@ -72,6 +72,7 @@ struct HostOp {
   std::vector<int> lev_ptr;          // nlev+1
   std::vector<HostLevel<T>> levels;
   GemslrParams prm;
+  bool solve_phase = false;  // set when the setup is complete: SolveLevel may use the inner iteration
   double setup_seconds[4] = {0, 0, 0, 0};  // partition, ilut, lowrank, total
   int nloc(int l) const { return lev_ptr[l + 1] - lev_ptr[l]; }
   int nrem(int l) const { return n - lev_ptr[l + 1]; }

@@ -116,6 +116,137 @@ void params_from_array(const double* p, GemslrParams& g) {
 }
 
 template <typename T>
+void host_solve_level(const HostOp<T>& op, int l, const T* rhs, T* x, HostApplyWork<T>& w);
+
+namespace {
+
+template <typename T>
+double abs_t(const T& v) { return std::abs(v); }
+
+// y = M (v + W H W^H v) with M the GeMSLR solve of the levels below 0
+// (GemslrSchurMatrixClass::Solve, gemslr.cpp:292-320); t: scratch of m values.
+template <typename T>
+void schur_precond(const HostOp<T>& op, const T* v, T* y, std::vector<T>& t, HostApplyWork<T>& w) {
+  const HostLevel<T>& L = op.levels[0];
+  const int m = op.nrem(0);
+  t.assign(v, v + m);
+  if (L.rank > 0) {
+    std::vector<T> c(L.rank, T(0));
+    for (int j = 0; j < L.rank; ++j) {
+      T s = T(0);
+      const T* wc = &L.W[(size_t)j * m];
+      for (int i = 0; i < m; ++i) s += sconj(wc[i]) * v[i];
+      c[j] = s;
+    }
+    // xlr = WH c, then xlr += v (SolveApplyLowRankLevel + Axpy)
+    std::vector<T> xlr(m, T(0));
+    for (int j = 0; j < L.rank; ++j) {
+      const T* whc = &L.WH[(size_t)j * m];
+      for (int i = 0; i < m; ++i) xlr[i] += whc[i] * c[j];
+    }
+    for (int i = 0; i < m; ++i) t[i] = xlr[i] + v[i];
+  }
+  host_solve_level(op, 1, t.data(), y, w);
+}
+
+// y = S x, S = C - E B^{-1} F on level 0 with B^{-1} the block ILU solve
+// (GemslrClass::SchurMatVec, gemslr.cpp:3799-3955, alpha = 1, beta = 0):
+// y = -E B^{-1} F x, then y += C x.
+template <typename T>
+void schur_matvec(const HostOp<T>& op, const T* x, T* y, std::vector<T>& u, std::vector<T>& b) {
+  const HostLevel<T>& L = op.levels[0];
+  const int nl = op.nloc(0), m = op.nrem(0);
+  u.assign(nl, T(0));
+  b.assign(nl, T(0));
+  for (int i = 0; i < nl; ++i) {
+    T s = T(0);
+    for (int k = L.F.ia[i]; k < L.F.ia[i + 1]; ++k) s += L.F.a[k] * x[L.F.ja[k]];
+    u[i] = s;
+  }
+  solve_b(op, 0, u.data(), b.data());
+  for (int i = 0; i < m; ++i) {
+    T e = T(0), c = T(0);
+    for (int k = L.E.ia[i]; k < L.E.ia[i + 1]; ++k) e += L.E.a[k] * b[L.E.ja[k]];
+    for (int k = L.C.ia[i]; k < L.C.ia[i + 1]; ++k) c += L.C.a[k] * x[L.C.ja[k]];
+    y[i] = -e + c;
+  }
+}
+
+// Inner FGMRES on the level-0 Schur complement with x0 = 0 (the solve phase
+// of SolveLevel with inner iterations, gemslr.cpp:3556-3563; FlexGmresClass::
+// Solve, fgmres.hpp:263-558, with Krylov dimension = iteration limit =
+// schur_maxits and relative tolerance schur_tol, MGS as PargemslrMgs).
+template <typename T>
+void schur_inner_solve(const HostOp<T>& op, const T* b, T* x, HostApplyWork<T>& w) {
+  const int m = op.nrem(0);
+  const int K = std::max(1, op.prm.schur_maxits);
+  const double eps = 2.220446049250313e-16;
+  for (int i = 0; i < m; ++i) x[i] = T(0);
+  double normb = 0.0;
+  {
+    T s = T(0);
+    for (int i = 0; i < m; ++i) s += sconj(b[i]) * b[i];
+    normb = std::sqrt(abs_t(s));
+  }
+  if (normb < eps) return;
+  // x0 = 0: the residual is b
+  const double tolr = op.prm.schur_tol * normb;
+  std::vector<T> V((size_t)(K + 1) * m), Z((size_t)K * m), H((size_t)(K + 1) * K, T(0)), c(K), sn(K), rs(K + 1);
+  std::vector<T> t, u, bb;
+  double normr = normb;
+  rs[0] = normr;
+  for (int r = 0; r < m; ++r) V[r] = b[r] * (1.0 / normr);
+  auto Hm = [&](int i, int j) -> T& { return H[(size_t)i + (size_t)j * (K + 1)]; };
+  int i = 0;
+  while (i < K) {
+    ++i;
+    T* v = &V[(size_t)(i - 1) * m];
+    T* z = &Z[(size_t)(i - 1) * m];
+    T* wv = &V[(size_t)i * m];
+    schur_precond(op, v, z, t, w);
+    schur_matvec(op, z, wv, u, bb);
+    for (int j = 0; j < i; ++j) {
+      const T* vj = &V[(size_t)j * m];
+      T h = T(0);
+      for (int r = 0; r < m; ++r) h += sconj(vj[r]) * wv[r];
+      Hm(j, i - 1) = h;
+      for (int r = 0; r < m; ++r) wv[r] -= h * vj[r];
+    }
+    T s2 = T(0);
+    for (int r = 0; r < m; ++r) s2 += sconj(wv[r]) * wv[r];
+    const double tn = std::sqrt(abs_t(s2));
+    Hm(i, i - 1) = tn;
+    for (int r = 0; r < m; ++r) wv[r] *= T(1.0) / T(tn);
+    for (int j = 1; j < i; ++j) {
+      const T hii = Hm(j - 1, i - 1);
+      Hm(j - 1, i - 1) = sconj(c[j - 1]) * hii + sn[j - 1] * Hm(j, i - 1);
+      Hm(j, i - 1) = -sn[j - 1] * hii + c[j - 1] * Hm(j, i - 1);
+    }
+    const T hii = Hm(i - 1, i - 1), hii1 = Hm(i, i - 1);
+    const double gam = std::sqrt(abs_t(hii) * abs_t(hii) + abs_t(hii1) * abs_t(hii1));
+    if (std::fabs(gam) < eps) return;  // breakdown: x stays at the initial guess
+    c[i - 1] = hii / gam;
+    sn[i - 1] = hii1 / gam;
+    rs[i] = -sn[i - 1] * rs[i - 1];
+    rs[i - 1] = sconj(c[i - 1]) * rs[i - 1];
+    Hm(i - 1, i - 1) = sconj(c[i - 1]) * hii + sn[i - 1] * hii1;
+    normr = abs_t(rs[i]);
+    if (normr <= tolr) break;
+  }
+  rs[i - 1] /= Hm(i - 1, i - 1);
+  for (int k = i - 2; k >= 0; --k) {
+    for (int j = k + 1; j < i; ++j) rs[k] -= Hm(k, j) * rs[j];
+    rs[k] /= Hm(k, k);
+  }
+  for (int j = 0; j < i; ++j) {
+    const T* z = &Z[(size_t)j * m];
+    for (int r = 0; r < m; ++r) x[r] += rs[j] * z[r];
+  }
+}
+
+}  // namespace
+
+template <typename T>
 void host_solve_level(const HostOp<T>& op, int l, const T* rhs, T* x, HostApplyWork<T>& w) {
   if (l == op.nlev - 1) {
     solve_b(op, l, rhs, x);
@@ -132,22 +263,27 @@ void host_solve_level(const HostOp<T>& op, int l, const T* rhs, T* x, HostApplyW
     for (int k = L.E.ia[i]; k < L.E.ia[i + 1]; ++k) s += L.E.a[k] * z[L.E.ja[k]];
     zl[i] = rhs[nl + i] - s;
   }
-  if (L.rank > 0) {
-    const int k = L.rank;
-    std::vector<T>& t = w.t;
-    t.assign(k, T(0));
-    for (int j = 0; j < k; ++j) {
-      T s = T(0);
-      const T* wc = &L.W[(size_t)j * m];
-      for (int i = 0; i < m; ++i) s += sconj(wc[i]) * zl[i];
-      t[j] = s;
+  if (l == 0 && op.solve_phase && op.prm.schur_enable) {
+    // x_l = inner FGMRES on the Schur complement (gemslr.cpp:3556-3563)
+    schur_inner_solve(op, zl, x + nl, w);
+  } else {
+    if (L.rank > 0) {
+      const int k = L.rank;
+      std::vector<T>& t = w.t;
+      t.assign(k, T(0));
+      for (int j = 0; j < k; ++j) {
+        T s = T(0);
+        const T* wc = &L.W[(size_t)j * m];
+        for (int i = 0; i < m; ++i) s += sconj(wc[i]) * zl[i];
+        t[j] = s;
+      }
+      for (int j = 0; j < k; ++j) {
+        const T* whc = &L.WH[(size_t)j * m];
+        for (int i = 0; i < m; ++i) zl[i] += whc[i] * t[j];
+      }
     }
-    for (int j = 0; j < k; ++j) {
-      const T* whc = &L.WH[(size_t)j * m];
-      for (int i = 0; i < m; ++i) zl[i] += whc[i] * t[j];
-    }
+    host_solve_level(op, l + 1, zl, x + nl, w);
   }
-  host_solve_level(op, l + 1, zl, x + nl, w);
   // z_u = rhs_u - F x_l
   for (int i = 0; i < nl; ++i) {
     T s = T(0);
@@ -566,7 +702,7 @@ int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::s
     L.E = csr_block(Apq, e, n - e, s, e - s);
     L.F = csr_block(Apq, s, e - s, e, n - e);
     int rank = (l == 0) ? prm.lr_rank1 : prm.lr_rank2;
-    if (rank > 0) L.C = csr_block(Apq, e, n - e, e, n - e);
+    if (rank > 0 || (l == 0 && prm.schur_enable)) L.C = csr_block(Apq, e, n - e, e, n - e);
   }
   // concatenated level factors (block-diagonal, level-local numbering)
   for (int l = 0; l < nlev; ++l) {
@@ -596,8 +732,10 @@ int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::s
   // 4. low-rank corrections, bottom level first
   for (int l = nlev - 2; l >= 0; --l) {
     lowrank_level(op, l);
-    op.levels[l].C = Csr<T>();
+    // the level-0 block C stays for the inner Schur iteration of the solve phase
+    if (!(l == 0 && prm.schur_enable)) op.levels[l].C = Csr<T>();
   }
+  op.solve_phase = true;
   const double t3 = now_s();
   op.setup_seconds[0] = t1 - t0;
   op.setup_seconds[1] = t2 - t1;

@@ -314,6 +314,7 @@ def test_setup_error_codes():
     A = pg.laplacian(6, 6, 1)
     p = pg.config_params(2, 4, 1e-2, 10, 0)
     p[pg.capi.IO["SCHUR_ENABLE"]] = 1
+    p[pg.capi.IO["SCHUR_MAXITS"]] = 0  # the inner iteration needs at least one step
     with pytest.raises(pg.PgsError) as e:
         pg.Gemslr(A, p)
     assert e.value.code == 100
