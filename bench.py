@@ -32,6 +32,12 @@ CONFIGS = {
     # small case for quick runs
     "c1": dict(grid=(64, 64, 1), nlev=2, ncomp=4, droptol=1e-2, rownnz=10, rank=0, kdim=30, tol=1e-8),
     "m64": dict(grid=(64, 64, 64), nlev=3, ncomp=64, droptol=1e-3, rownnz=20, rank=16, kdim=50, tol=1e-8),
+    # shifted indefinite 128^3 Laplacian (sigma = 0.3), b ~ N(0, 1), rank 32, tol 1e-10
+    "c3": dict(grid=(128, 128, 128), nlev=3, ncomp=64, droptol=1e-3, rownnz=20, rank=32, kdim=50, tol=1e-10,
+               shift=0.3, rhs="normal"),
+    # upwind convection-diffusion 96^3 (c = 0.3), b ~ N(0, 1)
+    "c4": dict(grid=(96, 96, 96), nlev=3, ncomp=64, droptol=1e-3, rownnz=20, rank=16, kdim=50, tol=1e-8,
+               convdiff=0.3, rhs="normal"),
 }
 HBM_PEAK_GBS = 8000.0
 # HBM bytes read + written per level-0 ring-solve launch of the 128^3 operator
@@ -60,6 +66,8 @@ def main():
     ap.add_argument("--requests", type=int, default=32, help="requests of the serving stream")
     ap.add_argument("--rate", type=float, default=2.0, help="mean request arrivals per second")
     ap.add_argument("--graphs", type=int, default=1, help="serving: replay captured FGMRES steps")
+    ap.add_argument("--schur", default="",
+                    help="inner Schur iteration 'maxits,tol' (e.g. 5,1e-2, the reference's default); off if empty")
     ap.add_argument("--probe", action="store_true",
                     help="counter probe: --nrhs FP32 applies of the 128^3 operator and a 1 GiB copy")
     args = ap.parse_args()
@@ -80,9 +88,17 @@ def main():
     dev = 0  # HIP device 0 through the library's own runtime
     nr = args.nrhs
     t0 = time.time()
-    A = pg.laplacian(*cfg["grid"])
+    if "convdiff" in cfg:
+        A = pg.convdiff(*cfg["grid"], cfg["convdiff"])
+    else:
+        A = pg.laplacian(*cfg["grid"], shift=cfg.get("shift", 0.0))
     log(f"[bench] matrix n={A.n} nnz={A.nnz} ({time.time() - t0:.1f}s)")
-    params = pg.config_params(cfg["nlev"], cfg["ncomp"], cfg["droptol"], cfg["rownnz"], cfg["rank"], kdim=cfg["kdim"])
+    schur = None
+    if args.schur:
+        k, t = args.schur.split(",")
+        schur = (int(k), float(t))
+    params = pg.config_params(cfg["nlev"], cfg["ncomp"], cfg["droptol"], cfg["rownnz"], cfg["rank"], kdim=cfg["kdim"],
+                              schur=schur)
     G = pg.Gemslr(A, params, seed=0)
     inf = G.info()
     log(f"[bench] setup {inf.setup_seconds[3]:.1f}s (partition {inf.setup_seconds[0]:.1f}, ilut {inf.setup_seconds[1]:.1f},"
@@ -94,8 +110,12 @@ def main():
     log(f"[bench] device operator {D.bytes() / 1e9:.2f} GB")
 
     n = A.n
-    Xs = np.stack([pg.rng_uniform(c, n) for c in range(nr)], axis=1)
-    Bh = A.matvec(Xs)
+    if cfg.get("rhs") == "normal":
+        Xs = None
+        Bh = np.stack([pg.rng_normal(c, n) for c in range(nr)], axis=1)
+    else:
+        Xs = np.stack([pg.rng_uniform(c, n) for c in range(nr)], axis=1)
+        Bh = A.matvec(Xs)
     B = pg.DeviceArray.from_numpy(Bh, device=dev)
     b1 = pg.DeviceArray.from_numpy(np.ascontiguousarray(Bh[:, 0]), device=dev)
     x1 = pg.DeviceArray(n, device=dev)
@@ -161,7 +181,7 @@ def main():
     XBh = XB.numpy()
     true_res = [float(np.linalg.norm(Bh[:, c] - A.matvec(XBh[:, c])) / np.linalg.norm(Bh[:, c])) for c in range(nr)]
     err_vs_fp64 = float(np.linalg.norm(XBh[:, 0] - x64) / np.linalg.norm(x64))
-    err_vs_xstar = float(np.linalg.norm(XBh - Xs) / np.linalg.norm(Xs))
+    err_vs_xstar = float(np.linalg.norm(XBh - Xs) / np.linalg.norm(Xs)) if Xs is not None else None
     iters = [r[1] for r in resB]
     ok = all(r[0] == 0 for r in resB) and max(true_res) <= tol
     solves_per_s = nr * args.steps / T
@@ -211,8 +231,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32-apply/f64-krylov",
         "data": "synthetic",
-        "config": {"workload": f"laplacian3d_{cfg['grid'][0]}^3 gemslr nlev={cfg['nlev']} ncomp={cfg['ncomp']} "
-                               f"ilut={cfg['droptol']}/{cfg['rownnz']} rank={cfg['rank']} fgmres({kd}) tol={tol}",
+        "config": {"workload": f"{'convdiff' if 'convdiff' in cfg else 'laplacian'}3d_{cfg['grid'][0]}^3"
+                               f"{' shift=' + str(cfg['shift']) if 'shift' in cfg else ''} gemslr nlev={cfg['nlev']} "
+                               f"ncomp={cfg['ncomp']} ilut={cfg['droptol']}/{cfg['rownnz']} rank={cfg['rank']} "
+                               f"fgmres({kd}) tol={tol}"
+                               f"{' inner=fgmres(%d,%g)' % schur if schur else ''}",
                    "nrhs_per_step": nr, "n": n, "nnz": A.nnz},
         "latency_ms": {"batched_step_p50": round(float(np.percentile(step_t, 50)) * 1e3, 2),
                        "batched_step_p99": round(float(np.percentile(step_t, 99)) * 1e3, 2),
@@ -283,7 +306,11 @@ def serve(args):
             return pg.rng_normal(seed, 2 * A.n).view(np.complex128)
         return pg.rng_normal(seed, A.n)
 
-    params = pg.config_params(3, 64, 1e-3, 20, 16, kdim=50)
+    schur = None
+    if args.schur:
+        k, t = args.schur.split(",")
+        schur = (int(k), float(t))
+    params = pg.config_params(3, 64, 1e-3, 20, 16, kdim=50, schur=schur)
     ops = []
     for name, A in mats:
         t0 = time.time()
@@ -340,7 +367,8 @@ def serve(args):
         "dtype": "f32-apply/f64-krylov (complex64/complex128 for the complex operator)", "data": "synthetic",
         "config": {"workload": "serving stream: " + ", ".join(n for n, _, _, _ in ops),
                    "requests": len(plan), "rhs_total": cols, "arrival_rate_per_s": args.rate,
-                   "max_batch": 16, "slots": 3, "graphs": bool(args.graphs)},
+                   "max_batch": 16, "slots": 3, "graphs": bool(args.graphs),
+                   "inner_schur": list(schur) if schur else None},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
                        "mean_queue": round(float(np.mean([i["queue_ms"] for _, _, i in out])), 2)},
         "batch_nrhs_mean": round(float(np.mean([i["batch_nrhs"] for _, _, i in out])), 2),
