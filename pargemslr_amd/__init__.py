@@ -6,7 +6,7 @@ behind the C ABI of include/pargemslr_amd.h; this package binds it with
 ctypes.  Importing does not touch the GPU.
 """
 from .capi import (  # noqa: F401
-    PGS_ORTH_CGS2, PGS_ORTH_MGS, PGS_PREC_FP32, PGS_PREC_FP64, CsrHost, DeviceArray, DevOp, Gemslr, PgsError,
+    PGS_ORTH_CGS2, PGS_ORTH_DCGS2, PGS_ORTH_MGS, PGS_PREC_FP32, PGS_PREC_FP64, CsrHost, DeviceArray, DevOp, Gemslr, PgsError,
     Server, Stream, Workspace, config_params, convdiff, default_params, device_count, laplacian, lib, rng_normal,
     rng_uniform, synchronize,
 )

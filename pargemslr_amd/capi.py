@@ -17,6 +17,7 @@ PGS_PREC_FP64 = 1
 PGS_PREC_FP32 = 2
 PGS_ORTH_MGS = 0
 PGS_ORTH_CGS2 = 1
+PGS_ORTH_DCGS2 = 2
 PGS_ERR_BREAKDOWN = 203
 PGS_ERR_NOT_CONVERGED = 204
 

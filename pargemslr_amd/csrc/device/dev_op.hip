@@ -209,6 +209,7 @@ struct pgs_ws_s {
   std::vector<std::unique_ptr<DevMem>> zlev;
   // Krylov
   DevMem V, Z, w, part, hbuf, h2buf, t2, H, cs, sn, rs, coef, upd, st, zflag, nb2, nr2, flag;
+  DevMem dc_d, dc_coef;  // delayed CGS2: the step's dot table and update coefficients
   // inner Schur iteration: basis, preconditioned basis, temporaries (vectors
   // in the apply's precision) and its Hessenberg / Givens state per column
   DevMem in_V, in_Z, in_T, in_U, in_Y, in_h, in_t2, in_H, in_cs, in_sn, in_rs, in_coef, in_upd, in_st, in_flag,
@@ -248,6 +249,7 @@ constexpr int kPMAX = 128;  // Krylov dimension + 1 limit of the shared-memory c
 constexpr int kLrBlocks = 256;  // most workgroups of the low-rank partial sums
 constexpr int kBasisNT = 256;
 constexpr int kBasisPM = 64;  // most basis vectors k_basis_reg takes
+constexpr int kPartRowsDcgs2 = 2 * 32 + 2;  // outputs per column of one k_dcgs2_dots chunk
 
 template <typename V>
 DevPrec<V>& prec_of(pgs_dev_op_s* op);
@@ -476,6 +478,12 @@ void set_kernel_attrs(int lds_max) {
     set_lds_limit(k_basis_reg<kBasisNT, 32, 0, 2>, lds_max);
     set_lds_limit(k_basis_reg<kBasisNT, 32, 1, 2>, lds_max);
     // coefficient tables of up to 127 x 64 values in LDS
+    set_lds_limit(k_dcgs2_update<kBasisNT, 8, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<kBasisNT, 16, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<kBasisNT, 32, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<kBasisNT, 64, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<kBasisNT, 32, 2>, lds_max);
+    set_lds_limit(k_dcgs2_update<kBasisNT, 64, 2>, lds_max);
     set_lds_limit(k_multi_axpy_norm<double, double, kNT>, lds_max);
     set_lds_limit(k_multi_axpy_norm<cplx<double>, cplx<double>, kNT>, lds_max);
     set_lds_limit(k_update_x<double, double>, lds_max);
@@ -839,7 +847,7 @@ int pgs_ws_estimate(pgs_dev_op_t dop, int max_nrhs, int kdim, int64_t* bytes) {
   const int64_t n = op->n, vb = n * nr * es;
   int64_t b = 2 * vb + (2 * (int64_t)kdim + 1) * vb;  // r0, x0, V, Z
   for (int l = 0; l < op->nlev; ++l) b += (n - op->lev_ptr[l]) * nr * es;
-  b += (int64_t)1025 * (kdim + 1) * nr * es;                // reduction partials
+  b += (int64_t)1025 * std::max(kdim + 1, kPartRowsDcgs2) * nr * es;  // reduction partials
   b += (int64_t)(kLrBlocks + 2) * 64 * nr * es;             // low-rank partials
   b += (int64_t)(kdim + 4) * (kdim + 1) * nr * es + 4096;   // Hessenberg, rotations, state
   if (op->inner_maxits > 0) {  // inner Schur iteration: 2K+2 vectors of the level-0 remainder, 2 of level 0
@@ -891,7 +899,8 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
     const int64_t rows_per_blk = std::max<int64_t>(256, (n + 1023) / 1024);
     ws->part_blocks = (int)((n + rows_per_blk - 1) / rows_per_blk);
     if (const char* v = std::getenv("PGS_FUSED_CGS2")) ws->fused_cgs2 = std::atoi(v) != 0;
-    ws->part.alloc((size_t)ws->part_blocks * (kdim + 1) * nr * es);
+    // partial rows per workgroup: kdim + 1 dots, or 2 * 32 + 2 of a delayed-CGS2 dot chunk
+    ws->part.alloc((size_t)ws->part_blocks * std::max(kdim + 1, kPartRowsDcgs2) * nr * es);
     ws->hbuf.alloc((size_t)(kdim + 1) * nr * es);
     ws->h2buf.alloc((size_t)(kdim + 1) * nr * es);
     ws->t2.alloc((size_t)nr * sizeof(double));
@@ -901,6 +910,8 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
     ws->rs.alloc((size_t)nr * (kdim + 1) * es);
     ws->coef.alloc((size_t)(kdim + 1) * nr * es);
     ws->upd.alloc((size_t)nr * sizeof(int));
+    ws->dc_d.alloc((size_t)(2 * kdim + 4) * nr * sizeof(double));
+    ws->dc_coef.alloc((size_t)(2 * kdim + 4) * nr * sizeof(double));
     ws->st.alloc((size_t)nr * sizeof(KState));
     ws->zflag.alloc((size_t)nr * sizeof(double));
     ws->nb2.alloc((size_t)nr * sizeof(double));
@@ -1068,6 +1079,53 @@ struct PrecTypes<cplx<double>> {
 
 // FGMRES (FlexGmresClass::Solve, fgmres.hpp:300-540) on nrhs columns at once:
 // Krylov scalar S (double or complex<double>), preconditioned basis Z in VZ.
+// Delayed CGS2, dots of step p: d = [s (p rows) | a (p rows) | q.q | q.w]
+// with s = V_{0..p-1} . q, a = V_{0..p-1} . w, q = basis slot p; with w ==
+// nullptr (the closing pass of a cycle) d = [s | q.q].  The basis is read in
+// chunks of 16 vectors per lane (two lanes per element up to 32 columns).
+void dcgs2_dots(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64_t ldv, const double* w, double* d,
+                hipStream_t s) {
+  const double* q = V + (int64_t)p * ldv;
+  const int lpe = nrhs <= 32 ? 2 : 1;
+  const int chunk = 16 * lpe;
+  const int64_t nthr = (n * nrhs * lpe + kBasisNT - 1) / kBasisNT;
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(ws->part_blocks, nthr));
+  const int hw = w != nullptr;
+  double* part = ws->part.as<double>();
+  for (int j0 = 0; j0 == 0 || j0 < p; j0 += chunk) {
+    const int cnt = std::min(chunk, p - j0);
+    const int first = j0 == 0;
+    const int nout = (hw ? 2 * cnt : cnt) + (first ? (hw ? 2 : 1) : 0);
+    const size_t lds = sizeof(double) * (size_t)nout * nrhs;
+    if (lpe == 2)
+      k_dcgs2_dots<kBasisNT, 16, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, j0, cnt, V, ldv, q, w, first, part);
+    else
+      k_dcgs2_dots<kBasisNT, 16, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, j0, cnt, V, ldv, q, w, first, part);
+    k_dcgs2_final<<<(nout * nrhs + 3) / 4, 256, 0, s>>>(nblk, nout, cnt, nrhs, hw, p, j0, part, d);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// Delayed CGS2, update of step p (k_dcgs2_update), then t2[c] = ||w(:, c)||^2.
+void dcgs2_update(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64_t ldv, double* q, double* w,
+                  const double* coef, double* t2, hipStream_t s) {
+  // one lane per element up to 32 vectors (64 with 64 columns), then two
+  const bool two = p > 32 && nrhs <= 32;
+  const int64_t nthr = (n * nrhs * (two ? 2 : 1) + kBasisNT - 1) / kBasisNT;
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(ws->part_blocks, nthr));
+  const size_t lds = sizeof(double) * ((size_t)(2 * p + 2) * nrhs + (size_t)(kBasisNT / kWave) * nrhs);
+  double* part = ws->part.as<double>();
+  if (p <= 8) k_dcgs2_update<kBasisNT, 8, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+  else if (p <= 16) k_dcgs2_update<kBasisNT, 16, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+  else if (p <= 32) k_dcgs2_update<kBasisNT, 32, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+  else if (two && p <= 64) k_dcgs2_update<kBasisNT, 32, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+  else if (two) k_dcgs2_update<kBasisNT, 64, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+  else if (p <= 64) k_dcgs2_update<kBasisNT, 64, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+  else throw std::runtime_error("delayed CGS2: more than 64 basis vectors need at most 32 columns");
+  k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, part, t2, 0);
+  HIPCHK(hipGetLastError());
+}
+
 template <typename S, typename VZ>
 int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, const pgs_fgmres_opts_t& o,
                 pgs_fgmres_result_t* res, hipStream_t s) {
@@ -1082,6 +1140,9 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
   const int nb = nblocks_for(n * nrhs, kNT);
   const int check = o.check_every > 0 ? o.check_every : 1;
   const int maxits = o.maxits;
+  // delayed CGS2 on real bases (more than 64 basis vectors need two lanes per
+  // element, i.e. at most 32 columns); complex bases use CGS2
+  const bool dcgs2 = o.orth == PGS_ORTH_DCGS2 && !scalar_traits<S>::is_complex && (m <= 64 || nrhs <= 32);
   // ||b||, v0 = b - A x, ||v0||
   colnorm2<S>(ws, n, nrhs, b, ws->nb2.as<double>(), s);
   launch_spmv<S, S, S, S>(op->A, x, V, b, S(1.0), S(0.0), 1, nrhs, s);
@@ -1120,7 +1181,23 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
     S* h = ws->hbuf.as<S>();
     S* h2 = ws->h2buf.as<S>();
     double* t2 = ws->t2.as<double>();
-    if (o.orth == PGS_ORTH_CGS2) {
+    if constexpr (!scalar_traits<S>::is_complex) {
+      if (dcgs2) {
+        // delayed CGS2: finalize v_{i-1} (and Hessenberg column i-2, with its
+        // Givens update) from the dots of this sweep, then project w once
+        const int p = i - 1;
+        double* d = ws->dc_d.as<double>();
+        double* coef = ws->dc_coef.as<double>();
+        dcgs2_dots(ws, n, nrhs, p, V, ldv, w, d, ss);
+        k_dcgs2_fix<<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef);
+        dcgs2_update(ws, n, nrhs, p, V, ldv, vprev, w, coef, t2, ss);
+        k_dcgs2_norm<<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st);
+        k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w);
+        HIPCHK(hipGetLastError());
+        return;
+      }
+    }
+    if (o.orth == PGS_ORTH_CGS2 || o.orth == PGS_ORTH_DCGS2) {
       // PargemslrCgs2 (matrixops.cpp:10588-10709): two projection passes, the
       // norm of the result accumulated by the second update.  Three sweeps
       // over the basis: h = V^T w; w -= V h fused with h2 = V^T w; w -= V h2.
@@ -1172,6 +1249,7 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
     k_cycle_start<S><<<1, 64, 0, s>>>(nrhs, m, st, rs);
     k_scale_cols<S><<<nb, kNT, 0, s>>>(n, nrhs, st, V);
     int i = 0;
+    bool none_active = false;
     while (i < m && iters_host < maxits) {
       ++i;
       ++iters_host;
@@ -1201,7 +1279,21 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
         inner_step(i, s);
       }
       if (i % check == 0 || i == m) {
-        if (read_flag(false) == 0) break;
+        if (read_flag(false) == 0) {
+          none_active = true;
+          break;
+        }
+      }
+    }
+    if constexpr (!scalar_traits<S>::is_complex) {
+      if (dcgs2 && !none_active) {
+        // close the cycle: finalize the last basis vector and Hessenberg
+        // column of the columns still running (one more dot sweep)
+        double* d = ws->dc_d.as<double>();
+        dcgs2_dots(ws, n, nrhs, i, V, ldv, nullptr, d, s);
+        k_dcgs2_fix<<<1, 64, 0, s>>>(nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st,
+                                     ws->dc_coef.as<double>());
+        HIPCHK(hipGetLastError());
       }
     }
     // x += Z y
@@ -1237,7 +1329,8 @@ extern "C" int pgs_fgmres(pgs_dev_op_t dop, pgs_ws_t w, const double* b, double*
   pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
   if (ws->op != op || !pow2(nrhs) || nrhs > ws->max_nrhs) return PGS_ERROR_INVALID_PARAM;
   if (opts->kdim < 1 || opts->kdim > ws->kdim || opts->maxits < 0) return PGS_ERROR_INVALID_PARAM;
-  if (opts->orth != PGS_ORTH_MGS && opts->orth != PGS_ORTH_CGS2) return PGS_ERROR_INVALID_OPTION;
+  if (opts->orth != PGS_ORTH_MGS && opts->orth != PGS_ORTH_CGS2 && opts->orth != PGS_ORTH_DCGS2)
+    return PGS_ERROR_INVALID_OPTION;
   if (opts->use_precond && opts->prec != PGS_PREC_FP64 && opts->prec != PGS_PREC_FP32) return PGS_ERROR_INVALID_OPTION;
   const bool has64 = op->is_complex ? op->pz64.present : op->p64.present;
   const bool has32 = op->is_complex ? op->pz32.present : op->p32.present;

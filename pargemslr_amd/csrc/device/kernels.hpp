@@ -415,6 +415,311 @@ __global__ __launch_bounds__(NT) void k_basis_reg(int64_t n, int nrhs, int p, co
   }
 }
 
+// ---------------------------------------------------------------------------
+// Delayed CGS2 (PGS_ORTH_DCGS2, real bases): the re-orthogonalization of the
+// newest basis vector q is done in the next step's sweep, together with the
+// first projection of the next vector w, so a step reads the basis twice
+// instead of three times.  Basis slots 0..p-1 are final, q is tentative.
+// ---------------------------------------------------------------------------
+
+// Dots of the basis chunk V_{j0..j0+cnt-1} with q and with w (w may be null):
+// per workgroup partials part[blk * nout * nrhs + o * nrhs + c] for
+//   o < cnt:             V_{j0+o} . q
+//   cnt <= o < 2 cnt:    V_{j0+o-cnt} . w          (w != null)
+//   o = 2 cnt, 2 cnt+1:  q . q and q . w           (first chunk, w != null)
+//   o = cnt:             q . q                     (first chunk, w == null)
+// nout = 2 cnt + 2 (w != null, first), 2 cnt (w != null), cnt + 1 / cnt
+// (w == null, first / not).  LPE lanes share an element, lane `sub` holds
+// V_{j0 + u LPE + sub}, u < ceil(cnt / LPE) <= PM; every lane walks the
+// elements with a grid stride; reductions over lanes, then over waves in a
+// fixed order (one LDS table of nout * nrhs doubles, the waves add in turn).
+template <int NT, int PM, int LPE>
+__global__ __launch_bounds__(NT) void k_dcgs2_dots(int64_t n, int nrhs, int j0, int cnt, const double* __restrict__ V,
+                                                   int64_t ldv, const double* __restrict__ q,
+                                                   const double* __restrict__ w, int first,
+                                                   double* __restrict__ part) {
+  extern __shared__ double dd_sm[];
+  const bool hw = w != nullptr;
+  const int nout = (hw ? 2 * cnt : cnt) + (first ? (hw ? 2 : 1) : 0);
+  const int tid = threadIdx.x;
+  const int sub = tid % LPE;
+  const int c = (tid / LPE) % nrhs;
+  const int pu = (cnt + LPE - 1) / LPE;
+  const double* Vl = V + (int64_t)(j0 + sub) * ldv;
+  double aq[PM], aw[PM];
+  double aa = 0.0, ab = 0.0;
+#pragma unroll
+  for (int u = 0; u < PM; ++u) {
+    aq[u] = 0.0;
+    aw[u] = 0.0;
+  }
+  const int64_t total = n * nrhs;
+  const int64_t stride = (int64_t)gridDim.x * (NT / LPE);
+  for (int64_t g = ((int64_t)blockIdx.x * NT + tid) / LPE; g < total; g += stride) {
+    const double qv = q[g];
+    const double wv = hw ? w[g] : 0.0;
+    double v[PM];
+#pragma unroll
+    for (int u = 0; u < PM; ++u)
+      if (u < pu && u * LPE + sub < cnt) v[u] = Vl[(int64_t)(u * LPE) * ldv + g];
+      else v[u] = 0.0;
+#pragma unroll
+    for (int u = 0; u < PM; ++u)
+      if (u < pu) {
+        aq[u] += v[u] * qv;
+        aw[u] += v[u] * wv;
+      }
+    if (sub == 0) {
+      aa += qv * qv;
+      ab += qv * wv;
+    }
+  }
+  // reduce over the lanes of a column (lanes c LPE + sub keep the sums)
+#pragma unroll
+  for (int u = 0; u < PM; ++u) {
+    if (u < pu) {
+      for (int mm = kWave / 2; mm >= nrhs * LPE; mm >>= 1) {
+        aq[u] += __shfl_xor(aq[u], mm);
+        aw[u] += __shfl_xor(aw[u], mm);
+      }
+    }
+  }
+  for (int mm = kWave / 2; mm >= nrhs * LPE; mm >>= 1) {
+    aa += __shfl_xor(aa, mm);
+    ab += __shfl_xor(ab, mm);
+  }
+  for (int t = tid; t < nout * nrhs; t += NT) dd_sm[t] = 0.0;
+  __syncthreads();
+  // the waves add their sums to the table in turn (a fixed order)
+  const int wid = tid / kWave, lane = tid % kWave;
+  for (int k = 0; k < NT / kWave; ++k) {
+    if (wid == k && lane < nrhs * LPE) {
+#pragma unroll
+      for (int u = 0; u < PM; ++u) {
+        const int j = u * LPE + sub;
+        if (u < pu && j < cnt) {
+          dd_sm[j * nrhs + lane / LPE] += aq[u];
+          if (hw) dd_sm[(cnt + j) * nrhs + lane / LPE] += aw[u];
+        }
+      }
+      if (first && lane % LPE == 0) {
+        const int o = hw ? 2 * cnt : cnt;
+        dd_sm[o * nrhs + lane / LPE] += aa;
+        if (hw) dd_sm[(o + 1) * nrhs + lane / LPE] += ab;
+      }
+    }
+    __syncthreads();
+  }
+  for (int t = tid; t < nout * nrhs; t += NT) part[(int64_t)blockIdx.x * nout * nrhs + t] = dd_sm[t];
+}
+
+// q <- (q - sum_{j<p} V_j s_j) inv_nu (the delayed second projection and
+// normalization of q) and w <- w - sum_{j<p} V_j g_j - gamma q_old (the first
+// projection of w on the final basis and q), with per-workgroup partials of
+// |w|^2 in part[blk * nrhs + c].  coef = [s (p x nrhs) | g (p x nrhs) |
+// inv_nu (nrhs) | gamma (nrhs)], copied to LDS.  Element layout as
+// k_basis_reg (LPE lanes per element, lane sub holds V_{u LPE + sub}).
+template <int NT, int PM, int LPE>
+__global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p, const double* __restrict__ V,
+                                                     int64_t ldv, double* __restrict__ q, double* __restrict__ w,
+                                                     const double* __restrict__ coef, double* __restrict__ part) {
+  extern __shared__ double du_sm[];
+  const int nc = (2 * p + 2) * nrhs;
+  double* cs = du_sm;
+  double* red = du_sm + nc;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < nc; t += NT) cs[t] = coef[t];
+  __syncthreads();
+  const double* sc = cs;
+  const double* gc = cs + p * nrhs;
+  const double* inu = cs + 2 * p * nrhs;
+  const double* gam = inu + nrhs;
+  const int sub = tid % LPE;
+  const int c = (tid / LPE) % nrhs;
+  const int pu = (p + LPE - 1) / LPE;
+  const double* Vl = V + (int64_t)sub * ldv;
+  double acc = 0.0;
+  const int64_t total = n * nrhs;
+  const int64_t stride = (int64_t)gridDim.x * (NT / LPE);
+  for (int64_t g = ((int64_t)blockIdx.x * NT + tid) / LPE; g < total; g += stride) {
+    double v[PM];
+#pragma unroll
+    for (int u = 0; u < PM; ++u)
+      if (u < pu) v[u] = (u * LPE + sub < p) ? Vl[(int64_t)(u * LPE) * ldv + g] : 0.0;
+    double sv = 0.0, gv = 0.0;
+#pragma unroll
+    for (int u = 0; u < PM; ++u)
+      if (u < pu) {
+        const int j = u * LPE + sub;
+        const bool in = j < p;
+        sv += v[u] * (in ? sc[j * nrhs + c] : 0.0);
+        gv += v[u] * (in ? gc[j * nrhs + c] : 0.0);
+      }
+    if constexpr (LPE == 2) {
+      sv += __shfl_xor(sv, 1);
+      gv += __shfl_xor(gv, 1);
+    }
+    if (sub == 0) {
+      const double qo = q[g];
+      const double un = w[g] - gv - gam[c] * qo;
+      q[g] = (qo - sv) * inu[c];
+      w[g] = un;
+      acc += un * un;
+    }
+  }
+  const int wid = tid / kWave, lane = tid % kWave;
+  for (int mm = kWave / 2; mm >= nrhs * LPE; mm >>= 1) acc += __shfl_xor(acc, mm);
+  if (lane < nrhs * LPE && lane % LPE == 0) red[wid * nrhs + lane / LPE] = acc;
+  __syncthreads();
+  for (int t = tid; t < nrhs; t += NT) {
+    double s2 = 0.0;
+    for (int k = 0; k < NT / kWave; ++k) s2 += red[k * nrhs + t];
+    part[(int64_t)blockIdx.x * nrhs + t] = s2;
+  }
+}
+
+// Per column, after the dots of step p (q = basis slot p): finalize q's norm
+// nu = sqrt(q.q - |s|^2), correct the tentative Hessenberg column p-1
+// (H[0..p-1, p-1] += H[p, p-1] s; H[p, p-1] *= nu), then the Givens update of
+// column p-1 as k_givens (convergence test, cycle end); then (not final) the
+// coefficients of w on the final basis: H[0..p-1, p] = a, H[p, p] = c_p =
+// (q.w - s.a) / nu, and coef = [s | g = a - gamma s | 1/nu | gamma = c_p / nu]
+// for k_dcgs2_update.  p = 0: q is the exact v0 (nu = 1, no column to finalize).
+// Columns that are not active get zero coefficients.
+__global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, const double* __restrict__ d,
+                            double* __restrict__ H, double* __restrict__ cs, double* __restrict__ sn,
+                            double* __restrict__ rs, KState* __restrict__ st, double* __restrict__ coef) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  KState s = st[c];
+  double* sco = coef;
+  double* gco = coef + (int64_t)p * nrhs;
+  double* inu = coef + (int64_t)2 * p * nrhs;
+  double* gam = inu + nrhs;
+  auto zero = [&]() {
+    if (final_pass) return;
+    for (int j = 0; j < p; ++j) {
+      sco[j * nrhs + c] = 0.0;
+      gco[j * nrhs + c] = 0.0;
+    }
+    inu[c] = 0.0;
+    gam[c] = 0.0;
+  };
+  if (!s.active) {
+    zero();
+    return;
+  }
+  const double eps = 2.220446049250313e-16;
+  // d rows: s_j (j < p), then a_j (j < p), then q.q, q.w (final: s_j, q.q)
+  const double qq = final_pass ? d[p * nrhs + c] : d[2 * p * nrhs + c];
+  const double qw = final_pass ? 0.0 : d[(2 * p + 1) * nrhs + c];
+  double nu = 1.0;
+  double* Hc = H + (int64_t)c * (m + 1) * m;
+  if (p > 0) {
+    double ss = 0.0;
+    for (int j = 0; j < p; ++j) ss += d[j * nrhs + c] * d[j * nrhs + c];
+    nu = sqrt(fmax(qq - ss, 0.0));
+    const int col = p - 1;
+    const double hq = Hc[p + col * (m + 1)];
+    for (int j = 0; j < p; ++j) Hc[j + col * (m + 1)] += hq * d[j * nrhs + c];
+    Hc[p + col * (m + 1)] = hq * nu;
+    // Givens update of column col (fgmres.hpp:428-461)
+    double* csc = cs + (int64_t)c * m;
+    double* snc = sn + (int64_t)c * m;
+    double* rsc = rs + (int64_t)c * (m + 1);
+    s.iters += 1;
+    s.icur = p;
+    for (int j = 1; j < p; ++j) {
+      const double hii = Hc[(j - 1) + col * (m + 1)];
+      const double hj = Hc[j + col * (m + 1)];
+      Hc[(j - 1) + col * (m + 1)] = csc[j - 1] * hii + snc[j - 1] * hj;
+      Hc[j + col * (m + 1)] = -snc[j - 1] * hii + csc[j - 1] * hj;
+    }
+    const double hii = Hc[(p - 1) + col * (m + 1)];
+    const double hii1 = Hc[p + col * (m + 1)];
+    const double g = sqrt(hii * hii + hii1 * hii1);
+    if (fabs(g) < eps) {  // breakdown: the reference leaves x untouched (fgmres.hpp:440-443)
+      s.status = 2;
+      s.active = 0;
+      s.scale = 0.0;
+      st[c] = s;
+      zero();
+      return;
+    }
+    csc[p - 1] = hii / g;
+    snc[p - 1] = hii1 / g;
+    rsc[p] = -snc[p - 1] * rsc[p - 1];
+    rsc[p - 1] = csc[p - 1] * rsc[p - 1];
+    Hc[(p - 1) + col * (m + 1)] = csc[p - 1] * hii + snc[p - 1] * hii1;
+    s.normr = fabs(rsc[p]);
+    if (s.normr <= s.tolr) {
+      s.status = 1;
+      s.active = 0;
+    } else if (p >= m || s.iters >= maxits) {
+      s.active = 0;  // cycle ends for this column
+    }
+    st[c] = s;
+    if (!s.active) {
+      zero();
+      return;
+    }
+  }
+  if (final_pass) return;
+  const double in = nu > 0.0 ? 1.0 / nu : 0.0;
+  double sa = 0.0;
+  for (int j = 0; j < p; ++j) sa += d[j * nrhs + c] * d[(p + j) * nrhs + c];
+  const double cp = (qw - sa) * in;
+  const double gm = cp * in;
+  for (int j = 0; j < p; ++j) {
+    const double sj = d[j * nrhs + c], aj = d[(p + j) * nrhs + c];
+    Hc[j + p * (m + 1)] = aj;
+    sco[j * nrhs + c] = sj;
+    gco[j * nrhs + c] = aj - gm * sj;
+  }
+  Hc[p + p * (m + 1)] = cp;
+  inu[c] = in;
+  gam[c] = gm;
+}
+
+// Sums the nblk partials of one k_dcgs2_dots launch into the rows of the
+// step's dot table d (rows: s_j, then a_j, then q.q and q.w; or s_j, q.q
+// without w), one wave per output, in a fixed order.
+__global__ __launch_bounds__(256) void k_dcgs2_final(int nblk, int nout, int cnt, int nrhs, int hw, int p, int j0,
+                                                     const double* __restrict__ part, double* __restrict__ d) {
+  const int g = blockIdx.x * 4 + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int total = nout * nrhs;
+  if (g >= total) return;  // whole wave exits together
+  double s = 0.0;
+  for (int b = lane; b < nblk; b += kWave) s += part[(int64_t)b * total + g];
+#pragma unroll
+  for (int mm = kWave / 2; mm > 0; mm >>= 1) s += __shfl_xor(s, mm);
+  if (lane != 0) return;
+  const int o = g / nrhs, c = g % nrhs;
+  int row;
+  if (o < cnt) row = j0 + o;
+  else if (hw && o < 2 * cnt) row = p + j0 + (o - cnt);
+  else row = (hw ? 2 * p : p) + (o - (hw ? 2 * cnt : cnt));
+  d[row * nrhs + c] = s;
+}
+
+// After k_dcgs2_update of step p: the tentative H[p+1, p] = ||w|| and the
+// scale that normalizes w (zero for columns that are not active).
+__global__ void k_dcgs2_norm(int nrhs, int m, int p, const double* __restrict__ t2, double* __restrict__ H,
+                             KState* __restrict__ st) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  KState s = st[c];
+  if (!s.active) {
+    s.scale = 0.0;
+  } else {
+    const double t = sqrt(t2[c]);
+    H[(int64_t)c * (m + 1) * m + (p + 1) + p * (m + 1)] = t;
+    s.scale = t > 0.0 ? 1.0 / t : 0.0;
+  }
+  st[c] = s;
+}
+
 // w[r, c] -= sum_{j<p} V_j[r, c] * h[j, c]   (dynamic shared memory: p*nrhs doubles)
 __global__ void k_multi_axpy(int64_t n, int nrhs, int p, const double* __restrict__ V, int64_t ldv,
                              const double* __restrict__ h, double* __restrict__ w) {

@@ -82,7 +82,8 @@ def test_apply_multi_rhs_matches_columns(small, gpu):
 
 
 @pytest.mark.parametrize("which,orth", [("c1", pg.PGS_ORTH_MGS), ("c1", pg.PGS_ORTH_CGS2),
-                                        ("small", pg.PGS_ORTH_CGS2)])
+                                        ("small", pg.PGS_ORTH_CGS2), ("c1", pg.PGS_ORTH_DCGS2),
+                                        ("small", pg.PGS_ORTH_DCGS2)])
 def test_fgmres_fp64_parity(which, orth, request, gpu):
     A, G, D, W = request.getfixturevalue(which)
     kdim = 30 if which == "c1" else 50
@@ -196,27 +197,32 @@ def test_apply_wide_rows(gpu):
         assert np.linalg.norm(Z[:, c] - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize("nr", [4, 64])
-def test_fgmres_cgs2_long_basis(nr, c1, gpu):
-    """Unpreconditioned FGMRES(50) with CGS2: the basis grows past 32 and 64
-    vectors' worth of register sweeps; every column's Givens residual after 60
-    iterations equals the oracle's (GMRES residuals are unique)."""
+@pytest.mark.parametrize("nr,orth,kdim", [(4, pg.PGS_ORTH_CGS2, 50), (64, pg.PGS_ORTH_CGS2, 50),
+                                          (4, pg.PGS_ORTH_DCGS2, 50), (64, pg.PGS_ORTH_DCGS2, 50),
+                                          (8, pg.PGS_ORTH_DCGS2, 80)])
+def test_fgmres_cgs2_long_basis(nr, orth, kdim, c1, gpu):
+    """Unpreconditioned FGMRES(kdim) with CGS2 or delayed CGS2: the basis
+    grows past 32 and 64 vectors' worth of register sweeps (two lanes per
+    element, 64 columns with one lane); every column's Givens residual after
+    60 (or kdim + 10) iterations equals the oracle's (GMRES residuals are
+    unique)."""
     A = c1[0]
     D = pg.DevOp(c1[1])
-    W = pg.Workspace(D, max_nrhs=nr, kdim=50)
+    W = pg.Workspace(D, max_nrhs=nr, kdim=kdim)
+    its = 60 if kdim <= 50 else kdim + 10
     B = np.stack([pg.rng_normal(80 + c, A.n) for c in range(nr)], axis=1)
     Xd = _dev(np.zeros((A.n, nr)), gpu)
-    res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=50, maxits=60, tol=1e-12, use_precond=False,
-                   orth=pg.PGS_ORTH_CGS2)
+    res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=kdim, maxits=its, tol=1e-12, use_precond=False, orth=orth)
     for c in (0, 1, nr - 1):
-        _, st, it_o, rr_o, _ = orc.fgmres(A, B[:, c], prec=None, kdim=50, maxits=60, tol=1e-12, orth=1)
+        _, st, it_o, rr_o, _ = orc.fgmres(A, B[:, c], prec=None, kdim=kdim, maxits=its, tol=1e-12, orth=1)
         st_g, it_g, rr_g = res[c]
-        assert it_g == it_o == 60
+        assert it_g == it_o == its
         assert rr_g == pytest.approx(rr_o, rel=1e-6), c
 
 
-@pytest.mark.parametrize("prec", [pg.PGS_PREC_FP64, pg.PGS_PREC_FP32])
-def test_fgmres_graph_replay_matches_direct(prec, small, gpu):
+@pytest.mark.parametrize("prec,orth", [(pg.PGS_PREC_FP64, pg.PGS_ORTH_CGS2), (pg.PGS_PREC_FP32, pg.PGS_ORTH_CGS2),
+                                       (pg.PGS_PREC_FP32, pg.PGS_ORTH_DCGS2)])
+def test_fgmres_graph_replay_matches_direct(prec, orth, small, gpu):
     """Inner steps captured as HIP graphs and replayed (FGMRES(10): every
     graph is replayed once per restart cycle, and the whole solve twice with
     direct-launch solves in between) give the same bits as direct launches."""
@@ -226,9 +232,53 @@ def test_fgmres_graph_replay_matches_direct(prec, small, gpu):
     outs = []
     for use_graph in (False, True, False, True):
         Xd = _dev(np.zeros((A.n, nr)), gpu)
-        res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=10, maxits=500, tol=1e-8, prec=prec, use_graph=use_graph)
+        res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=10, maxits=500, tol=1e-8, prec=prec, use_graph=use_graph,
+                       orth=orth)
         outs.append((_host(Xd), [r[:2] for r in res]))
     for X, res in outs[1:]:
         assert res == outs[0][1]
         assert np.array_equal(X, outs[0][0])
     assert all(st == 0 for st, _ in outs[0][1]) and max(it for _, it in outs[0][1]) > 10
+
+
+def test_fgmres_dcgs2_batched_columns_match_single(small, gpu):
+    """Delayed CGS2, mixed form, 4 right-hand sides (one zero): every column
+    reaches the tolerance and matches its single solve; iterations within one
+    of the oracle's (modified Gram-Schmidt)."""
+    A, G, D, W = small
+    nr = 4
+    B = np.stack([pg.rng_normal(100 + c, A.n) for c in range(nr)], axis=1)
+    B[:, 3] = 0.0
+    Xd = _dev(np.zeros((A.n, nr)), gpu)
+    res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=50, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP32,
+                   orth=pg.PGS_ORTH_DCGS2)
+    X = _host(Xd)
+    O = P.oracle_of(G)
+    for c in range(nr):
+        st, it, rr = res[c]
+        assert st == 0
+        if c == 3:
+            assert it == 0 and not X[:, c].any()
+            continue
+        assert np.linalg.norm(B[:, c] - A.matvec(X[:, c])) <= 1e-8 * np.linalg.norm(B[:, c])
+        _, _, it_o, _, _ = orc.fgmres(A, B[:, c], prec=O, kdim=50, maxits=500, tol=1e-8, orth=0)
+        assert abs(it - it_o) <= 1, (c, it, it_o)
+        x1 = _dev(np.zeros(A.n), gpu)
+        (st1, it1, _), = W.fgmres(_dev(B[:, c], gpu), x1, kdim=50, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP32,
+                                  orth=pg.PGS_ORTH_DCGS2)
+        assert st1 == 0 and abs(it1 - it) <= 1
+        assert np.linalg.norm(X[:, c] - _host(x1)) <= 1e-6 * np.linalg.norm(X[:, c])
+
+
+def test_fgmres_dcgs2_restarts(c1, gpu):
+    """Delayed CGS2 with FGMRES(10): several restart cycles, each closed by
+    the final dot pass; iterations within one of the oracle's."""
+    A, G, D, W = c1
+    b = pg.rng_normal(110, A.n)
+    _, st, it_o, _, _ = orc.fgmres(A, b, prec=P.oracle_of(G), kdim=10, maxits=500, tol=1e-10, orth=0)
+    xd = _dev(np.zeros(A.n), gpu)
+    (status, it, rr), = W.fgmres(_dev(b, gpu), xd, kdim=10, maxits=500, tol=1e-10, prec=pg.PGS_PREC_FP64,
+                                 orth=pg.PGS_ORTH_DCGS2)
+    x = _host(xd)
+    assert status == 0 and it > 20 and abs(it - it_o) <= 1, (it, it_o)
+    assert np.linalg.norm(b - A.matvec(x)) <= 1e-10 * np.linalg.norm(b)
