@@ -271,7 +271,7 @@ void launch_spmv(const DevCsr& M, const VX* x, VY* y, const VY* r, VACC alpha, V
   if (M.nrows == 0) return;
   if (nrhs == 1) {
     const int64_t threads = (int64_t)M.nrows * M.lpr;
-    const unsigned g = nblocks_for(threads, kNT);
+    const unsigned g = (nblocks_for(threads, kNT) + kXcds - 1) / kXcds * kXcds;  // see k_spmvN
 #define PGS_SPMV_CASE(L) \
   case L: k_spmv1<VA, VX, VY, VACC, L><<<g, kNT, 0, s>>>(M.nrows, M.ia.as<int>(), M.ja.as<int>(), M.a.as<VA>(), x, y, r, alpha, beta, mode); break;
     switch (M.lpr) {
@@ -285,8 +285,11 @@ void launch_spmv(const DevCsr& M, const VX* x, VY* y, const VY* r, VACC alpha, V
 #undef PGS_SPMV_CASE
   } else {
     const int64_t threads = (int64_t)M.nrows * nrhs;
-    k_spmvN<VA, VX, VY, VACC><<<nblocks_for(threads, kNT), kNT, 0, s>>>(M.nrows, nrhs, M.ia.as<int>(), M.ja.as<int>(),
-                                                                        M.a.as<VA>(), x, y, r, alpha, beta, mode);
+    // a multiple of the XCD count (the SpMV kernels renumber workgroups per
+    // XCD; the padding workgroups find no rows)
+    const unsigned g = (nblocks_for(threads, kNT) + kXcds - 1) / kXcds * kXcds;
+    k_spmvN<VA, VX, VY, VACC><<<g, kNT, 0, s>>>(M.nrows, nrhs, M.ia.as<int>(), M.ja.as<int>(), M.a.as<VA>(), x, y, r,
+                                                alpha, beta, mode);
   }
   HIPCHK(hipGetLastError());
 }

@@ -62,11 +62,23 @@ __global__ void k_convert(int64_t m, const VI* __restrict__ in, VO* __restrict__
 // Single RHS: LPR lanes per row, entries strided over the lanes, DPP/shuffle
 // reduction.  Multi RHS: one thread per (row, column) over the row's entries.
 // ---------------------------------------------------------------------------
+// Workgroups are dealt to the 8 XCDs round-robin; the SpMV kernels renumber
+// them so that XCD x takes the contiguous range [x * per, (x + 1) * per) of
+// logical workgroups (per = gridDim.x / 8, the grid padded to a multiple of
+// 8): the rows of one XCD are contiguous, and the x values their neighbours
+// read stay in that XCD's L2.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t per = gridDim.x / kXcds;
+  return (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
 template <typename VA, typename VX, typename VY, typename VACC, int LPR>
 __global__ void k_spmv1(int nrows, const int* __restrict__ ia, const int* __restrict__ ja, const VA* __restrict__ a,
                         const VX* __restrict__ x, VY* __restrict__ y, const VY* __restrict__ r, VACC alpha, VACC beta,
                         int mode) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t g = xcd_block() * blockDim.x + threadIdx.x;
   int row = (int)(g / LPR);
   int q = (int)(g % LPR);
   VACC acc = 0;
@@ -87,13 +99,30 @@ template <typename VA, typename VX, typename VY, typename VACC>
 __global__ void k_spmvN(int nrows, int nrhs, const int* __restrict__ ia, const int* __restrict__ ja,
                         const VA* __restrict__ a, const VX* __restrict__ x, VY* __restrict__ y,
                         const VY* __restrict__ r, VACC alpha, VACC beta, int mode) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= (int64_t)nrows * nrhs) return;
   int row = (int)(t / nrhs);
   int c = (int)(t - (int64_t)row * nrhs);
   VACC acc = 0;
-  int k1 = ia[row + 1];
-  for (int k = ia[row]; k < k1; ++k) acc += (VACC)a[k] * (VACC)x[(int64_t)ja[k] * nrhs + c];
+  const int k1 = ia[row + 1];
+  // eight entries at a time: their indices and values, then their x gathers,
+  // are all in flight before the first multiply-add (summed in entry order)
+  for (int k = ia[row]; k < k1; k += 8) {
+    int jj[8];
+    VACC av[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const bool in = k + u < k1;
+      jj[u] = in ? ja[k + u] : 0;
+      av[u] = in ? (VACC)a[k + u] : VACC(0);
+    }
+    VACC xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = k + u < k1 ? (VACC)x[(int64_t)jj[u] * nrhs + c] : VACC(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k + u < k1) acc += av[u] * xv[u];
+  }
   if (mode == 1) y[t] = cvt<VY>((VACC)r[t] - acc);
   else if (beta == (VACC)0) y[t] = cvt<VY>(alpha * acc);
   else y[t] = cvt<VY>(alpha * acc + beta * (VACC)y[t]);
