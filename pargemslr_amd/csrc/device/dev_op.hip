@@ -184,7 +184,9 @@ struct pgs_dev_op_s {
   int is_complex = 0;
   int lds_max = 65536;
   int trsv_variant = 0;  // 0: pipelined kernels (ring when packed for it); 1: basic level-set kernel
-  int trsv_cg = 4;       // most right-hand-side columns per workgroup of the ring kernel
+  int trsv_cg = 4;       // most right-hand-side columns per column group of the ring kernel
+  int trsv_groups = 0;   // column groups per workgroup of the 128-thread ring kernel: 0 choose, 1, 2
+  int num_cus = 256;     // compute units of the device
   int trsv_nt = 128;     // threads of the pipelined kernel
   int trsv_lpr = 4;      // lanes per row (steps hold at most trsv_nt / trsv_lpr rows)
   int ring_nt = 128;     // ring kernel: threads (64: one wave, no barrier per step)
@@ -368,21 +370,25 @@ void launch_trsv_pipe(int nt, int lpr, const DevTrsv& T, const V* rhs, V* x, int
 // lanes (32 rows) per row.
 constexpr int kRingWidth = 20;
 
-// stage depth of the ring kernel's register ring: 8 for float, 6 for double
-// and complex<float>, 4 for complex<double> (registers per stage grow with
-// the value size)
-template <typename V>
-constexpr int ring_depth() { return sizeof(V) == 4 ? 8 : sizeof(V) == 8 ? 6 : 4; }
+// stage depth of the ring kernel's register ring (steps whose entries are in
+// flight), measured at 128^3 level 0: float 20 for one column per group, 12
+// for 2-4 columns (more registers per stage); double and complex<float> 6 /
+// 10; complex<double> 4.  The kernel keeps one wave per SIMD, so deep rings
+// may use the whole register file (the compiler moves the excess to AGPRs).
+template <typename V, int CG>
+constexpr int ring_depth() {
+  return sizeof(V) == 4 ? (CG == 1 ? 20 : 12) : sizeof(V) == 8 ? (CG == 1 ? 6 : 10) : 4;
+}
 
-template <typename V, int NT, int LPR, int CG>
+template <typename V, int NT, int LPR, int CG, int G = 1>
 void launch_trsv_ring_cg(const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
-  constexpr int D = ring_depth<V>();
-  auto kern = k_trsv_ring<V, NT, LPR, kRingWidth / LPR, D, CG>;  // LDS limit raised in set_kernel_attrs
-  const size_t lds = (size_t)(T.window + 1) * CG * sizeof(V);  // ring + one dummy slot
-  const unsigned grid = (unsigned)T.nblocks * (unsigned)(nrhs / CG);
-  kern<<<grid, NT, lds, s>>>(T.nblocks, T.window - 1, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
-                             T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), T.desc.as<int4>(), T.ent.as<TPair<V>>(),
-                             T.ri.as<TPair<V>>(), rhs, x, nrhs);
+  constexpr int D = ring_depth<V, CG>();
+  auto kern = k_trsv_ring<V, NT, LPR, kRingWidth / LPR, D, CG, G>;  // LDS limit raised in set_kernel_attrs
+  const size_t lds = (size_t)(T.window + 1) * CG * G * sizeof(V);  // a ring + dummy slot per group
+  const unsigned grid = (unsigned)T.nblocks * (unsigned)(nrhs / (CG * G));
+  kern<<<grid, NT * G, lds, s>>>(T.nblocks, T.window - 1, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
+                                 T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), T.desc.as<int4>(),
+                                 T.ent.as<TPair<V>>(), T.ri.as<TPair<V>>(), rhs, x, nrhs);
 }
 
 template <typename V>
@@ -404,7 +410,13 @@ void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
     else if (cg == 2) launch_trsv_ring_cg<V, 64, 4, 2>(T, rhs, x, nrhs, s);
     else launch_trsv_ring_cg<V, 64, 4, 1>(T, rhs, x, nrhs, s);
   } else {
-    if (cg >= 4) launch_trsv_ring_cg<V, 128, 4, 4>(T, rhs, x, nrhs, s);
+    // two 4-column groups per workgroup when both rings fit in LDS and the
+    // paired grid still gives every CU a workgroup
+    const bool pair = cg >= 4 && op->trsv_groups != 1 && nrhs >= 8 &&
+                      (op->trsv_groups == 2 || (int64_t)T.nblocks * (nrhs / 8) >= op->num_cus) &&
+                      (size_t)(T.window + 1) * 8 * sizeof(V) <= (size_t)op->lds_max;
+    if (pair) launch_trsv_ring_cg<V, 128, 4, 4, 2>(T, rhs, x, nrhs, s);
+    else if (cg >= 4) launch_trsv_ring_cg<V, 128, 4, 4>(T, rhs, x, nrhs, s);
     else if (cg == 2) launch_trsv_ring_cg<V, 128, 4, 2>(T, rhs, x, nrhs, s);
     else launch_trsv_ring_cg<V, 128, 4, 1>(T, rhs, x, nrhs, s);
   }
@@ -443,21 +455,23 @@ void set_lds_limit(K kern, int lds_max) {
 
 template <typename V>
 void set_trsv_attrs(int lds_max) {
-  constexpr int D = ring_depth<V>();
-  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D, 2>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D, 4>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D, 2>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D, 4>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, D, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, D, 2>, lds_max);
+  constexpr int D1 = ring_depth<V, 1>(), D2 = ring_depth<V, 2>(), D4 = ring_depth<V, 4>();
+  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D1, 1, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D2, 2, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D4, 4, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D4, 4, 2>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D1, 1, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D2, 2, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D4, 4, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, D1, 1, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, D2, 2, 1>, lds_max);
   if constexpr (scalar_traits<V>::is_complex) return;  // no pipelined kernel for complex values
-  constexpr int D2 = sizeof(V) == 4 ? 5 : 4;
-  set_lds_limit(k_trsv_pipe<V, true, 64, 2, 10, D2>, lds_max);
-  set_lds_limit(k_trsv_pipe<V, true, 128, 2, 10, D2>, lds_max);
-  set_lds_limit(k_trsv_pipe<V, true, 128, 4, 5, D>, lds_max);
-  set_lds_limit(k_trsv_pipe<V, true, 256, 4, 5, D>, lds_max);
+  constexpr int DL2 = sizeof(V) == 4 ? 5 : 4;  // the pipelined kernel's stage depths (launch_trsv_pipe_nt)
+  constexpr int DP = sizeof(V) == 4 ? 8 : 6;
+  set_lds_limit(k_trsv_pipe<V, true, 64, 2, 10, DL2>, lds_max);
+  set_lds_limit(k_trsv_pipe<V, true, 128, 2, 10, DL2>, lds_max);
+  set_lds_limit(k_trsv_pipe<V, true, 128, 4, 5, DP>, lds_max);
+  set_lds_limit(k_trsv_pipe<V, true, 256, 4, 5, DP>, lds_max);
 }
 
 template <int PM>
@@ -782,6 +796,7 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     op->lds_max = lds;
+    HIPCHK(hipDeviceGetAttribute(&op->num_cus, hipDeviceAttributeMultiprocessorCount, device));
     set_kernel_attrs(lds);
     if (const char* v = std::getenv("PGS_TRSV_VARIANT")) op->trsv_variant = std::atoi(v);
     if (const char* v = std::getenv("PGS_TRSV_NT")) {
@@ -799,6 +814,10 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
       if (!std::strcmp(v, "64x4")) { op->ring_nt = 64; op->ring_lpr = 4; }
       else if (!std::strcmp(v, "64x2")) { op->ring_nt = 64; op->ring_lpr = 2; }
       else if (!std::strcmp(v, "128x4")) { op->ring_nt = 128; op->ring_lpr = 4; }
+    }
+    if (const char* v = std::getenv("PGS_TRSV_GROUPS")) {
+      const int g = std::atoi(v);
+      if (g == 1 || g == 2) op->trsv_groups = g;
     }
     if (const char* v = std::getenv("PGS_TRSV_CG")) {
       const int cg = std::atoi(v);

@@ -392,7 +392,7 @@ __device__ __forceinline__ void bstore_val(__amdgpu_buffer_rsrc_t r, int off, cp
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, off, 0, 0);
 }
 
-template <typename V, int NT, int LPR, int EPL, int D, int CG, bool UPPER>
+template <typename V, int NT, int LPR, int EPL, int D, int CG, bool UPPER, int G>
 __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int wmask, int nrhs, int c0,
                                            __amdgpu_buffer_rsrc_t rs_desc, __amdgpu_buffer_rsrc_t rs_ri,
                                            __amdgpu_buffer_rsrc_t rs_ent, __amdgpu_buffer_rsrc_t rs_src, V* xs,
@@ -404,7 +404,7 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   static_assert((LPR == 4 && (CG == 1 || CG == 2 || CG == 4)) || (LPR == 2 && (CG == 1 || CG == 2)),
                 "quad lanes with 1, 2 or 4 columns, or lane pairs with 1 or 2 columns");
   constexpr int NCL = 1;  // columns finalized per lane
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x % NT;  // thread within this column group
   const int q = tid % LPR;
   const int t0 = tid / LPR;
   // after the reduction lane q holds column q % CG; lanes q < CG write it
@@ -509,7 +509,7 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
       // the buffer's range (dropped): no branch
       xs[wr ? slot * CG + cl[0] : dummy] = v;
       bstore_val(rs_dst, wr ? (row * nrhs + c0 + cl[0]) * VS : 0x7ffffff0, v);
-      if constexpr (NT == 64) {
+      if constexpr (NT == 64 && G == 1) {
         // one wave: its LDS operations complete in issue order, so the next
         // step's gathers see this step's writes without a wait or barrier;
         // only the compiler must keep the order
@@ -524,17 +524,23 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
 #undef PGS_RING_RHS
 }
 
-template <typename V, int NT, int LPR, int EPL, int D, int CG>
-__global__ __launch_bounds__(NT) void k_trsv_ring(int nblocks, int wmask, const int* __restrict__ blk_row0,
+// G column groups of CG columns each share a workgroup of G * NT threads: the
+// groups walk the same steps in lockstep (one barrier per step), so their
+// identical loads of the block's entries and row pairs are served once by the
+// CU's L1 instead of each group streaming them from L2; each group has its own
+// ring in LDS.
+template <typename V, int NT, int LPR, int EPL, int D, int CG, int G>
+__global__ __launch_bounds__(NT * G) void k_trsv_ring(int nblocks, int wmask, const int* __restrict__ blk_row0,
                                                   const int* __restrict__ blk_step, const int* __restrict__ blk_stepU,
                                                   const int64_t* __restrict__ blk_ent0,
                                                   const int64_t* __restrict__ blk_ri0, const int4* __restrict__ desc,
                                                   const TPair<V>* __restrict__ ent, const TPair<V>* __restrict__ ri,
                                                   const V* __restrict__ rhs, V* __restrict__ x, int nrhs) {
   extern __shared__ unsigned char smem_raw[];
-  V* xs = reinterpret_cast<V*>(smem_raw);
+  const int grp = threadIdx.x / NT;
+  V* xs = reinterpret_cast<V*>(smem_raw) + grp * (wmask + 2) * CG;
   const int b = blockIdx.x % nblocks;
-  const int c0 = (blockIdx.x / nblocks) * CG;
+  const int c0 = (blockIdx.x / nblocks) * CG * G + grp * CG;
   const int r0 = blk_row0[b], nb = blk_row0[b + 1] - r0;
   const int sbeg = blk_step[b], send = blk_step[b + 1], sU = blk_stepU[b];
   const int64_t e0 = blk_ent0[b], e1 = blk_ent0[b + 1];
@@ -544,12 +550,14 @@ __global__ __launch_bounds__(NT) void k_trsv_ring(int nblocks, int wmask, const 
   const __amdgpu_buffer_rsrc_t rs_desc = make_rsrc(desc + sbeg, (int64_t)(send - sbeg) * 16);
   const int64_t vbytes = (int64_t)nb * nrhs * (int64_t)sizeof(V);
   V* xb = x + (int64_t)r0 * nrhs;
-  for (int i = threadIdx.x; i < (wmask + 2) * CG; i += NT) xs[i] = 0;  // ring + dummy slot
+  V* xall = reinterpret_cast<V*>(smem_raw);
+  for (int i = threadIdx.x; i < (wmask + 2) * CG * G; i += NT * G) xall[i] = 0;  // rings + dummy slots
   __syncthreads();
-  ring_sweep<V, NT, LPR, EPL, D, CG, false>(sbeg, sU, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
-                                            make_rsrc(rhs + (int64_t)r0 * nrhs, vbytes), xs, make_rsrc(xb, vbytes));
-  ring_sweep<V, NT, LPR, EPL, D, CG, true>(sU, send, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
-                                           make_rsrc(xb, vbytes), xs, make_rsrc(xb, vbytes));
+  ring_sweep<V, NT, LPR, EPL, D, CG, false, G>(sbeg, sU, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
+                                               make_rsrc(rhs + (int64_t)r0 * nrhs, vbytes), xs,
+                                               make_rsrc(xb, vbytes));
+  ring_sweep<V, NT, LPR, EPL, D, CG, true, G>(sU, send, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
+                                              make_rsrc(xb, vbytes), xs, make_rsrc(xb, vbytes));
 }
 
 }  // namespace dev
