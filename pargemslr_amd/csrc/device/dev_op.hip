@@ -210,7 +210,7 @@ struct pgs_ws_s {
   DevMem r0, x0, lrpart, lrt;
   std::vector<std::unique_ptr<DevMem>> zlev;
   // Krylov
-  DevMem V, Z, w, part, hbuf, h2buf, t2, H, cs, sn, rs, coef, upd, st, zflag, nb2, nr2, flag;
+  DevMem V, Z, part, hbuf, h2buf, t2, H, cs, sn, rs, coef, upd, st, zflag, nb2, nr2, flag;
   DevMem dc_d, dc_coef;  // delayed CGS2: the step's dot table and update coefficients
   // inner Schur iteration: basis, preconditioned basis, temporaries (vectors
   // in the apply's precision) and its Hessenberg / Givens state per column

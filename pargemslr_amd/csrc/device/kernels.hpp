@@ -49,11 +49,6 @@ __global__ void k_scatter(int64_t n, int nrhs, const int* __restrict__ perm, con
   out[(int64_t)perm[i] * nrhs + c] = cvt<VO>(in[t]);
 }
 
-template <typename VI, typename VO>
-__global__ void k_convert(int64_t m, const VI* __restrict__ in, VO* __restrict__ out) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < m) out[t] = cvt<VO>(in[t]);
-}
 
 // ---------------------------------------------------------------------------
 // CSR SpMV (CsrMatrixPMatVecHostTemplate, ParGeMSLR/SRC/matrices/
@@ -290,16 +285,6 @@ __global__ __launch_bounds__(NT) void k_colnorm2(int64_t n, int nrhs, const S* _
     for (int q = 0; q < NT / kWave; ++q) s += sh[q * nrhs + tid];
     part[(int64_t)blockIdx.x * nrhs + tid] = s;
   }
-}
-
-// out[j*nrhs + c] = sum over blocks of the partials (+ out if accumulate)
-__global__ void k_dot_final(int nblk, int p, int nrhs, const double* __restrict__ part, double* __restrict__ out,
-                            int accumulate) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= p * nrhs) return;
-  double s = 0;
-  for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * p * nrhs + g];
-  out[g] = accumulate ? out[g] + s : s;
 }
 
 // out[g] = sum over the nblk partials of output g (one wave per output; the
@@ -747,20 +732,6 @@ __global__ void k_dcgs2_norm(int nrhs, int m, int p, const double* __restrict__ 
     s.scale = t > 0.0 ? 1.0 / t : 0.0;
   }
   st[c] = s;
-}
-
-// w[r, c] -= sum_{j<p} V_j[r, c] * h[j, c]   (dynamic shared memory: p*nrhs doubles)
-__global__ void k_multi_axpy(int64_t n, int nrhs, int p, const double* __restrict__ V, int64_t ldv,
-                             const double* __restrict__ h, double* __restrict__ w) {
-  extern __shared__ double hs[];
-  for (int t = threadIdx.x; t < p * nrhs; t += blockDim.x) hs[t] = h[t];
-  __syncthreads();
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n * nrhs) return;
-  const int c = (int)(g % nrhs);
-  double acc = w[g];
-  for (int j = 0; j < p; ++j) acc -= V[j * ldv + g] * hs[j * nrhs + c];
-  w[g] = acc;
 }
 
 // v[r, c] *= s[c]   (s from KState.scale)
