@@ -185,7 +185,8 @@ struct pgs_dev_op_s {
   int lds_max = 65536;
   int trsv_variant = 0;  // 0: pipelined kernels (ring when packed for it); 1: basic level-set kernel
   int trsv_cg = 4;       // most right-hand-side columns per column group of the ring kernel
-  int trsv_groups = 0;   // column groups per workgroup of the 128-thread ring kernel: 0 choose, 1, 2
+  int trsv_groups = 1;   // column groups per workgroup of the 128-thread ring kernel: 1, 2, or 0 = 2 when
+                         // the paired grid fills the device (measured slower at 128^3: kept selectable)
   int num_cus = 256;     // compute units of the device
   int trsv_nt = 128;     // threads of the pipelined kernel
   int trsv_lpr = 4;      // lanes per row (steps hold at most trsv_nt / trsv_lpr rows)
@@ -817,7 +818,7 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
     }
     if (const char* v = std::getenv("PGS_TRSV_GROUPS")) {
       const int g = std::atoi(v);
-      if (g == 1 || g == 2) op->trsv_groups = g;
+      if (g >= 0 && g <= 2) op->trsv_groups = g;
     }
     if (const char* v = std::getenv("PGS_TRSV_CG")) {
       const int cg = std::atoi(v);
