@@ -761,6 +761,90 @@ void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask) {
   if (prec_mask & PGS_PREC_FP32) build(prec_of<V32>(op), V32(0));
 }
 
+// Byte sizes of every buffer of a workspace (nr columns, a power of two; Krylov
+// dimension kdim): one list serves the allocation and pgs_ws_estimate.
+struct WsSizes {
+  size_t vb = 0;  // one vector block of the operator's values
+  int part_blocks = 0;
+  std::vector<size_t> zlev;
+  std::vector<std::pair<DevMem pgs_ws_s::*, size_t>> bufs;
+  size_t total() const {
+    size_t t = 0;
+    for (size_t b : zlev) t += b;
+    for (const auto& e : bufs) t += e.second;
+    return t;
+  }
+};
+
+WsSizes ws_sizes(const pgs_dev_op_s* op, int nr, int kdim) {
+  WsSizes z;
+  const int64_t n = op->n;
+  const size_t es = vec_es(op);  // bytes of one value of the operator (double or complex<double>)
+  const size_t vb = (size_t)n * nr * es;
+  z.vb = vb;
+  for (int l = 0; l < op->nlev; ++l) z.zlev.push_back((size_t)(n - op->lev_ptr[l]) * nr * es);
+  int maxrank = 0;
+  for (int l = 0; l + 1 < op->nlev; ++l) {
+    const auto& lv = op->p64.present ? op->p64.levels : op->p32.present ? op->p32.levels
+                     : op->pz64.present ? op->pz64.levels : op->pz32.levels;
+    maxrank = std::max(maxrank, lv[l]->rank);
+  }
+  // dot products use at most 1024 workgroups (four per CU); the partials
+  // are summed by one wave per output
+  const int64_t rows_per_blk = std::max<int64_t>(256, (n + 1023) / 1024);
+  z.part_blocks = (int)std::max<int64_t>(1, (n + rows_per_blk - 1) / rows_per_blk);
+  const size_t ik = (size_t)kdim;
+  auto add = [&](DevMem pgs_ws_s::*m, size_t b) { z.bufs.push_back({m, b}); };
+  add(&pgs_ws_s::r0, vb);
+  add(&pgs_ws_s::x0, vb);
+  add(&pgs_ws_s::lrpart, (size_t)std::max(1, (kLrBlocks + 1) * maxrank * nr) * es);
+  add(&pgs_ws_s::lrt, (size_t)std::max(1, maxrank * nr) * es);
+  add(&pgs_ws_s::V, (ik + 1) * vb);
+  add(&pgs_ws_s::Z, ik * vb);
+  // partial rows per workgroup: kdim + 1 dots, or 2 * 32 + 2 of a delayed-CGS2 dot chunk
+  add(&pgs_ws_s::part, (size_t)z.part_blocks * std::max(kdim + 1, kPartRowsDcgs2) * nr * es);
+  add(&pgs_ws_s::hbuf, (ik + 1) * nr * es);
+  add(&pgs_ws_s::h2buf, (ik + 1) * nr * es);
+  add(&pgs_ws_s::t2, (size_t)nr * sizeof(double));
+  add(&pgs_ws_s::H, (size_t)nr * (ik + 1) * ik * es);
+  add(&pgs_ws_s::cs, (size_t)nr * ik * es);
+  add(&pgs_ws_s::sn, (size_t)nr * ik * es);
+  add(&pgs_ws_s::rs, (size_t)nr * (ik + 1) * es);
+  add(&pgs_ws_s::coef, (ik + 1) * nr * es);
+  add(&pgs_ws_s::upd, (size_t)nr * sizeof(int));
+  add(&pgs_ws_s::dc_d, (2 * ik + 4) * nr * sizeof(double));
+  add(&pgs_ws_s::dc_coef, (2 * ik + 4) * nr * sizeof(double));
+  add(&pgs_ws_s::st, (size_t)nr * sizeof(KState));
+  add(&pgs_ws_s::zflag, (size_t)nr * sizeof(double));
+  add(&pgs_ws_s::nb2, (size_t)nr * sizeof(double));
+  add(&pgs_ws_s::nr2, (size_t)nr * sizeof(double));
+  add(&pgs_ws_s::flag, 16);
+  if (op->inner_maxits > 0) {
+    // inner Schur iteration: basis, preconditioned basis and temporaries of
+    // the level-0 remainder (m0) and of level 0 (nl0); its scalars are
+    // double or complex<double>, as the operator's values
+    const size_t K = (size_t)op->inner_maxits;
+    const size_t m0 = (size_t)(n - op->lev_ptr[1]), nl0 = (size_t)std::max(1, op->lev_ptr[1]);
+    add(&pgs_ws_s::in_V, (K + 1) * m0 * nr * es);
+    add(&pgs_ws_s::in_Z, K * m0 * nr * es);
+    add(&pgs_ws_s::in_T, m0 * nr * es);
+    add(&pgs_ws_s::in_U, nl0 * nr * es);
+    add(&pgs_ws_s::in_Y, nl0 * nr * es);
+    add(&pgs_ws_s::in_h, K * nr * es);
+    add(&pgs_ws_s::in_t2, (size_t)nr * sizeof(double));
+    add(&pgs_ws_s::in_H, (size_t)nr * (K + 1) * K * es);
+    add(&pgs_ws_s::in_cs, (size_t)nr * K * es);
+    add(&pgs_ws_s::in_sn, (size_t)nr * K * es);
+    add(&pgs_ws_s::in_rs, (size_t)nr * (K + 1) * es);
+    add(&pgs_ws_s::in_coef, K * nr * es);
+    add(&pgs_ws_s::in_upd, (size_t)nr * sizeof(int));
+    add(&pgs_ws_s::in_st, (size_t)nr * sizeof(KState));
+    add(&pgs_ws_s::in_flag, (size_t)nr * sizeof(double));
+    add(&pgs_ws_s::in_nb2, (size_t)nr * sizeof(double));
+  }
+  return z;
+}
+
 int set_err(const HipError& e) {
   g_last_hip_error = hipGetErrorString(e.e);
   if (e.e == hipErrorOutOfMemory) return PGS_ERR_OOM;
@@ -866,18 +950,7 @@ int pgs_ws_estimate(pgs_dev_op_t dop, int max_nrhs, int kdim, int64_t* bytes) {
   const pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
   int nr = 1;
   while (nr < max_nrhs) nr <<= 1;
-  const int64_t es = (int64_t)vec_es(op);
-  const int64_t n = op->n, vb = n * nr * es;
-  int64_t b = 2 * vb + (2 * (int64_t)kdim + 1) * vb;  // r0, x0, V, Z
-  for (int l = 0; l < op->nlev; ++l) b += (n - op->lev_ptr[l]) * nr * es;
-  b += (int64_t)1025 * std::max(kdim + 1, kPartRowsDcgs2) * nr * es;  // reduction partials
-  b += (int64_t)(kLrBlocks + 2) * 64 * nr * es;             // low-rank partials
-  b += (int64_t)(kdim + 4) * (kdim + 1) * nr * es + 4096;   // Hessenberg, rotations, state
-  if (op->inner_maxits > 0) {  // inner Schur iteration: 2K+2 vectors of the level-0 remainder, 2 of level 0
-    const int64_t K = op->inner_maxits, m0 = n - op->lev_ptr[1], nl0 = op->lev_ptr[1];
-    b += ((2 * K + 2) * m0 + 2 * nl0) * nr * es + (K + 4) * (K + 1) * nr * 16 + 4096;
-  }
-  *bytes = b;
+  *bytes = (int64_t)ws_sizes(op, nr, kdim).total() + 16;  // + the pinned flag
   return 0;
 }
 
@@ -893,74 +966,17 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
   ws->kdim = kdim;
   try {
     HIPCHK(hipSetDevice(op->device));
-    const int64_t n = op->n;
-    const size_t es = vec_es(op);  // bytes of one value of the operator (double or complex<double>)
-    const size_t vb = (size_t)n * nr * es;
-    ws->r0.alloc(vb);
-    ws->x0.alloc(vb);
+    const WsSizes z = ws_sizes(op, nr, kdim);
+    ws->part_blocks = z.part_blocks;
+    ws->v_bytes = (int64_t)z.vb;
+    if (const char* v = std::getenv("PGS_FUSED_CGS2")) ws->fused_cgs2 = std::atoi(v) != 0;
     ws->zlev.clear();
-    for (int l = 0; l < op->nlev; ++l) {
+    for (size_t b : z.zlev) {
       std::unique_ptr<DevMem> m(new DevMem());
-      m->alloc((size_t)(n - op->lev_ptr[l]) * nr * es);
+      m->alloc(b);
       ws->zlev.push_back(std::move(m));
     }
-    int maxrank = 0;
-    for (int l = 0; l + 1 < op->nlev; ++l) {
-      const auto& lv = op->p64.present ? op->p64.levels : op->p32.present ? op->p32.levels
-                       : op->pz64.present ? op->pz64.levels : op->pz32.levels;
-      maxrank = std::max(maxrank, lv[l]->rank);
-    }
-    const int lr_blocks = kLrBlocks + 1;
-    ws->lrpart.alloc((size_t)std::max(1, lr_blocks * maxrank * nr) * es);
-    ws->lrt.alloc((size_t)std::max(1, maxrank * nr) * es);
-    // Krylov
-    ws->V.alloc((size_t)(kdim + 1) * vb);
-    ws->Z.alloc((size_t)kdim * vb);
-    ws->v_bytes = (int64_t)vb;
-    // dot products use at most 1024 workgroups (four per CU); the partials
-    // are summed by one wave per output
-    const int64_t rows_per_blk = std::max<int64_t>(256, (n + 1023) / 1024);
-    ws->part_blocks = (int)((n + rows_per_blk - 1) / rows_per_blk);
-    if (const char* v = std::getenv("PGS_FUSED_CGS2")) ws->fused_cgs2 = std::atoi(v) != 0;
-    // partial rows per workgroup: kdim + 1 dots, or 2 * 32 + 2 of a delayed-CGS2 dot chunk
-    ws->part.alloc((size_t)ws->part_blocks * std::max(kdim + 1, kPartRowsDcgs2) * nr * es);
-    ws->hbuf.alloc((size_t)(kdim + 1) * nr * es);
-    ws->h2buf.alloc((size_t)(kdim + 1) * nr * es);
-    ws->t2.alloc((size_t)nr * sizeof(double));
-    ws->H.alloc((size_t)nr * (kdim + 1) * kdim * es);
-    ws->cs.alloc((size_t)nr * kdim * es);
-    ws->sn.alloc((size_t)nr * kdim * es);
-    ws->rs.alloc((size_t)nr * (kdim + 1) * es);
-    ws->coef.alloc((size_t)(kdim + 1) * nr * es);
-    ws->upd.alloc((size_t)nr * sizeof(int));
-    ws->dc_d.alloc((size_t)(2 * kdim + 4) * nr * sizeof(double));
-    ws->dc_coef.alloc((size_t)(2 * kdim + 4) * nr * sizeof(double));
-    ws->st.alloc((size_t)nr * sizeof(KState));
-    ws->zflag.alloc((size_t)nr * sizeof(double));
-    ws->nb2.alloc((size_t)nr * sizeof(double));
-    ws->nr2.alloc((size_t)nr * sizeof(double));
-    ws->flag.alloc(16);
-    if (op->inner_maxits > 0) {
-      const int K = op->inner_maxits;
-      const int64_t m0 = n - op->lev_ptr[1], nl0 = op->lev_ptr[1];
-      const size_t ses = es;  // inner scalars: double or complex<double>, as the operator's values
-      ws->in_V.alloc((size_t)(K + 1) * m0 * nr * es);
-      ws->in_Z.alloc((size_t)K * m0 * nr * es);
-      ws->in_T.alloc((size_t)m0 * nr * es);
-      ws->in_U.alloc((size_t)std::max<int64_t>(1, nl0) * nr * es);
-      ws->in_Y.alloc((size_t)std::max<int64_t>(1, nl0) * nr * es);
-      ws->in_h.alloc((size_t)K * nr * ses);
-      ws->in_t2.alloc((size_t)nr * sizeof(double));
-      ws->in_H.alloc((size_t)nr * (K + 1) * K * ses);
-      ws->in_cs.alloc((size_t)nr * K * ses);
-      ws->in_sn.alloc((size_t)nr * K * ses);
-      ws->in_rs.alloc((size_t)nr * (K + 1) * ses);
-      ws->in_coef.alloc((size_t)K * nr * ses);
-      ws->in_upd.alloc((size_t)nr * sizeof(int));
-      ws->in_st.alloc((size_t)nr * sizeof(KState));
-      ws->in_flag.alloc((size_t)nr * sizeof(double));
-      ws->in_nb2.alloc((size_t)nr * sizeof(double));
-    }
+    for (const auto& e : z.bufs) (ws.get()->*(e.first)).alloc(e.second);
     HIPCHK(hipHostMalloc((void**)&ws->flag_host, 16, hipHostMallocDefault));
   } catch (const HipError& e) {
     return set_err(e);
