@@ -72,6 +72,9 @@ def main():
                     help="inner Schur iteration 'maxits,tol' (e.g. 5,1e-2, the reference's default); off if empty")
     ap.add_argument("--probe", action="store_true",
                     help="counter probe: --nrhs FP32 applies of the 128^3 operator and a 1 GiB copy")
+    ap.add_argument("--restart", default="",
+                    help="restart path: save the --config operator to this file, load it, upload it and compare"
+                         " the time with the setup")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is not built",
@@ -82,6 +85,9 @@ def main():
         return
     if args.probe:
         probe(args)
+        return
+    if args.restart:
+        restart(args)
         return
 
     import pargemslr_amd as pg
@@ -257,6 +263,55 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def restart(args):
+    """Restart without setup: the --config operator is set up once, written to
+    an operator file, read back and uploaded; the line reports the setup time
+    beside the save, load and upload times, and checks that the reloaded
+    operator's batched FP32 apply equals the original's bit for bit."""
+    import pargemslr_amd as pg
+
+    cfg = CONFIGS[args.config]
+    if "convdiff" in cfg:
+        A = pg.convdiff(*cfg["grid"], cfg["convdiff"])
+    else:
+        A = pg.laplacian(*cfg["grid"], shift=cfg.get("shift", 0.0))
+    params = pg.config_params(cfg["nlev"], cfg["ncomp"], cfg["droptol"], cfg["rownnz"], cfg["rank"], kdim=cfg["kdim"])
+    t0 = time.perf_counter()
+    G = pg.Gemslr(A, params, seed=0)
+    t_setup = time.perf_counter() - t0
+    log(f"[restart] setup {t_setup:.2f}s")
+    t0 = time.perf_counter()
+    G.save(args.restart)
+    t_save = time.perf_counter() - t0
+    size = os.path.getsize(args.restart)
+    log(f"[restart] saved {size / 1e6:.1f} MB in {t_save:.2f}s")
+    t0 = time.perf_counter()
+    H = pg.Gemslr.load(args.restart)
+    t_load = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    DH = pg.DevOp(H, prec_mask=pg.PGS_PREC_FP64 | pg.PGS_PREC_FP32)
+    pg.synchronize()
+    t_upload = time.perf_counter() - t0
+    log(f"[restart] loaded in {t_load:.2f}s, uploaded in {t_upload:.2f}s")
+    nr = args.nrhs
+    V = pg.DeviceArray.from_numpy(np.stack([pg.rng_normal(c, A.n) for c in range(nr)], axis=1))
+    outs = []
+    for D in (pg.DevOp(G, prec_mask=pg.PGS_PREC_FP32), DH):
+        W = pg.Workspace(D, max_nrhs=nr, kdim=2)
+        Z = pg.DeviceArray(V.shape)
+        W.apply(V, Z, nrhs=nr, prec=pg.PGS_PREC_FP32)
+        pg.synchronize()
+        outs.append(Z.numpy())
+    same = bool(np.array_equal(outs[0], outs[1]))
+    os.remove(args.restart)
+    print(json.dumps({"restart": f"{args.config}: grid {cfg['grid']}, nlev={cfg['nlev']} ncomp={cfg['ncomp']}", "setup_s": round(t_setup, 3), "save_s": round(t_save, 3),
+                      "file_mb": round(size / 1e6, 1), "load_s": round(t_load, 3),
+                      "upload_s": round(t_upload, 3), "load_plus_upload_s": round(t_load + t_upload, 3),
+                      "apply_bit_identical": same}), flush=True)
+    if not same:
+        sys.exit(1)
 
 
 def probe(args):

@@ -118,6 +118,8 @@ typedef struct {
   int64_t nnz_ilu;    /* sum over blocks of n_b + nnz(L_b) + nnz(U_b) */
   int64_t nnz_lr;     /* sum over levels of 2 * m_l * k_l */
   double setup_seconds[4]; /* partition+ordering, ILUT, low-rank, total */
+  int schur_maxits;        /* inner Schur iteration steps in every apply; 0 when off */
+  double schur_tol;        /* its relative tolerance */
 } pgs_gemslr_info_t;
 int pgs_gemslr_info(pgs_gemslr_t op, pgs_gemslr_info_t* info);
 
@@ -142,6 +144,22 @@ typedef struct {
 int pgs_gemslr_level_view(pgs_gemslr_t op, int level, pgs_level_view_t* view);
 /* perm[i]: (P b)[i] = b[perm[i]] */
 int pgs_gemslr_perm(pgs_gemslr_t op, const int** perm);
+/* The matrix the operator was set up on (0-based CSR, n x n; values as in
+ * pgs_gemslr_setup).  Valid until pgs_gemslr_destroy. */
+int pgs_gemslr_matrix(pgs_gemslr_t op, int* n, const int** ia, const int** ja, const double** a);
+
+/* Operator files: the complete setup output (matrix, permutation, levels,
+ * ILUT factors, couplings, low-rank terms, parameters) in one little-endian
+ * binary file closed by a 64-bit FNV-1a checksum, so that a restarted process
+ * uploads an operator without repeating GemslrClass::Setup.  The reference
+ * keeps its operators in memory only: these two calls have no counterpart
+ * there.  save writes path + ".tmp" and renames it to path.  load returns
+ * PGS_ERROR_IO for a missing, truncated, corrupted or structurally invalid
+ * file (CSR shapes and index ranges, strictly triangular factors, a true
+ * permutation, level and block offsets are all checked), else a handle that
+ * behaves exactly like the one that was saved (same factors, bit for bit). */
+int pgs_gemslr_save(pgs_gemslr_t op, const char* path);
+int pgs_gemslr_load(const char* path, pgs_gemslr_t* out);
 
 /* Host-resident apply z = M^{-1} v in FP64 (GemslrClass::Solve on kMemoryHost,
  * gemslr.cpp:3431-3467), with the inner Schur iteration on level 0 when it is
@@ -295,6 +313,28 @@ int pgs_gen_laplacian(int nx, int ny, int nz, double shift_re, double shift_im, 
  * (-x, -y, -z) neighbours -1 - c, downstream neighbours -1. */
 int pgs_gen_convdiff(int nx, int ny, int nz, double c, int* n, int64_t* nnz, int** ia, int** ja, double** a);
 void pgs_free(void* p);
+
+/* ---------------- MatrixMarket ingest ---------------- */
+
+/* CsrMatrixClass::ReadFromMMFile (ParGeMSLR/SRC/matrices/csr_matrix.cpp:
+ * 1706-1714 via CooMatrixReadFromFile, matrixops.cpp:11134-11453, and
+ * CooMatrixClass::ToCsr, coo_matrix.cpp:479-512): a "matrix coordinate" file
+ * with indices based at idxin (1 for standard files, 0 for the reference's
+ * "zerobase" option) into a CSR with rows in file order.  is_complex = 0 reads
+ * like the reference's double reader (real and integer data; off-diagonal
+ * entries mirrored for symmetric storage only); is_complex = 1 like its
+ * complexd reader (also complex data; symmetric, hermitian and skew-symmetric
+ * storage mirrored) and returns (re, im) pairs.  Returns PGS_ERROR_IO for a
+ * missing file, an unsupported type, a short entry list or an index outside
+ * the declared size.  Arrays are malloc'd; release with pgs_free. */
+int pgs_read_mm_csr(const char* path, int idxin, int is_complex, int* nrows, int* ncols, int64_t* nnz, int** ia,
+                    int** ja, double** a);
+/* SequentialVectorClass::ReadFromMMFile (ParGeMSLR/SRC/vectors/
+ * sequential_vector.cpp:1192-1195 via SequentialVectorReadFromFile,
+ * vectorops.cpp:1116-1336): a coordinate file of M rows into a dense vector
+ * of length n = M, zero where the file has no entry; entry (I, J) sets v[I].
+ * The complex reader requires one column.  Release v with pgs_free. */
+int pgs_read_mm_vector(const char* path, int idxin, int is_complex, int* n, double** v);
 /* counter-based splitmix64 streams: uniform [lo, hi) and standard normal */
 int pgs_rng_uniform(uint64_t seed, int64_t n, double lo, double hi, double* out);
 int pgs_rng_normal(uint64_t seed, int64_t n, double* out);

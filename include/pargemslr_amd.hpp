@@ -19,6 +19,7 @@
 #ifndef PARGEMSLR_AMD_HPP
 #define PARGEMSLR_AMD_HPP
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -110,6 +111,18 @@ class SequentialVectorClass {
     }
     return pgs_memcpy_dtoh(device_, dst, data_, (int64_t)n_ * (int64_t)sizeof(T), nullptr);
   }
+  // SequentialVectorClass::ReadFromMMFile (sequential_vector.cpp:1192-1195):
+  // a MatrixMarket coordinate vector, on the host
+  int ReadFromMMFile(const char* vecfile, int idxin) {
+    int n = 0;
+    double* v = nullptr;
+    int err = pgs_read_mm_vector(vecfile, idxin, ValueTraits<T>::is_complex, &n, &v);
+    if (err) return err;
+    err = Setup(n, kMemoryHost);
+    if (!err) std::memcpy(static_cast<void*>(data_), v, sizeof(T) * (size_t)n);
+    pgs_free(v);
+    return err;
+  }
   // SequentialVectorClass::MoveData
   int MoveData(int location) {
     if (location == location_) return 0;
@@ -159,6 +172,21 @@ class CsrMatrixClass {
                                 ValueTraits<T>::is_complex, &n, &nnz, &ia, &ja, &a);
     if (err) return err;
     err = Setup(n, ia, ja, reinterpret_cast<const T*>(a));
+    pgs_free(ia);
+    pgs_free(ja);
+    pgs_free(a);
+    return err;
+  }
+  // CsrMatrixClass::ReadFromMMFile (csr_matrix.cpp:1706-1714): a MatrixMarket
+  // coordinate matrix, indices based at idxin; the matrix must be square
+  int ReadFromMMFile(const char* matfile, int idxin) {
+    int nr = 0, nc = 0;
+    int64_t nnz = 0;
+    int *ia = nullptr, *ja = nullptr;
+    double* a = nullptr;
+    int err = pgs_read_mm_csr(matfile, idxin, ValueTraits<T>::is_complex, &nr, &nc, &nnz, &ia, &ja, &a);
+    if (err) return err;
+    err = nr == nc ? Setup(nr, ia, ja, reinterpret_cast<const T*>(a)) : PGS_ERROR_INVALID_PARAM;
     pgs_free(ia);
     pgs_free(ja);
     pgs_free(a);
@@ -247,11 +275,43 @@ class GemslrClass {
     return 0;
   }
   void SetSeed(uint64_t seed) { seed_ = seed; }
+  // Writes the host operator (after Setup or Load) to an operator file.
+  int Save(const char* path) const {
+    if (!host_) return PGS_ERROR_FUNCTION_CALL;
+    return pgs_gemslr_save(host_, path);
+  }
+  // Reads an operator file written by Save for the matrix given to SetMatrix
+  // (the file's matrix must equal it); the next Setup uploads it without
+  // running the host setup.
+  int Load(const char* path) {
+    if (!matrix_) return PGS_ERROR_INVALID_PARAM;
+    Clear();
+    int err = pgs_gemslr_load(path, &host_);
+    if (err) return err;
+    pgs_gemslr_info_t info;
+    int n = 0;
+    const int *ia = nullptr, *ja = nullptr;
+    const double* a = nullptr;
+    err = pgs_gemslr_info(host_, &info);
+    if (!err) err = pgs_gemslr_matrix(host_, &n, &ia, &ja, &a);
+    const MatrixType& A = *matrix_;
+    const bool same = !err && info.is_complex == VT::is_complex && n == A.n_ && ia[n] == A.ia_[n] &&
+                      std::equal(ia, ia + n + 1, A.ia_.begin()) && std::equal(ja, ja + ia[n], A.ja_.begin()) &&
+                      std::memcmp(a, A.a_.data(), sizeof(DataType) * (size_t)ia[n]) == 0;
+    if (!same) {
+      pgs_gemslr_destroy(host_);
+      host_ = nullptr;
+      return err ? err : PGS_ERROR_INVALID_PARAM;
+    }
+    return 0;
+  }
   virtual int Setup(VectorType&, VectorType&) {
     if (ready_) return 0;
     if (!matrix_) return PGS_ERROR_INVALID_PARAM;
-    int err = pgs_gemslr_setup(matrix_->n_, matrix_->ia_.data(), matrix_->ja_.data(), VT::raw(matrix_->a_.data()),
-                               VT::is_complex, params_, seed_, &host_);
+    int err = 0;
+    if (!host_)
+      err = pgs_gemslr_setup(matrix_->n_, matrix_->ia_.data(), matrix_->ja_.data(), VT::raw(matrix_->a_.data()),
+                             VT::is_complex, params_, seed_, &host_);
     if (err) return err;
     if (location_ == kMemoryDevice) {
       err = pgs_dev_op_create(host_, 0, PGS_PREC_FP64 | PGS_PREC_FP32, &dop_);

@@ -13,7 +13,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpargemslr_amd.so")
 
 HOST_SRC = ["host/partition.cpp", "host/rcm_ilut.cpp", "host/dense.cpp", "host/gemslr_setup.cpp",
-            "host/gen.cpp", "host/trsv_pack.cpp", "host/capi_host.cpp", "host/capi_gen.cpp"]
+            "host/gen.cpp", "host/trsv_pack.cpp", "host/capi_host.cpp", "host/capi_gen.cpp",
+            "host/mm_io.cpp", "host/op_store.cpp"]
 HIP_SRC = ["device/dev_op.hip", "device/server.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -52,11 +53,12 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     # C++ driver on the class surface (include/pargemslr_amd.hpp), run by the tests
     root = os.path.dirname(HERE)
-    drv_src = os.path.join(root, "tests", "cpp", "driver_laplacian.cpp")
-    if os.path.exists(drv_src):
-        drv = os.path.join(root, "tests", "cpp", "driver_laplacian")
-        _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), drv_src, "-o", drv, "-L", HERE,
-              "-lpargemslr_amd", "-Wl,-rpath,$ORIGIN/../../pargemslr_amd"])
+    for name in ("driver_laplacian", "driver_gen"):
+        drv_src = os.path.join(root, "tests", "cpp", name + ".cpp")
+        if os.path.exists(drv_src):
+            drv = os.path.join(root, "tests", "cpp", name)
+            _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), drv_src, "-o", drv,
+                  "-L", HERE, "-lpargemslr_amd", "-Wl,-rpath,$ORIGIN/../../pargemslr_amd"])
     return LIB
 
 

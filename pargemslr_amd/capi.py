@@ -41,7 +41,8 @@ class PgsError(RuntimeError):
 
 class GemslrInfo(C.Structure):
     _fields_ = [("n", C.c_int), ("nlev", C.c_int), ("is_complex", C.c_int), ("nnz_a", C.c_int64),
-                ("nnz_ilu", C.c_int64), ("nnz_lr", C.c_int64), ("setup_seconds", C.c_double * 4)]
+                ("nnz_ilu", C.c_int64), ("nnz_lr", C.c_int64), ("setup_seconds", C.c_double * 4),
+                ("schur_maxits", C.c_int), ("schur_tol", C.c_double)]
 
 
 class LevelView(C.Structure):
@@ -91,7 +92,8 @@ EXPORTED = [
     "pgs_ws_profile", "pgs_ws_profile_read", "pgs_dev_malloc", "pgs_dev_free", "pgs_memcpy_htod", "pgs_memcpy_dtoh",
     "pgs_memset_zero", "pgs_stream_create", "pgs_stream_destroy", "pgs_stream_synchronize", "pgs_device_synchronize",
     "pgs_memcpy_dtod", "pgs_dev_op_dims", "pgs_ws_estimate", "pgs_server_create", "pgs_server_add_operator", "pgs_server_submit",
-    "pgs_server_wait", "pgs_server_destroy",
+    "pgs_server_wait", "pgs_server_destroy", "pgs_gemslr_matrix", "pgs_gemslr_save", "pgs_gemslr_load",
+    "pgs_read_mm_csr", "pgs_read_mm_vector",
 ]
 
 _lib = None
@@ -115,6 +117,12 @@ def lib():
     L.pgs_gemslr_level_view.argtypes = [vp, C.c_int, C.POINTER(LevelView)]
     L.pgs_gemslr_perm.argtypes = [vp, C.POINTER(ip)]
     L.pgs_gemslr_host_apply.argtypes = [vp, dp, dp]
+    L.pgs_gemslr_matrix.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(dp)]
+    L.pgs_gemslr_save.argtypes = [vp, C.c_char_p]
+    L.pgs_gemslr_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.pgs_read_mm_csr.argtypes = [C.c_char_p, C.c_int, C.c_int, ip, ip, C.POINTER(C.c_int64), C.POINTER(ip),
+                                  C.POINTER(ip), C.POINTER(dp)]
+    L.pgs_read_mm_vector.argtypes = [C.c_char_p, C.c_int, C.c_int, ip, C.POINTER(dp)]
     L.pgs_device_count.argtypes = [ip]
     L.pgs_dev_op_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
     L.pgs_dev_op_destroy.argtypes = [vp]
@@ -227,6 +235,34 @@ def _take_csr(n, nnz, ia, ja, a, is_complex):
     return CsrHost(n, ia_np, ja_np, a_np)
 
 
+def read_mm(path, complex_=False, idxin=1):
+    """MatrixMarket coordinate file into a square CsrHost
+    (CsrMatrixClass::ReadFromMMFile rules; complex_ selects the complexd
+    reader)."""
+    n, m, nnz = C.c_int(), C.c_int(), C.c_int64()
+    ia, ja, a = C.POINTER(C.c_int)(), C.POINTER(C.c_int)(), C.POINTER(C.c_double)()
+    _chk(lib().pgs_read_mm_csr(os.fsencode(path), idxin, int(complex_), C.byref(n), C.byref(m), C.byref(nnz),
+                               C.byref(ia), C.byref(ja), C.byref(a)), "pgs_read_mm_csr")
+    if n.value != m.value:
+        for p in (ia, ja, a):
+            lib().pgs_free(C.cast(p, C.c_void_p))
+        raise PgsError(101, f"read_mm: {path} is {n.value} x {m.value}, not square")
+    return _take_csr(n, nnz, ia, ja, a, complex_)
+
+
+def read_mm_vector(path, complex_=False, idxin=1):
+    """MatrixMarket coordinate vector file into a dense numpy vector
+    (SequentialVectorClass::ReadFromMMFile rules)."""
+    n = C.c_int()
+    v = C.POINTER(C.c_double)()
+    _chk(lib().pgs_read_mm_vector(os.fsencode(path), idxin, int(complex_), C.byref(n), C.byref(v)),
+         "pgs_read_mm_vector")
+    nd = 2 if complex_ else 1
+    out = np.ctypeslib.as_array(v, shape=(max(n.value, 1) * nd,)).copy()[: n.value * nd]
+    lib().pgs_free(C.cast(v, C.c_void_p))
+    return out.view(np.complex128) if complex_ else out
+
+
 def laplacian(nx, ny, nz, shift=0.0, alphas=(0.0, 0.0, 0.0), complex_=False):
     n = C.c_int()
     nnz = C.c_int64()
@@ -276,11 +312,37 @@ class Gemslr:
         _chk(lib().pgs_gemslr_setup(A.n, _iptr(A.ia), _iptr(A.ja), _dptr(a), int(A.is_complex), _dptr(params),
                                     C.c_uint64(seed), C.byref(self._h)), "pgs_gemslr_setup")
 
+    @classmethod
+    def load(cls, path):
+        """Operator read from a file written by save() (no setup is run); its
+        matrix comes from the file too."""
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        self.params = None
+        _chk(lib().pgs_gemslr_load(os.fsencode(path), C.byref(self._h)), "pgs_gemslr_load")
+        n = C.c_int()
+        ia, ja, a = C.POINTER(C.c_int)(), C.POINTER(C.c_int)(), C.POINTER(C.c_double)()
+        _chk(lib().pgs_gemslr_matrix(self._h, C.byref(n), C.byref(ia), C.byref(ja), C.byref(a)), "pgs_gemslr_matrix")
+        n = n.value
+        ia_np = np.ctypeslib.as_array(ia, shape=(n + 1,)).copy()
+        nnz = int(ia_np[-1])
+        ja_np = np.ctypeslib.as_array(ja, shape=(max(nnz, 1),)).copy()[:nnz]
+        cplx = bool(self.info().is_complex)
+        nd = 2 if cplx else 1
+        a_np = np.ctypeslib.as_array(a, shape=(max(nnz, 1) * nd,)).copy()[: nnz * nd]
+        self.A = CsrHost(n, ia_np, ja_np, a_np.view(np.complex128) if cplx else a_np)
+        return self
+
+    def save(self, path):
+        """Writes the operator (the whole setup output) to path."""
+        _chk(lib().pgs_gemslr_save(self._h, os.fsencode(path)), "pgs_gemslr_save")
+
     def schur(self):
         """(maxits, tol) of the inner Schur iteration, or None when it is off."""
-        if self.params[IO["SCHUR_ENABLE"]] == 0 or self.info().nlev < 2:
+        inf = self.info()
+        if inf.schur_maxits == 0:
             return None
-        return int(self.params[IO["SCHUR_MAXITS"]]), float(self.params[IO["SCHUR_ITER_TOL"]])
+        return int(inf.schur_maxits), float(inf.schur_tol)
 
     def close(self):
         if self._h:

@@ -94,6 +94,10 @@ void fill_info(const HostOp<T>& H, pgs_gemslr_info_t* info) {
     info->nnz_lr += 2LL * H.nrem(l) * H.levels[l].rank;
   }
   for (int i = 0; i < 4; ++i) info->setup_seconds[i] = H.setup_seconds[i];
+  // the inner iteration runs on level 0 when there is a level below it
+  const bool inner = H.prm.schur_enable && H.nlev > 1;
+  info->schur_maxits = inner ? H.prm.schur_maxits : 0;
+  info->schur_tol = inner ? H.prm.schur_tol : 0.0;
 }
 
 template <typename T>
@@ -159,6 +163,66 @@ int pgs_gemslr_perm(pgs_gemslr_t op, const int** perm) {
   if (!op || !perm) return PGS_ERROR_INVALID_PARAM;
   pgs_gemslr_s* h = reinterpret_cast<pgs_gemslr_s*>(op);
   *perm = h->is_complex ? h->op_z->perm.data() : h->op_d->perm.data();
+  return 0;
+}
+
+int pgs_gemslr_save(pgs_gemslr_t op, const char* path) {
+  if (!op || !path) return PGS_ERROR_INVALID_PARAM;
+  pgs_gemslr_s* h = reinterpret_cast<pgs_gemslr_s*>(op);
+  try {
+    return h->is_complex ? op_save(*h->op_z, path) : op_save(*h->op_d, path);
+  } catch (const std::bad_alloc&) {
+    return PGS_ERR_OOM;
+  }
+}
+
+int pgs_gemslr_load(const char* path, pgs_gemslr_t* out) {
+  if (!out) return PGS_ERROR_INVALID_PARAM;
+  *out = nullptr;
+  if (!path) return PGS_ERROR_INVALID_PARAM;
+  int kind = 0;
+  int err = op_file_kind(path, &kind);
+  if (err) return err;
+  pgs_gemslr_s* h = new (std::nothrow) pgs_gemslr_s();
+  if (!h) return PGS_ERR_OOM;
+  h->is_complex = kind;
+  try {
+    if (!kind) {
+      h->op_d.reset(new HostOp<double>());
+      err = op_load(path, *h->op_d, &h->msg);
+      h->work_d.reset(new HostApplyWork<double>());
+    } else {
+      h->op_z.reset(new HostOp<zdouble>());
+      err = op_load(path, *h->op_z, &h->msg);
+      h->work_z.reset(new HostApplyWork<zdouble>());
+    }
+  } catch (const std::bad_alloc&) {
+    err = PGS_ERR_OOM;
+  }
+  if (err) {
+    delete h;
+    return err;
+  }
+  *out = reinterpret_cast<pgs_gemslr_t>(h);
+  return 0;
+}
+
+int pgs_gemslr_matrix(pgs_gemslr_t op, int* n, const int** ia, const int** ja, const double** a) {
+  if (!op || !n || !ia || !ja || !a) return PGS_ERROR_INVALID_PARAM;
+  pgs_gemslr_s* h = reinterpret_cast<pgs_gemslr_s*>(op);
+  if (h->is_complex) {
+    const Csr<zdouble>& A = h->op_z->A;
+    *n = A.nrows;
+    *ia = A.ia.data();
+    *ja = A.ja.data();
+    *a = reinterpret_cast<const double*>(A.a.data());
+  } else {
+    const Csr<double>& A = h->op_d->A;
+    *n = A.nrows;
+    *ia = A.ia.data();
+    *ja = A.ja.data();
+    *a = A.a.data();
+  }
   return 0;
 }
 

@@ -83,6 +83,24 @@ struct HostOp {
 template <typename T>
 int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::string* msg = nullptr);
 
+// Lcat / Ucat / Dcat of level l from its per-block factors (part of the setup;
+// also run when an operator is read back from a file).
+template <typename T>
+void build_level_cat(HostOp<T>& op, int l);
+
+// Operator files: the complete output of the setup (matrix, permutation,
+// levels, per-block ILUT factors, couplings, low-rank terms, parameters) in
+// a little-endian binary layout closed by a 64-bit checksum.  save writes to
+// path + ".tmp" and renames, so a reader never sees a partial file.  load
+// checks the checksum and the structure (CSR shapes and index ranges, strict
+// triangles, permutation, level and block offsets) before it returns 0.
+template <typename T>
+int op_save(const HostOp<T>& op, const char* path);
+// Reads the value kind stored in a file: 0 double, 1 complex.
+int op_file_kind(const char* path, int* is_complex);
+template <typename T>
+int op_load(const char* path, HostOp<T>& op, std::string* msg = nullptr);
+
 // Host apply z = M^{-1} v (GemslrClass::Solve, gemslr.cpp:3431-3467); used by
 // the setup's Arnoldi process.  Not the product solve path.
 template <typename T>

@@ -603,6 +603,33 @@ int lowrank_level(HostOp<T>& op, int l) {
 
 }  // namespace
 
+// Concatenated factors of one level (block-diagonal, level-local numbering)
+// from its per-block ILUT factors.
+template <typename T>
+void build_level_cat(HostOp<T>& op, int l) {
+  HostLevel<T>& L = op.levels[l];
+  const int nl = op.nloc(l), s = op.lev_ptr[l];
+  for (int pass = 0; pass < 2; ++pass) {
+    Csr<T>& M = pass == 0 ? L.Lcat : L.Ucat;
+    M = Csr<T>();
+    M.nrows = M.ncols = nl;
+    M.ia.assign(nl + 1, 0);
+    for (int b = 0; b < L.ncomp; ++b) {
+      const Csr<T>& f = pass == 0 ? L.ilu[b].L : L.ilu[b].U;
+      const int off = L.dom_ptr[b] - s;
+      for (int i = 0; i < f.nrows; ++i) {
+        for (int k = f.ia[i]; k < f.ia[i + 1]; ++k) { M.ja.push_back(f.ja[k] + off); M.a.push_back(f.a[k]); }
+        M.ia[off + i + 1] = (int)M.ja.size();
+      }
+    }
+  }
+  L.Dcat.resize(nl);
+  for (int b = 0; b < L.ncomp; ++b) {
+    const int off = L.dom_ptr[b] - s;
+    std::copy(L.ilu[b].Dinv.begin(), L.ilu[b].Dinv.end(), L.Dcat.begin() + off);
+  }
+}
+
 template <typename T>
 int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::string* msg) {
   const double t0 = now_s();
@@ -704,30 +731,7 @@ int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::s
     int rank = (l == 0) ? prm.lr_rank1 : prm.lr_rank2;
     if (rank > 0 || (l == 0 && prm.schur_enable)) L.C = csr_block(Apq, e, n - e, e, n - e);
   }
-  // concatenated level factors (block-diagonal, level-local numbering)
-  for (int l = 0; l < nlev; ++l) {
-    HostLevel<T>& L = op.levels[l];
-    const int nl = op.nloc(l), s = op.lev_ptr[l];
-    for (int pass = 0; pass < 2; ++pass) {
-      Csr<T>& M = pass == 0 ? L.Lcat : L.Ucat;
-      M = Csr<T>();
-      M.nrows = M.ncols = nl;
-      M.ia.assign(nl + 1, 0);
-      for (int b = 0; b < L.ncomp; ++b) {
-        const Csr<T>& f = pass == 0 ? L.ilu[b].L : L.ilu[b].U;
-        const int off = L.dom_ptr[b] - s;
-        for (int i = 0; i < f.nrows; ++i) {
-          for (int k = f.ia[i]; k < f.ia[i + 1]; ++k) { M.ja.push_back(f.ja[k] + off); M.a.push_back(f.a[k]); }
-          M.ia[off + i + 1] = (int)M.ja.size();
-        }
-      }
-    }
-    L.Dcat.resize(nl);
-    for (int b = 0; b < L.ncomp; ++b) {
-      const int off = L.dom_ptr[b] - s;
-      std::copy(L.ilu[b].Dinv.begin(), L.ilu[b].Dinv.end(), L.Dcat.begin() + off);
-    }
-  }
+  for (int l = 0; l < nlev; ++l) build_level_cat(op, l);
   const double t2 = now_s();
   // 4. low-rank corrections, bottom level first
   for (int l = nlev - 2; l >= 0; --l) {
@@ -744,6 +748,8 @@ int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::s
   return 0;
 }
 
+template void build_level_cat<double>(HostOp<double>&, int);
+template void build_level_cat<zdouble>(HostOp<zdouble>&, int);
 template int gemslr_setup<double>(const Csr<double>&, const GemslrParams&, HostOp<double>&, std::string*);
 template int gemslr_setup<zdouble>(const Csr<zdouble>&, const GemslrParams&, HostOp<zdouble>&, std::string*);
 template void host_apply<double>(const HostOp<double>&, const double*, double*, HostApplyWork<double>&);
