@@ -120,7 +120,40 @@ struct DevTrsv {
   int64_t nnz_padded = 0, nnz_real = 0, nrows_total = 0;
   int64_t max_block_ent = 0, max_block_steps = 0;  // largest entry and step counts of one block
   DevMem row0, step, stepU, ent0, ri0, desc, ent, ri;
+  // Half steps for the one-wave ring kernel (64 threads, 4 lanes per row, 16
+  // rows per pass) over the same ring packing: a step of more than 16 rows is
+  // walked as two consecutive half steps (rows 0-15, then 16-31).  Rows of a
+  // step are independent, and a half step reads and writes a subset of its
+  // step's positions, so the ring window still holds.
+  int64_t max_block_steps_h = 0;
+  DevMem step_h, stepU_h, desc_h;
 };
+
+// Half-step descriptors of a ring packing whose steps hold at most 32 rows.
+void half_steps(const std::vector<int>& desc, const std::vector<int>& blk_step, const std::vector<int>& blk_stepU,
+                std::vector<int>& desc_h, std::vector<int>& step_h, std::vector<int>& stepU_h) {
+  const int nb = (int)blk_step.size() - 1;
+  desc_h.clear();
+  step_h.assign(nb + 1, 0);
+  stepU_h.assign(nb, 0);
+  auto push = [&](int x, int y, int z, int w) {
+    desc_h.push_back(x);
+    desc_h.push_back(y);
+    desc_h.push_back(z);
+    desc_h.push_back(w);
+  };
+  for (int b = 0; b < nb; ++b) {
+    step_h[b] = (int)(desc_h.size() / 4);
+    for (int s = blk_step[b]; s < blk_step[b + 1]; ++s) {
+      if (s == blk_stepU[b]) stepU_h[b] = (int)(desc_h.size() / 4);
+      const int x = desc[4 * s], y = desc[4 * s + 1], z = desc[4 * s + 2], w = desc[4 * s + 3];
+      push(x, y, std::min(z, 16), w);
+      if (z > 16) push(x + 16, y + 16 * w, z - 16, w);
+    }
+    if (blk_stepU[b] == blk_step[b + 1]) stepU_h[b] = (int)(desc_h.size() / 4);
+  }
+  step_h[nb] = (int)(desc_h.size() / 4);
+}
 
 template <typename V, typename T>
 void upload_trsv(DevTrsv& d, const TrsvPack<T>& P, int64_t* counter) {
@@ -144,6 +177,18 @@ void upload_trsv(DevTrsv& d, const TrsvPack<T>& P, int64_t* counter) {
   upload(d.ent0, P.blk_ent0, counter);
   upload(d.ri0, P.blk_ri0, counter);
   upload(d.desc, P.desc, counter);
+  d.max_block_steps_h = 0;
+  int step_rows = 0;
+  for (size_t i = 2; i < P.desc.size(); i += 4) step_rows = std::max(step_rows, P.desc[i]);
+  if (P.window > 0 && step_rows <= 32) {
+    std::vector<int> dh, sh, suh;
+    half_steps(P.desc, P.blk_step, P.blk_stepU, dh, sh, suh);
+    for (int b = 0; b < P.nblocks; ++b)
+      d.max_block_steps_h = std::max<int64_t>(d.max_block_steps_h, sh[b + 1] - sh[b]);
+    upload(d.desc_h, dh, counter);
+    upload(d.step_h, sh, counter);
+    upload(d.stepU_h, suh, counter);
+  }
   std::vector<TPair<V>> ent(P.col.size() + 1), ri(P.rows.size() + 1);
   for (size_t i = 0; i < P.col.size(); ++i) {
     ent[i] = TPair<V>();
@@ -190,8 +235,10 @@ struct pgs_dev_op_s {
   int num_cus = 256;     // compute units of the device
   int trsv_nt = 128;     // threads of the pipelined kernel
   int trsv_lpr = 4;      // lanes per row (steps hold at most trsv_nt / trsv_lpr rows)
-  int ring_nt = 128;     // ring kernel: threads (64: one wave, no barrier per step)
-  int ring_lpr = 4;      // ring kernel: lanes per row (steps hold at most ring_nt / ring_lpr rows)
+  // ring kernel geometry: 0 = chosen per launch (128x4 or one-wave 64x4 on
+  // half steps); 128 / 64 with ring_lpr 4 or 2 force one geometry
+  int ring_nt = 0;
+  int ring_lpr = 4;
   int inner_maxits = 0;     // > 0: inner FGMRES on the level-0 Schur complement in every apply
   double inner_tol = 0.0;
   int64_t bytes = 0;
@@ -378,7 +425,7 @@ constexpr int kRingWidth = 20;
 // may use the whole register file (the compiler moves the excess to AGPRs).
 template <typename V, int CG>
 constexpr int ring_depth() {
-  return sizeof(V) == 4 ? (CG == 1 ? 20 : 12) : sizeof(V) == 8 ? (CG == 1 ? 6 : 10) : 4;
+  return sizeof(V) == 4 ? 12 : sizeof(V) == 8 ? 6 : 4;
 }
 
 template <typename V, int NT, int LPR, int CG, int G = 1>
@@ -387,8 +434,13 @@ void launch_trsv_ring_cg(const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStre
   auto kern = k_trsv_ring<V, NT, LPR, kRingWidth / LPR, D, CG, G>;  // LDS limit raised in set_kernel_attrs
   const size_t lds = (size_t)(T.window + 1) * CG * G * sizeof(V);  // a ring + dummy slot per group
   const unsigned grid = (unsigned)T.nblocks * (unsigned)(nrhs / (CG * G));
-  kern<<<grid, NT * G, lds, s>>>(T.nblocks, T.window - 1, T.row0.as<int>(), T.step.as<int>(), T.stepU.as<int>(),
-                                 T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), T.desc.as<int4>(),
+  // 64 threads with 4 lanes per row take 16 rows per pass: they walk the half steps
+  const bool half = NT / LPR < 32;
+  const DevMem& step = half ? T.step_h : T.step;
+  const DevMem& stepU = half ? T.stepU_h : T.stepU;
+  const DevMem& desc = half ? T.desc_h : T.desc;
+  kern<<<grid, NT * G, lds, s>>>(T.nblocks, T.window - 1, T.row0.as<int>(), step.as<int>(), stepU.as<int>(),
+                                 T.ent0.as<int64_t>(), T.ri0.as<int64_t>(), desc.as<int4>(),
                                  T.ent.as<TPair<V>>(), T.ri.as<TPair<V>>(), rhs, x, nrhs);
 }
 
@@ -398,15 +450,23 @@ void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
   // step descriptors with 32-bit byte offsets
   if ((int64_t)T.max_rows * nrhs * (int64_t)sizeof(V) >= (int64_t)INT32_MAX ||
       (T.max_block_ent + 1) * (int64_t)sizeof(TPair<V>) >= (int64_t)INT32_MAX ||
-      (T.max_block_steps + 64) * 16 >= (int64_t)INT32_MAX)
+      (std::max(T.max_block_steps, T.max_block_steps_h) + 64) * 16 >= (int64_t)INT32_MAX)
     throw std::runtime_error("diagonal block too large for 32-bit offsets");
   int cg = std::min(nrhs, op->trsv_cg);
   if (op->ring_lpr == 2) cg = std::min(cg, 2);  // a lane pair finalizes at most two columns
   while (cg > 1 && (size_t)(T.window + 1) * cg * sizeof(V) > (size_t)op->lds_max) cg >>= 1;
-  if (op->ring_nt == 64 && op->ring_lpr == 2) {
+  // Per launch: at two or more workgroups per CU the CUs' issue and memory
+  // pipes bound the solve, and one-wave workgroups on half steps (no barrier
+  // per step) do the same work in less time; below that, 128 threads take the
+  // steps whole and halve the dependent chain.
+  int nt = op->ring_nt;
+  if (nt == 0)
+    nt = (T.desc_h.p && (int64_t)T.nblocks * (nrhs / cg) >= 2 * (int64_t)op->num_cus) ? 64 : 128;
+  if (nt == 64 && op->ring_lpr == 4 && !T.desc_h.p) nt = 128;
+  if (nt == 64 && op->ring_lpr == 2) {
     if (cg >= 2) launch_trsv_ring_cg<V, 64, 2, 2>(T, rhs, x, nrhs, s);
     else launch_trsv_ring_cg<V, 64, 2, 1>(T, rhs, x, nrhs, s);
-  } else if (op->ring_nt == 64) {
+  } else if (nt == 64) {
     if (cg >= 4) launch_trsv_ring_cg<V, 64, 4, 4>(T, rhs, x, nrhs, s);
     else if (cg == 2) launch_trsv_ring_cg<V, 64, 4, 2>(T, rhs, x, nrhs, s);
     else launch_trsv_ring_cg<V, 64, 4, 1>(T, rhs, x, nrhs, s);
@@ -724,7 +784,9 @@ void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask) {
   // ring packing (ring kernel) unless another kernel geometry was asked for;
   // the ring holds at least one column of the operator's values
   const bool ring = op->trsv_variant == 0 && op->trsv_nt == 128 && op->trsv_lpr == 4;
-  const int max_rows = ring ? op->ring_nt / op->ring_lpr : op->trsv_nt / op->trsv_lpr;
+  // ring steps hold at most 32 rows: 128 threads x 4 lanes or 64 x 2 take a
+  // step whole, 64 x 4 takes it in two half steps
+  const int max_rows = ring ? 32 : op->trsv_nt / op->trsv_lpr;
   int max_window = 0;
   if (ring) {
     max_window = 1;

@@ -444,10 +444,11 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
 #pragma unroll
   for (int k = 0; k < D / 2; ++k) PGS_RING_RHS(k);
 
-  for (int base = s_begin; base < s_end; base += D) {
+  for (int base = s_begin; base < s_end; base += 2 * D) {
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const int s = base + k;
+    for (int kk = 0; kk < 2 * D; ++kk) {
+      const int k = kk % D;
+      const int s = base + kk;
       // gather this step's slots from the ring and take what the arithmetic
       // needs out of the stage registers
       RingRow<V, CG> xv[EPL];
@@ -457,8 +458,6 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
         xv[e] = reinterpret_cast<const RingRow<V, CG>*>(xs)[pe[k][e].col];
         wgt[e] = pe[k][e].val;  // padded slots weigh zero
       }
-      // keep the gathers ahead of the refill loads below
-      __builtin_amdgcn_sched_barrier(0);
       const int slot = sw[k];
       const int row = pr[k].col;
       const V dv = pr[k].val;
@@ -466,12 +465,6 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
       V xin[NCL];
 #pragma unroll
       for (int j = 0; j < NCL; ++j) xin[j] = xr[k][j];
-      // refill this stage with step s + D and load the right-hand side of
-      // step s + D/2, whose row pair arrived D/2 steps ago; the loads are in
-      // flight while the LDS gathers return
-      PGS_RING_STAGE(k, dn[k]);
-      dn[k] = bload_i3(rs_desc, (s - sbeg + 2 * D) * 16);
-      PGS_RING_RHS((k + D / 2) % D);
       V acc[CG];
 #pragma unroll
       for (int c = 0; c < CG; ++c) acc[c] = 0;
@@ -509,6 +502,15 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
       // the buffer's range (dropped): no branch
       xs[wr ? slot * CG + cl[0] : dummy] = v;
       bstore_val(rs_dst, wr ? (row * nrhs + c0 + cl[0]) * VS : 0x7ffffff0, v);
+      // refill this stage with step s + D and load the right-hand side of
+      // step s + D/2, whose row pair arrived D/2 steps ago.  The refill
+      // follows the last use of the stage's registers, so the old and the
+      // new contents are never live together: the loop needs no register
+      // copies at its back edge, and no copy waits for a load in flight.
+      __builtin_amdgcn_sched_barrier(0);
+      PGS_RING_STAGE(k, dn[k]);
+      dn[k] = bload_i3(rs_desc, (s - sbeg + 2 * D) * 16);
+      PGS_RING_RHS((k + D / 2) % D);
       if constexpr (NT == 64 && G == 1) {
         // one wave: its LDS operations complete in issue order, so the next
         // step's gathers see this step's writes without a wait or barrier;
