@@ -401,15 +401,21 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   constexpr int VS = (int)sizeof(V);
   constexpr int RW = LPR * EPL;  // slab width of every ring-packed step
   static_assert(D % 2 == 0, "stage depth must be even");
-  static_assert((LPR == 4 && (CG == 1 || CG == 2 || CG == 4)) || (LPR == 2 && (CG == 1 || CG == 2)),
-                "quad lanes with 1, 2 or 4 columns, or lane pairs with 1 or 2 columns");
-  constexpr int NCL = 1;  // columns finalized per lane
+  static_assert((LPR == 4 && (CG == 1 || CG == 2 || CG == 4 || CG == 8)) || (LPR == 2 && (CG == 1 || CG == 2)),
+                "quad lanes with 1, 2, 4 or 8 columns, or lane pairs with 1 or 2 columns");
+  constexpr int NCL = CG == 8 ? 2 : 1;  // columns finalized per lane
   const int tid = threadIdx.x % NT;  // thread within this column group
   const int q = tid % LPR;
   const int t0 = tid / LPR;
-  // after the reduction lane q holds column q % CG; lanes q < CG write it
+  // after the reduction lane q holds column q % CG (lanes q < CG write it), or
+  // with 8 columns columns 2q and 2q + 1
   int cl[NCL];
-  cl[0] = q % CG;
+  if constexpr (NCL == 2) {
+    cl[0] = 2 * q;
+    cl[1] = 2 * q + 1;
+  } else {
+    cl[0] = q % CG;
+  }
   const bool writer = q < CG;
   const int dummy = (wmask + 1) * CG;  // LDS slot written by lanes with nothing to write
 
@@ -474,13 +480,23 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
         for (int c = 0; c < CG; ++c) acc[c] += wgt[e] * xv[e].v[c];
       }
       // sum over the row's lanes, scattered so that lane q ends with column q % CG
-      V a;
-      if constexpr (LPR == 2) {
+      // (with 8 columns: columns 2q and 2q + 1)
+      V a[NCL];
+      if constexpr (CG == 8) {
+        // xor 2: keep columns 0-3 (q bit 1 clear) or 4-7, send the other half
+        const bool b0 = q & 1, b1 = q & 2;
+        V h[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) h[c] = (b1 ? acc[c + 4] : acc[c]) + dpp_xor2(b1 ? acc[c] : acc[c + 4]);
+        // xor 1: keep columns 0-1 of the half (q bit 0 clear) or 2-3
+#pragma unroll
+        for (int c = 0; c < 2; ++c) a[c] = (b0 ? h[c + 2] : h[c]) + dpp_xor1(b0 ? h[c] : h[c + 2]);
+      } else if constexpr (LPR == 2) {
         if constexpr (CG == 2) {
           const bool b0 = q & 1;
-          a = (b0 ? acc[1] : acc[0]) + dpp_xor1(b0 ? acc[0] : acc[1]);
+          a[0] = (b0 ? acc[1] : acc[0]) + dpp_xor1(b0 ? acc[0] : acc[1]);
         } else {
-          a = pair_sum(acc[0]);
+          a[0] = pair_sum(acc[0]);
         }
       } else if constexpr (CG == 4) {
         // xor 1: keep columns with bit 0 = q bit 0, send the other two
@@ -488,20 +504,23 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
         const V k0 = (b0 ? acc[1] : acc[0]) + dpp_xor1(b0 ? acc[0] : acc[1]);
         const V k1 = (b0 ? acc[3] : acc[2]) + dpp_xor1(b0 ? acc[2] : acc[3]);
         // xor 2: keep the pair with bit 1 = q bit 1
-        a = (b1 ? k1 : k0) + dpp_xor2(b1 ? k0 : k1);
+        a[0] = (b1 ? k1 : k0) + dpp_xor2(b1 ? k0 : k1);
       } else if constexpr (CG == 2) {
         const bool b0 = q & 1;
         const V k0 = (b0 ? acc[1] : acc[0]) + dpp_xor1(b0 ? acc[0] : acc[1]);
-        a = k0 + dpp_xor2(k0);
+        a[0] = k0 + dpp_xor2(k0);
       } else {
-        a = quad_sum(acc[0]);
+        a[0] = quad_sum(acc[0]);
       }
-      V v = xin[0] - a;
-      if (UPPER) v *= dv;
       // lanes with nothing to write store to the dummy LDS slot and out of
       // the buffer's range (dropped): no branch
-      xs[wr ? slot * CG + cl[0] : dummy] = v;
-      bstore_val(rs_dst, wr ? (row * nrhs + c0 + cl[0]) * VS : 0x7ffffff0, v);
+#pragma unroll
+      for (int j = 0; j < NCL; ++j) {
+        V v = xin[j] - a[j];
+        if (UPPER) v *= dv;
+        xs[wr ? slot * CG + cl[j] : dummy] = v;
+        bstore_val(rs_dst, wr ? (row * nrhs + c0 + cl[j]) * VS : 0x7ffffff0, v);
+      }
       // refill this stage with step s + D and load the right-hand side of
       // step s + D/2, whose row pair arrived D/2 steps ago.  The refill
       // follows the last use of the stage's registers, so the old and the
