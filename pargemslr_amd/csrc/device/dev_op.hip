@@ -421,19 +421,21 @@ void launch_trsv_pipe(int nt, int lpr, const DevTrsv& T, const V* rhs, V* x, int
 // lanes (32 rows) per row.
 constexpr int kRingWidth = 20;
 
-// stage depth of the ring kernel's register ring (steps whose entries are in
-// flight), measured at 128^3 level 0: float 20 for one column per group, 12
-// for 2-4 columns (more registers per stage); double and complex<float> 6 /
-// 10; complex<double> 4.  The kernel keeps one wave per SIMD, so deep rings
-// may use the whole register file (the compiler moves the excess to AGPRs).
-template <typename V, int CG>
+// Stage depth of the ring kernel's register ring (steps whose entries are in
+// flight): the deepest ring whose registers stay within 256 VGPRs, so that
+// two waves fit on a SIMD and no stage is moved to AGPRs (a moved stage costs
+// register copies that wait for loads in flight).  With 4 lanes per row:
+// float 12 (8 for 8-column groups), 8-byte values 6, complex<double> 4; lane
+// pairs hold twice the slots per lane: 6 / 4 / 2.
+template <typename V, int CG, int LPR = 4>
 constexpr int ring_depth() {
+  if constexpr (LPR == 2) return sizeof(V) == 4 ? 6 : sizeof(V) == 8 ? 4 : 2;
   return sizeof(V) == 4 ? (CG == 8 ? 8 : 12) : sizeof(V) == 8 ? 6 : 4;
 }
 
 template <typename V, int NT, int LPR, int CG, int G = 1>
 void launch_trsv_ring_cg(const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
-  constexpr int D = ring_depth<V, CG>();
+  constexpr int D = ring_depth<V, CG, LPR>();
   auto kern = k_trsv_ring<V, NT, LPR, kRingWidth / LPR, D, CG, G>;  // LDS limit raised in set_kernel_attrs
   const size_t lds = (size_t)(T.window + 1) * CG * G * sizeof(V);  // a ring + dummy slot per group
   const unsigned grid = (unsigned)T.nblocks * (unsigned)(nrhs / (CG * G));
@@ -456,6 +458,12 @@ void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
       (std::max(T.max_block_steps, T.max_block_steps_h) + 64) * 16 >= (int64_t)INT32_MAX)
     throw std::runtime_error("diagonal block too large for 32-bit offsets");
   int cg = std::min(nrhs, op->trsv_cg > 0 ? op->trsv_cg : 4);
+  // Below one workgroup per CU each group's step chain is the bound, and a
+  // narrower group has shorter steps: spread the columns over as many groups
+  // as there are CUs (measured at 128^3 level 0, FP32: 4 RHS with 4 columns
+  // per group 0.93 ms, with 1 column per group on 256 workgroups ~0.75 ms).
+  if (op->trsv_cg == 0)
+    while (cg > 1 && (int64_t)T.nblocks * (nrhs / cg) * 2 <= (int64_t)op->num_cus) cg >>= 1;
   // Eight columns per group halve the per-column loads, address arithmetic
   // and reductions but lengthen each step: they pay only when four-column
   // groups would already keep every CU four workgroups deep (measured at
@@ -546,8 +554,8 @@ void set_trsv_attrs(int lds_max) {
   set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D1, 1, 1>, lds_max);
   set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D2, 2, 1>, lds_max);
   set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D4, 4, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, D1, 1, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, D2, 2, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, (ring_depth<V, 1, 2>()), 1, 1>, lds_max);
+  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, (ring_depth<V, 2, 2>()), 2, 1>, lds_max);
   if constexpr (sizeof(V) == 4) {
     constexpr int D8 = ring_depth<V, 8>();
     set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D8, 8, 1>, lds_max);
