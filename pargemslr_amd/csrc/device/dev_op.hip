@@ -591,12 +591,19 @@ void set_kernel_attrs(int lds_max) {
     set_lds_limit(k_basis_reg<kBasisNT, 32, 0, 2>, lds_max);
     set_lds_limit(k_basis_reg<kBasisNT, 32, 1, 2>, lds_max);
     // coefficient tables of up to 127 x 64 values in LDS
-    set_lds_limit(k_dcgs2_update<kBasisNT, 8, 1>, lds_max);
-    set_lds_limit(k_dcgs2_update<kBasisNT, 16, 1>, lds_max);
-    set_lds_limit(k_dcgs2_update<kBasisNT, 32, 1>, lds_max);
-    set_lds_limit(k_dcgs2_update<kBasisNT, 64, 1>, lds_max);
-    set_lds_limit(k_dcgs2_update<kBasisNT, 32, 2>, lds_max);
-    set_lds_limit(k_dcgs2_update<kBasisNT, 64, 2>, lds_max);
+    set_lds_limit(k_dcgs2_update<double, kBasisNT, 8, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<double, kBasisNT, 16, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<double, kBasisNT, 32, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<double, kBasisNT, 64, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<double, kBasisNT, 32, 2>, lds_max);
+    set_lds_limit(k_dcgs2_update<double, kBasisNT, 64, 2>, lds_max);
+    using Z = cplx<double>;
+    set_lds_limit(k_dcgs2_update<Z, kBasisNT, 8, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<Z, kBasisNT, 16, 1>, lds_max);
+    set_lds_limit(k_dcgs2_update<Z, kBasisNT, 16, 2>, lds_max);
+    set_lds_limit(k_dcgs2_update<Z, kBasisNT, 32, 2>, lds_max);
+    set_lds_limit(k_dcgs2_dots<Z, kBasisNT, 8, 1>, lds_max);
+    set_lds_limit(k_dcgs2_dots<Z, kBasisNT, 8, 2>, lds_max);
     set_lds_limit(k_multi_axpy_norm<double, double, kNT>, lds_max);
     set_lds_limit(k_multi_axpy_norm<cplx<double>, cplx<double>, kNT>, lds_max);
     set_lds_limit(k_update_x<double, double>, lds_max);
@@ -909,8 +916,8 @@ WsSizes ws_sizes(const pgs_dev_op_s* op, int nr, int kdim) {
   add(&pgs_ws_s::rs, (size_t)nr * (ik + 1) * es);
   add(&pgs_ws_s::coef, (ik + 1) * nr * es);
   add(&pgs_ws_s::upd, (size_t)nr * sizeof(int));
-  add(&pgs_ws_s::dc_d, (2 * ik + 4) * nr * sizeof(double));
-  add(&pgs_ws_s::dc_coef, (2 * ik + 4) * nr * sizeof(double));
+  add(&pgs_ws_s::dc_d, (2 * ik + 4) * nr * es);
+  add(&pgs_ws_s::dc_coef, (2 * ik + 4) * nr * es);
   add(&pgs_ws_s::st, (size_t)nr * sizeof(KState));
   add(&pgs_ws_s::zflag, (size_t)nr * sizeof(double));
   add(&pgs_ws_s::nb2, (size_t)nr * sizeof(double));
@@ -1216,48 +1223,63 @@ struct PrecTypes<cplx<double>> {
 // FGMRES (FlexGmresClass::Solve, fgmres.hpp:300-540) on nrhs columns at once:
 // Krylov scalar S (double or complex<double>), preconditioned basis Z in VZ.
 // Delayed CGS2, dots of step p: d = [s (p rows) | a (p rows) | q.q | q.w]
-// with s = V_{0..p-1} . q, a = V_{0..p-1} . w, q = basis slot p; with w ==
+// with s = V_{0..p-1}^H q, a = V_{0..p-1}^H w, q = basis slot p; with w ==
 // nullptr (the closing pass of a cycle) d = [s | q.q].  The basis is read in
-// chunks of 16 vectors per lane (two lanes per element up to 32 columns).
-void dcgs2_dots(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64_t ldv, const double* w, double* d,
+// chunks of 16 vectors per lane for real values, 8 for complex ones (two
+// lanes per element up to 32 columns).
+template <typename S>
+void dcgs2_dots(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ldv, const S* w, S* d,
                 hipStream_t s) {
-  const double* q = V + (int64_t)p * ldv;
+  constexpr int PM = scalar_traits<S>::is_complex ? 8 : 16;
+  const S* q = V + (int64_t)p * ldv;
   const int lpe = nrhs <= 32 ? 2 : 1;
-  const int chunk = 16 * lpe;
+  const int chunk = PM * lpe;
   const int64_t nthr = (n * nrhs * lpe + kBasisNT - 1) / kBasisNT;
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(ws->part_blocks, nthr));
   const int hw = w != nullptr;
-  double* part = ws->part.as<double>();
+  S* part = ws->part.as<S>();
   for (int j0 = 0; j0 == 0 || j0 < p; j0 += chunk) {
     const int cnt = std::min(chunk, p - j0);
     const int first = j0 == 0;
     const int nout = (hw ? 2 * cnt : cnt) + (first ? (hw ? 2 : 1) : 0);
-    const size_t lds = sizeof(double) * (size_t)nout * nrhs;
+    const size_t lds = sizeof(S) * (size_t)nout * nrhs;
     if (lpe == 2)
-      k_dcgs2_dots<kBasisNT, 16, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, j0, cnt, V, ldv, q, w, first, part);
+      k_dcgs2_dots<S, kBasisNT, PM, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, j0, cnt, V, ldv, q, w, first, part);
     else
-      k_dcgs2_dots<kBasisNT, 16, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, j0, cnt, V, ldv, q, w, first, part);
-    k_dcgs2_final<<<(nout * nrhs + 3) / 4, 256, 0, s>>>(nblk, nout, cnt, nrhs, hw, p, j0, part, d);
+      k_dcgs2_dots<S, kBasisNT, PM, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, j0, cnt, V, ldv, q, w, first, part);
+    k_dcgs2_final<S><<<(nout * nrhs + 3) / 4, 256, 0, s>>>(nblk, nout, cnt, nrhs, hw, p, j0, part, d);
   }
   HIPCHK(hipGetLastError());
 }
 
 // Delayed CGS2, update of step p (k_dcgs2_update), then t2[c] = ||w(:, c)||^2.
-void dcgs2_update(pgs_ws_s* ws, int64_t n, int nrhs, int p, const double* V, int64_t ldv, double* q, double* w,
-                  const double* coef, double* t2, hipStream_t s) {
-  // one lane per element up to 32 vectors (64 with 64 columns), then two
-  const bool two = p > 32 && nrhs <= 32;
+// Real values: one lane per element up to 32 vectors (64 with 64 columns),
+// then two; complex values (twice the registers per vector): one lane up to
+// 16 vectors, then two, at most 64 vectors.
+template <typename S>
+void dcgs2_update(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ldv, S* q, S* w, const S* coef,
+                  double* t2, hipStream_t s) {
+  constexpr bool cx = scalar_traits<S>::is_complex;
+  const bool two = cx ? p > 16 : (p > 32 && nrhs <= 32);
   const int64_t nthr = (n * nrhs * (two ? 2 : 1) + kBasisNT - 1) / kBasisNT;
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(ws->part_blocks, nthr));
-  const size_t lds = sizeof(double) * ((size_t)(2 * p + 2) * nrhs + (size_t)(kBasisNT / kWave) * nrhs);
+  const size_t lds = sizeof(S) * (size_t)(2 * p + 2) * nrhs + sizeof(double) * (size_t)(kBasisNT / kWave) * nrhs;
   double* part = ws->part.as<double>();
-  if (p <= 8) k_dcgs2_update<kBasisNT, 8, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-  else if (p <= 16) k_dcgs2_update<kBasisNT, 16, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-  else if (p <= 32) k_dcgs2_update<kBasisNT, 32, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-  else if (two && p <= 64) k_dcgs2_update<kBasisNT, 32, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-  else if (two) k_dcgs2_update<kBasisNT, 64, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-  else if (p <= 64) k_dcgs2_update<kBasisNT, 64, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-  else throw std::runtime_error("delayed CGS2: more than 64 basis vectors need at most 32 columns");
+  if constexpr (cx) {
+    if (p <= 8) k_dcgs2_update<S, kBasisNT, 8, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else if (p <= 16) k_dcgs2_update<S, kBasisNT, 16, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else if (p <= 32) k_dcgs2_update<S, kBasisNT, 16, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else if (p <= 64) k_dcgs2_update<S, kBasisNT, 32, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else throw std::runtime_error("delayed CGS2: complex bases hold at most 64 vectors");
+  } else {
+    if (p <= 8) k_dcgs2_update<S, kBasisNT, 8, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else if (p <= 16) k_dcgs2_update<S, kBasisNT, 16, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else if (p <= 32) k_dcgs2_update<S, kBasisNT, 32, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else if (two && p <= 64) k_dcgs2_update<S, kBasisNT, 32, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else if (two) k_dcgs2_update<S, kBasisNT, 64, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else if (p <= 64) k_dcgs2_update<S, kBasisNT, 64, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else throw std::runtime_error("delayed CGS2: more than 64 basis vectors need at most 32 columns");
+  }
   k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, part, t2, 0);
   HIPCHK(hipGetLastError());
 }
@@ -1278,7 +1300,10 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
   const int maxits = o.maxits;
   // delayed CGS2 on real bases (more than 64 basis vectors need two lanes per
   // element, i.e. at most 32 columns); complex bases use CGS2
-  const bool dcgs2 = o.orth == PGS_ORTH_DCGS2 && !scalar_traits<S>::is_complex && (m <= 64 || nrhs <= 32);
+  // delayed CGS2 keeps a step's coefficients in LDS and its basis values in
+  // registers: real bases up to 64 vectors, or more with at most 32 columns;
+  // complex bases up to 64 vectors
+  const bool dcgs2 = o.orth == PGS_ORTH_DCGS2 && (scalar_traits<S>::is_complex ? m <= 64 : (m <= 64 || nrhs <= 32));
   // ||b||, v0 = b - A x, ||v0||
   colnorm2<S>(ws, n, nrhs, b, ws->nb2.as<double>(), s);
   launch_spmv<S, S, S, S>(op->A, x, V, b, S(1.0), S(0.0), 1, nrhs, s);
@@ -1322,12 +1347,12 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
         // delayed CGS2: finalize v_{i-1} (and Hessenberg column i-2, with its
         // Givens update) from the dots of this sweep, then project w once
         const int p = i - 1;
-        double* d = ws->dc_d.as<double>();
-        double* coef = ws->dc_coef.as<double>();
-        dcgs2_dots(ws, n, nrhs, p, V, ldv, w, d, ss);
-        k_dcgs2_fix<<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef);
-        dcgs2_update(ws, n, nrhs, p, V, ldv, vprev, w, coef, t2, ss);
-        k_dcgs2_norm<<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st);
+        S* d = ws->dc_d.as<S>();
+        S* coef = ws->dc_coef.as<S>();
+        dcgs2_dots<S>(ws, n, nrhs, p, V, ldv, w, d, ss);
+        k_dcgs2_fix<S><<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef);
+        dcgs2_update<S>(ws, n, nrhs, p, V, ldv, vprev, w, coef, t2, ss);
+        k_dcgs2_norm<S><<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st);
         k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w);
         HIPCHK(hipGetLastError());
         return;
@@ -1425,10 +1450,10 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
       if (dcgs2 && !none_active) {
         // close the cycle: finalize the last basis vector and Hessenberg
         // column of the columns still running (one more dot sweep)
-        double* d = ws->dc_d.as<double>();
-        dcgs2_dots(ws, n, nrhs, i, V, ldv, nullptr, d, s);
-        k_dcgs2_fix<<<1, 64, 0, s>>>(nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st,
-                                     ws->dc_coef.as<double>());
+        S* d = ws->dc_d.as<S>();
+        dcgs2_dots<S>(ws, n, nrhs, i, V, ldv, nullptr, d, s);
+        k_dcgs2_fix<S><<<1, 64, 0, s>>>(nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st,
+                                        ws->dc_coef.as<S>());
         HIPCHK(hipGetLastError());
       }
     }

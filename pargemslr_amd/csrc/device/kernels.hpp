@@ -447,45 +447,46 @@ __global__ __launch_bounds__(NT) void k_basis_reg(int64_t n, int nrhs, int p, co
 // V_{j0 + u LPE + sub}, u < ceil(cnt / LPE) <= PM; every lane walks the
 // elements with a grid stride; reductions over lanes, then over waves in a
 // fixed order (one LDS table of nout * nrhs doubles, the waves add in turn).
-template <int NT, int PM, int LPE>
-__global__ __launch_bounds__(NT) void k_dcgs2_dots(int64_t n, int nrhs, int j0, int cnt, const double* __restrict__ V,
-                                                   int64_t ldv, const double* __restrict__ q,
-                                                   const double* __restrict__ w, int first,
-                                                   double* __restrict__ part) {
-  extern __shared__ double dd_sm[];
+template <typename S, int NT, int PM, int LPE>
+__global__ __launch_bounds__(NT) void k_dcgs2_dots(int64_t n, int nrhs, int j0, int cnt, const S* __restrict__ V,
+                                                   int64_t ldv, const S* __restrict__ q, const S* __restrict__ w,
+                                                   int first, S* __restrict__ part) {
+  extern __shared__ unsigned char dd_raw[];
+  S* dd_sm = reinterpret_cast<S*>(dd_raw);
   const bool hw = w != nullptr;
   const int nout = (hw ? 2 * cnt : cnt) + (first ? (hw ? 2 : 1) : 0);
   const int tid = threadIdx.x;
   const int sub = tid % LPE;
-  const int c = (tid / LPE) % nrhs;
   const int pu = (cnt + LPE - 1) / LPE;
-  const double* Vl = V + (int64_t)(j0 + sub) * ldv;
-  double aq[PM], aw[PM];
-  double aa = 0.0, ab = 0.0;
+  const S* Vl = V + (int64_t)(j0 + sub) * ldv;
+  S aq[PM], aw[PM];
+  S aa = S(0.0), ab = S(0.0);
 #pragma unroll
   for (int u = 0; u < PM; ++u) {
-    aq[u] = 0.0;
-    aw[u] = 0.0;
+    aq[u] = S(0.0);
+    aw[u] = S(0.0);
   }
   const int64_t total = n * nrhs;
   const int64_t stride = (int64_t)gridDim.x * (NT / LPE);
   for (int64_t g = ((int64_t)blockIdx.x * NT + tid) / LPE; g < total; g += stride) {
-    const double qv = q[g];
-    const double wv = hw ? w[g] : 0.0;
-    double v[PM];
+    const S qv = q[g];
+    const S wv = hw ? w[g] : S(0.0);
+    S v[PM];
 #pragma unroll
     for (int u = 0; u < PM; ++u)
       if (u < pu && u * LPE + sub < cnt) v[u] = Vl[(int64_t)(u * LPE) * ldv + g];
-      else v[u] = 0.0;
+      else v[u] = S(0.0);
+    // dots conjugate the basis (x^H y)
 #pragma unroll
     for (int u = 0; u < PM; ++u)
       if (u < pu) {
-        aq[u] += v[u] * qv;
-        aw[u] += v[u] * wv;
+        const S vc = sconj(v[u]);
+        aq[u] += vc * qv;
+        aw[u] += vc * wv;
       }
     if (sub == 0) {
-      aa += qv * qv;
-      ab += qv * wv;
+      aa += sconj(qv) * qv;
+      ab += sconj(qv) * wv;
     }
   }
   // reduce over the lanes of a column (lanes c LPE + sub keep the sums)
@@ -493,16 +494,16 @@ __global__ __launch_bounds__(NT) void k_dcgs2_dots(int64_t n, int nrhs, int j0, 
   for (int u = 0; u < PM; ++u) {
     if (u < pu) {
       for (int mm = kWave / 2; mm >= nrhs * LPE; mm >>= 1) {
-        aq[u] += __shfl_xor(aq[u], mm);
-        aw[u] += __shfl_xor(aw[u], mm);
+        aq[u] += shfl_xor_v(aq[u], mm);
+        aw[u] += shfl_xor_v(aw[u], mm);
       }
     }
   }
   for (int mm = kWave / 2; mm >= nrhs * LPE; mm >>= 1) {
-    aa += __shfl_xor(aa, mm);
-    ab += __shfl_xor(ab, mm);
+    aa += shfl_xor_v(aa, mm);
+    ab += shfl_xor_v(ab, mm);
   }
-  for (int t = tid; t < nout * nrhs; t += NT) dd_sm[t] = 0.0;
+  for (int t = tid; t < nout * nrhs; t += NT) dd_sm[t] = S(0.0);
   __syncthreads();
   // the waves add their sums to the table in turn (a fixed order)
   const int wid = tid / kWave, lane = tid % kWave;
@@ -533,52 +534,52 @@ __global__ __launch_bounds__(NT) void k_dcgs2_dots(int64_t n, int nrhs, int j0, 
 // |w|^2 in part[blk * nrhs + c].  coef = [s (p x nrhs) | g (p x nrhs) |
 // inv_nu (nrhs) | gamma (nrhs)], copied to LDS.  Element layout as
 // k_basis_reg (LPE lanes per element, lane sub holds V_{u LPE + sub}).
-template <int NT, int PM, int LPE>
-__global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p, const double* __restrict__ V,
-                                                     int64_t ldv, double* __restrict__ q, double* __restrict__ w,
-                                                     const double* __restrict__ coef, double* __restrict__ part) {
-  extern __shared__ double du_sm[];
+template <typename S, int NT, int PM, int LPE>
+__global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p, const S* __restrict__ V,
+                                                     int64_t ldv, S* __restrict__ q, S* __restrict__ w,
+                                                     const S* __restrict__ coef, double* __restrict__ part) {
+  extern __shared__ unsigned char du_raw[];
   const int nc = (2 * p + 2) * nrhs;
-  double* cs = du_sm;
-  double* red = du_sm + nc;
+  S* cs = reinterpret_cast<S*>(du_raw);
+  double* red = reinterpret_cast<double*>(cs + nc);
   const int tid = threadIdx.x;
   for (int t = tid; t < nc; t += NT) cs[t] = coef[t];
   __syncthreads();
-  const double* sc = cs;
-  const double* gc = cs + p * nrhs;
-  const double* inu = cs + 2 * p * nrhs;
-  const double* gam = inu + nrhs;
+  const S* sc = cs;
+  const S* gc = cs + p * nrhs;
+  const S* inu = cs + 2 * p * nrhs;
+  const S* gam = inu + nrhs;
   const int sub = tid % LPE;
   const int c = (tid / LPE) % nrhs;
   const int pu = (p + LPE - 1) / LPE;
-  const double* Vl = V + (int64_t)sub * ldv;
+  const S* Vl = V + (int64_t)sub * ldv;
   double acc = 0.0;
   const int64_t total = n * nrhs;
   const int64_t stride = (int64_t)gridDim.x * (NT / LPE);
   for (int64_t g = ((int64_t)blockIdx.x * NT + tid) / LPE; g < total; g += stride) {
-    double v[PM];
+    S v[PM];
 #pragma unroll
     for (int u = 0; u < PM; ++u)
-      if (u < pu) v[u] = (u * LPE + sub < p) ? Vl[(int64_t)(u * LPE) * ldv + g] : 0.0;
-    double sv = 0.0, gv = 0.0;
+      if (u < pu) v[u] = (u * LPE + sub < p) ? Vl[(int64_t)(u * LPE) * ldv + g] : S(0.0);
+    S sv = S(0.0), gv = S(0.0);
 #pragma unroll
     for (int u = 0; u < PM; ++u)
       if (u < pu) {
         const int j = u * LPE + sub;
         const bool in = j < p;
-        sv += v[u] * (in ? sc[j * nrhs + c] : 0.0);
-        gv += v[u] * (in ? gc[j * nrhs + c] : 0.0);
+        sv += v[u] * (in ? sc[j * nrhs + c] : S(0.0));
+        gv += v[u] * (in ? gc[j * nrhs + c] : S(0.0));
       }
     if constexpr (LPE == 2) {
-      sv += __shfl_xor(sv, 1);
-      gv += __shfl_xor(gv, 1);
+      sv += shfl_xor_v(sv, 1);
+      gv += shfl_xor_v(gv, 1);
     }
     if (sub == 0) {
-      const double qo = q[g];
-      const double un = w[g] - gv - gam[c] * qo;
+      const S qo = q[g];
+      const S un = w[g] - gv - gam[c] * qo;
       q[g] = (qo - sv) * inu[c];
       w[g] = un;
-      acc += un * un;
+      acc += (double)sabs2(un);
     }
   }
   const int wid = tid / kWave, lane = tid % kWave;
@@ -600,24 +601,25 @@ __global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p,
 // (q.w - s.a) / nu, and coef = [s | g = a - gamma s | 1/nu | gamma = c_p / nu]
 // for k_dcgs2_update.  p = 0: q is the exact v0 (nu = 1, no column to finalize).
 // Columns that are not active get zero coefficients.
-__global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, const double* __restrict__ d,
-                            double* __restrict__ H, double* __restrict__ cs, double* __restrict__ sn,
-                            double* __restrict__ rs, KState* __restrict__ st, double* __restrict__ coef) {
+template <typename S>
+__global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, const S* __restrict__ d,
+                            S* __restrict__ H, S* __restrict__ cs, S* __restrict__ sn, S* __restrict__ rs,
+                            KState* __restrict__ st, S* __restrict__ coef) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
   KState s = st[c];
-  double* sco = coef;
-  double* gco = coef + (int64_t)p * nrhs;
-  double* inu = coef + (int64_t)2 * p * nrhs;
-  double* gam = inu + nrhs;
+  S* sco = coef;
+  S* gco = coef + (int64_t)p * nrhs;
+  S* inu = coef + (int64_t)2 * p * nrhs;
+  S* gam = inu + nrhs;
   auto zero = [&]() {
     if (final_pass) return;
     for (int j = 0; j < p; ++j) {
-      sco[j * nrhs + c] = 0.0;
-      gco[j * nrhs + c] = 0.0;
+      sco[j * nrhs + c] = S(0.0);
+      gco[j * nrhs + c] = S(0.0);
     }
-    inu[c] = 0.0;
-    gam[c] = 0.0;
+    inu[c] = S(0.0);
+    gam[c] = S(0.0);
   };
   if (!s.active) {
     zero();
@@ -625,33 +627,33 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
   }
   const double eps = 2.220446049250313e-16;
   // d rows: s_j (j < p), then a_j (j < p), then q.q, q.w (final: s_j, q.q)
-  const double qq = final_pass ? d[p * nrhs + c] : d[2 * p * nrhs + c];
-  const double qw = final_pass ? 0.0 : d[(2 * p + 1) * nrhs + c];
+  const double qq = (double)sreal(final_pass ? d[p * nrhs + c] : d[2 * p * nrhs + c]);
+  const S qw = final_pass ? S(0.0) : d[(2 * p + 1) * nrhs + c];
   double nu = 1.0;
-  double* Hc = H + (int64_t)c * (m + 1) * m;
+  S* Hc = H + (int64_t)c * (m + 1) * m;
   if (p > 0) {
     double ss = 0.0;
-    for (int j = 0; j < p; ++j) ss += d[j * nrhs + c] * d[j * nrhs + c];
+    for (int j = 0; j < p; ++j) ss += (double)sabs2(d[j * nrhs + c]);
     nu = sqrt(fmax(qq - ss, 0.0));
     const int col = p - 1;
-    const double hq = Hc[p + col * (m + 1)];
+    const S hq = Hc[p + col * (m + 1)];
     for (int j = 0; j < p; ++j) Hc[j + col * (m + 1)] += hq * d[j * nrhs + c];
-    Hc[p + col * (m + 1)] = hq * nu;
-    // Givens update of column col (fgmres.hpp:428-461)
-    double* csc = cs + (int64_t)c * m;
-    double* snc = sn + (int64_t)c * m;
-    double* rsc = rs + (int64_t)c * (m + 1);
+    Hc[p + col * (m + 1)] = hq * S(nu);
+    // Givens update of column col, as k_givens (fgmres.hpp:428-461)
+    S* csc = cs + (int64_t)c * m;
+    S* snc = sn + (int64_t)c * m;
+    S* rsc = rs + (int64_t)c * (m + 1);
     s.iters += 1;
     s.icur = p;
     for (int j = 1; j < p; ++j) {
-      const double hii = Hc[(j - 1) + col * (m + 1)];
-      const double hj = Hc[j + col * (m + 1)];
-      Hc[(j - 1) + col * (m + 1)] = csc[j - 1] * hii + snc[j - 1] * hj;
+      const S hii = Hc[(j - 1) + col * (m + 1)];
+      const S hj = Hc[j + col * (m + 1)];
+      Hc[(j - 1) + col * (m + 1)] = sconj(csc[j - 1]) * hii + snc[j - 1] * hj;
       Hc[j + col * (m + 1)] = -snc[j - 1] * hii + csc[j - 1] * hj;
     }
-    const double hii = Hc[(p - 1) + col * (m + 1)];
-    const double hii1 = Hc[p + col * (m + 1)];
-    const double g = sqrt(hii * hii + hii1 * hii1);
+    const S hii = Hc[(p - 1) + col * (m + 1)];
+    const S hii1 = Hc[p + col * (m + 1)];
+    const double g = sqrt((double)sabs2(hii) + (double)sabs2(hii1));
     if (fabs(g) < eps) {  // breakdown: the reference leaves x untouched (fgmres.hpp:440-443)
       s.status = 2;
       s.active = 0;
@@ -660,12 +662,12 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
       zero();
       return;
     }
-    csc[p - 1] = hii / g;
-    snc[p - 1] = hii1 / g;
+    csc[p - 1] = hii / S(g);
+    snc[p - 1] = hii1 / S(g);
     rsc[p] = -snc[p - 1] * rsc[p - 1];
-    rsc[p - 1] = csc[p - 1] * rsc[p - 1];
-    Hc[(p - 1) + col * (m + 1)] = csc[p - 1] * hii + snc[p - 1] * hii1;
-    s.normr = fabs(rsc[p]);
+    rsc[p - 1] = sconj(csc[p - 1]) * rsc[p - 1];
+    Hc[(p - 1) + col * (m + 1)] = sconj(csc[p - 1]) * hii + snc[p - 1] * hii1;
+    s.normr = sabs_d(rsc[p]);
     if (s.normr <= s.tolr) {
       s.status = 1;
       s.active = 0;
@@ -680,34 +682,36 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
   }
   if (final_pass) return;
   const double in = nu > 0.0 ? 1.0 / nu : 0.0;
-  double sa = 0.0;
-  for (int j = 0; j < p; ++j) sa += d[j * nrhs + c] * d[(p + j) * nrhs + c];
-  const double cp = (qw - sa) * in;
-  const double gm = cp * in;
+  // coefficient of w on the final v_p: (q - V s)^H w / nu = (q.w - s^H a) / nu
+  S sa = S(0.0);
+  for (int j = 0; j < p; ++j) sa += sconj(d[j * nrhs + c]) * d[(p + j) * nrhs + c];
+  const S cp = (qw - sa) * S(in);
+  const S gm = cp * S(in);
   for (int j = 0; j < p; ++j) {
-    const double sj = d[j * nrhs + c], aj = d[(p + j) * nrhs + c];
+    const S sj = d[j * nrhs + c], aj = d[(p + j) * nrhs + c];
     Hc[j + p * (m + 1)] = aj;
     sco[j * nrhs + c] = sj;
     gco[j * nrhs + c] = aj - gm * sj;
   }
   Hc[p + p * (m + 1)] = cp;
-  inu[c] = in;
+  inu[c] = S(in);
   gam[c] = gm;
 }
 
 // Sums the nblk partials of one k_dcgs2_dots launch into the rows of the
 // step's dot table d (rows: s_j, then a_j, then q.q and q.w; or s_j, q.q
 // without w), one wave per output, in a fixed order.
+template <typename S>
 __global__ __launch_bounds__(256) void k_dcgs2_final(int nblk, int nout, int cnt, int nrhs, int hw, int p, int j0,
-                                                     const double* __restrict__ part, double* __restrict__ d) {
+                                                     const S* __restrict__ part, S* __restrict__ d) {
   const int g = blockIdx.x * 4 + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
   const int total = nout * nrhs;
   if (g >= total) return;  // whole wave exits together
-  double s = 0.0;
+  S s = S(0.0);
   for (int b = lane; b < nblk; b += kWave) s += part[(int64_t)b * total + g];
 #pragma unroll
-  for (int mm = kWave / 2; mm > 0; mm >>= 1) s += __shfl_xor(s, mm);
+  for (int mm = kWave / 2; mm > 0; mm >>= 1) s += shfl_xor_v(s, mm);
   if (lane != 0) return;
   const int o = g / nrhs, c = g % nrhs;
   int row;
@@ -719,7 +723,8 @@ __global__ __launch_bounds__(256) void k_dcgs2_final(int nblk, int nout, int cnt
 
 // After k_dcgs2_update of step p: the tentative H[p+1, p] = ||w|| and the
 // scale that normalizes w (zero for columns that are not active).
-__global__ void k_dcgs2_norm(int nrhs, int m, int p, const double* __restrict__ t2, double* __restrict__ H,
+template <typename S>
+__global__ void k_dcgs2_norm(int nrhs, int m, int p, const double* __restrict__ t2, S* __restrict__ H,
                              KState* __restrict__ st) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
@@ -728,7 +733,7 @@ __global__ void k_dcgs2_norm(int nrhs, int m, int p, const double* __restrict__ 
     s.scale = 0.0;
   } else {
     const double t = sqrt(t2[c]);
-    H[(int64_t)c * (m + 1) * m + (p + 1) + p * (m + 1)] = t;
+    H[(int64_t)c * (m + 1) * m + (p + 1) + p * (m + 1)] = S(t);
     s.scale = t > 0.0 ? 1.0 / t : 0.0;
   }
   st[c] = s;

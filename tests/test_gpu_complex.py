@@ -100,7 +100,7 @@ def test_complex_apply_host_agrees(cplx, gpu):
     assert np.linalg.norm(_host(zd) - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize("orth", [pg.PGS_ORTH_MGS, pg.PGS_ORTH_CGS2])
+@pytest.mark.parametrize("orth", [pg.PGS_ORTH_MGS, pg.PGS_ORTH_CGS2, pg.PGS_ORTH_DCGS2])
 def test_complex_fgmres_fp64_parity(orth, cplx, gpu):
     A, G, D, W = cplx
     b = _crandn(21, A.n)
@@ -153,7 +153,47 @@ def test_complex_fgmres_unpreconditioned_matches_oracle(cplx, gpu):
     assert rr == pytest.approx(rr_o, rel=1e-6)
 
 
-def test_complex_graph_replay_matches_direct(cplx, gpu):
+@pytest.mark.parametrize("kdim", [20, 40, 64])
+def test_complex_dcgs2_unpreconditioned_matches_oracle(kdim, cplx, gpu):
+    """Delayed CGS2 on a complex basis (conjugated dots, the lagged
+    normalization and Hessenberg correction, complex Givens): without a
+    preconditioner the Givens residual estimates equal the oracle's MGS ones
+    over 2 cycles (GMRES residuals are unique), for bases that take the
+    one-lane and two-lane update kernels."""
+    A, G, D, W = cplx
+    b = _crandn(42, A.n)
+    W2 = pg.Workspace(D, max_nrhs=2, kdim=kdim)
+    _, st, it_o, rr_o, _ = orc.fgmres(A, b, prec=None, kdim=kdim, maxits=2 * kdim, tol=1e-14, orth=0)
+    xd = _dev(np.zeros(A.n), gpu)
+    (status, it, rr), = W2.fgmres(_dev(b, gpu), xd, kdim=kdim, maxits=2 * kdim, tol=1e-14, use_precond=False,
+                                  orth=pg.PGS_ORTH_DCGS2)
+    assert it == it_o == 2 * kdim
+    assert rr == pytest.approx(rr_o, rel=1e-6)
+
+
+def test_complex_dcgs2_batched_equals_single(cplx, gpu):
+    """Two complex columns with delayed CGS2 and the complex64 apply: each
+    column reaches the tolerance and equals its single solve."""
+    A, G, D, W = cplx
+    nr = 2
+    B = np.stack([_crandn(60 + c, A.n) for c in range(nr)], axis=1)
+    Xd = _dev(np.zeros((A.n, nr)), gpu)
+    res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=30, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP32,
+                   orth=pg.PGS_ORTH_DCGS2)
+    X = _host(Xd)
+    for c in range(nr):
+        st, it, _ = res[c]
+        assert st == 0
+        assert np.linalg.norm(B[:, c] - A.matvec(X[:, c])) <= 1e-8 * np.linalg.norm(B[:, c])
+        x1 = _dev(np.zeros(A.n), gpu)
+        (st1, it1, _), = W.fgmres(_dev(B[:, c], gpu), x1, kdim=30, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP32,
+                                  orth=pg.PGS_ORTH_DCGS2)
+        assert st1 == 0 and abs(it1 - it) <= 1
+        assert np.linalg.norm(X[:, c] - _host(x1)) <= 1e-6 * np.linalg.norm(X[:, c])
+
+
+@pytest.mark.parametrize("orth", [pg.PGS_ORTH_CGS2, pg.PGS_ORTH_DCGS2])
+def test_complex_graph_replay_matches_direct(orth, cplx, gpu):
     A, G, D, W = cplx
     nr = 2
     B = np.stack([_crandn(50 + c, A.n) for c in range(nr)], axis=1)
@@ -161,7 +201,7 @@ def test_complex_graph_replay_matches_direct(cplx, gpu):
     for use_graph in (False, True, True):
         Xd = _dev(np.zeros((A.n, nr)), gpu)
         res = W.fgmres(_dev(B, gpu), Xd, nrhs=nr, kdim=10, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP32,
-                       use_graph=use_graph)
+                       use_graph=use_graph, orth=orth)
         outs.append((_host(Xd), [r[:2] for r in res]))
     for X, res in outs[1:]:
         assert res == outs[0][1]
