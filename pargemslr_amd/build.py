@@ -53,7 +53,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     # C++ driver on the class surface (include/pargemslr_amd.hpp), run by the tests
     root = os.path.dirname(HERE)
-    for name in ("driver_laplacian", "driver_gen"):
+    for name in ("driver_laplacian", "driver_gen", "pack_check"):
         drv_src = os.path.join(root, "tests", "cpp", name + ".cpp")
         if os.path.exists(drv_src):
             drv = os.path.join(root, "tests", "cpp", name)

@@ -169,4 +169,29 @@ TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_ro
 template TrsvPack<double> trsv_pack<double>(const HostLevel<double>&, int, int, int, int, int);
 template TrsvPack<zdouble> trsv_pack<zdouble>(const HostLevel<zdouble>&, int, int, int, int, int);
 
+void half_steps(const std::vector<int>& desc, const std::vector<int>& blk_step, const std::vector<int>& blk_stepU,
+                std::vector<int>& desc_h, std::vector<int>& step_h, std::vector<int>& stepU_h) {
+  const int nb = (int)blk_step.size() - 1;
+  desc_h.clear();
+  step_h.assign(nb + 1, 0);
+  stepU_h.assign(nb, 0);
+  auto push = [&](int x, int y, int z, int w) {
+    desc_h.push_back(x);
+    desc_h.push_back(y);
+    desc_h.push_back(z);
+    desc_h.push_back(w);
+  };
+  for (int b = 0; b < nb; ++b) {
+    step_h[b] = (int)(desc_h.size() / 4);
+    for (int s = blk_step[b]; s < blk_step[b + 1]; ++s) {
+      if (s == blk_stepU[b]) stepU_h[b] = (int)(desc_h.size() / 4);
+      const int x = desc[4 * s], y = desc[4 * s + 1], z = desc[4 * s + 2], w = desc[4 * s + 3];
+      push(x, y, std::min(z, 16), w);
+      if (z > 16) push(x + 16, y + 16 * w, z - 16, w);
+    }
+    if (blk_stepU[b] == blk_step[b + 1]) stepU_h[b] = (int)(desc_h.size() / 4);
+  }
+  step_h[nb] = (int)(desc_h.size() / 4);
+}
+
 }  // namespace pgs

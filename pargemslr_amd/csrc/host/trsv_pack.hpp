@@ -67,4 +67,13 @@ template <typename T>
 TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_rows, int max_window = 0,
                       int ring_width = 0);
 
+// Half steps of a ring packing whose steps hold at most 32 rows, for a solve
+// that takes 16 rows per pass: a step of more than 16 rows becomes two
+// consecutive half steps (rows 0-15, then the rest) over the same slabs and
+// row pairs.  Rows of a step are independent and a half step reads and
+// writes a subset of its step's positions, so the ring window still holds.
+// desc_h / step_h / stepU_h have the layout of desc / blk_step / blk_stepU.
+void half_steps(const std::vector<int>& desc, const std::vector<int>& blk_step, const std::vector<int>& blk_stepU,
+                std::vector<int>& desc_h, std::vector<int>& step_h, std::vector<int>& stepU_h);
+
 }  // namespace pgs
