@@ -312,12 +312,18 @@ void launch_spmv(const DevCsr& M, const VX* x, VY* y, const VY* r, VACC alpha, V
     }
 #undef PGS_SPMV_CASE
   } else {
-    const int64_t threads = (int64_t)M.nrows * nrhs;
+    auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+    const bool wide = nrhs % 4 == 0 && al16(x) && al16(y) && (r == nullptr || al16(r));
+    const int64_t threads = (int64_t)M.nrows * nrhs / (wide ? 4 : 1);
     // a multiple of the XCD count (the SpMV kernels renumber workgroups per
     // XCD; the padding workgroups find no rows)
     const unsigned g = (nblocks_for(threads, kNT) + kXcds - 1) / kXcds * kXcds;
-    k_spmvN<VA, VX, VY, VACC><<<g, kNT, 0, s>>>(M.nrows, nrhs, M.ia.as<int>(), M.ja.as<int>(), M.a.as<VA>(), x, y, r,
-                                                alpha, beta, mode);
+    if (wide)
+      k_spmvN4<VA, VX, VY, VACC><<<g, kNT, 0, s>>>(M.nrows, nrhs, M.ia.as<int>(), M.ja.as<int>(), M.a.as<VA>(), x, y,
+                                                   r, alpha, beta, mode);
+    else
+      k_spmvN<VA, VX, VY, VACC><<<g, kNT, 0, s>>>(M.nrows, nrhs, M.ia.as<int>(), M.ja.as<int>(), M.a.as<VA>(), x, y,
+                                                  r, alpha, beta, mode);
   }
   HIPCHK(hipGetLastError());
 }
@@ -777,10 +783,14 @@ void apply_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const VI* v, VZ* z, int nrhs, hi
   const int64_t n = op->n;
   V* r0 = ws->r0.as<V>();
   V* x0 = ws->x0.as<V>();
-  k_gather<VI, V><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
+  auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  const bool wide = nrhs % 4 == 0 && al16(v) && al16(z);  // workspace buffers are allocation-aligned
+  if (wide) k_gather4<VI, V><<<nblocks_for(n * nrhs / 4, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
+  else k_gather<VI, V><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
   HIPCHK(hipGetLastError());
   solve_level<V>(op, ws, P, 0, r0, x0, nrhs, s);
-  k_scatter<V, VZ><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), x0, z);
+  if (wide) k_scatter4<V, VZ><<<nblocks_for(n * nrhs / 4, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), x0, z);
+  else k_scatter<V, VZ><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), x0, z);
   HIPCHK(hipGetLastError());
 }
 

@@ -27,6 +27,17 @@ __device__ __forceinline__ cplx<R> shfl_xor_v(cplx<R> v, int m, int width = kWav
 template <typename To, typename From>
 __device__ __forceinline__ To cvt(From v) { return (To)v; }
 
+// Row and column of element t of a row-interleaved block of nrhs columns.
+// Every entry point admits only power-of-two column counts, so a shift and a
+// mask replace the 64-bit division (tens of instructions per element in the
+// per-element kernels); other counts take the division.
+__device__ __forceinline__ int64_t row_of(int64_t t, int nrhs) {
+  return (nrhs & (nrhs - 1)) == 0 ? t >> __builtin_ctz((unsigned)nrhs) : t / nrhs;
+}
+__device__ __forceinline__ int col_of(int64_t t, int nrhs) {
+  return (nrhs & (nrhs - 1)) == 0 ? (int)(t & (nrhs - 1)) : (int)(t % nrhs);
+}
+
 // ---------------------------------------------------------------------------
 // permutations (IntVectorClass::GatherPerm / ScatterRperm,
 // ParGeMSLR/SRC/vectors/int_vector.cpp:629-696) fused with a precision change
@@ -35,17 +46,55 @@ template <typename VI, typename VO>
 __global__ void k_gather(int64_t n, int nrhs, const int* __restrict__ perm, const VI* __restrict__ in, VO* __restrict__ out) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * nrhs) return;
-  int64_t i = t / nrhs;
-  int c = (int)(t - i * nrhs);
+  const int64_t i = row_of(t, nrhs);
+  const int c = col_of(t, nrhs);
   out[t] = cvt<VO>(in[(int64_t)perm[i] * nrhs + c]);
+}
+
+// Four consecutive columns of a row per thread (nrhs a multiple of 4, both
+// buffers 16-byte aligned): 16-byte and wider accesses instead of one value
+// per lane.
+template <typename V, int W>
+struct alignas(W * sizeof(V) < 16 ? W * sizeof(V) : 16) ColPack {
+  V v[W];
+};
+
+template <typename VI, typename VO>
+__global__ void k_gather4(int64_t n, int nrhs, const int* __restrict__ perm, const VI* __restrict__ in,
+                          VO* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = 4 * t;
+  if (e >= n * nrhs) return;
+  const int64_t i = row_of(e, nrhs);
+  const int c = col_of(e, nrhs);
+  const ColPack<VI, 4> a = *reinterpret_cast<const ColPack<VI, 4>*>(in + (int64_t)perm[i] * nrhs + c);
+  ColPack<VO, 4> b;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) b.v[u] = cvt<VO>(a.v[u]);
+  *reinterpret_cast<ColPack<VO, 4>*>(out + e) = b;
+}
+
+template <typename VI, typename VO>
+__global__ void k_scatter4(int64_t n, int nrhs, const int* __restrict__ perm, const VI* __restrict__ in,
+                           VO* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = 4 * t;
+  if (e >= n * nrhs) return;
+  const int64_t i = row_of(e, nrhs);
+  const int c = col_of(e, nrhs);
+  const ColPack<VI, 4> a = *reinterpret_cast<const ColPack<VI, 4>*>(in + e);
+  ColPack<VO, 4> b;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) b.v[u] = cvt<VO>(a.v[u]);
+  *reinterpret_cast<ColPack<VO, 4>*>(out + (int64_t)perm[i] * nrhs + c) = b;
 }
 
 template <typename VI, typename VO>
 __global__ void k_scatter(int64_t n, int nrhs, const int* __restrict__ perm, const VI* __restrict__ in, VO* __restrict__ out) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * nrhs) return;
-  int64_t i = t / nrhs;
-  int c = (int)(t - i * nrhs);
+  const int64_t i = row_of(t, nrhs);
+  const int c = col_of(t, nrhs);
   out[(int64_t)perm[i] * nrhs + c] = cvt<VO>(in[t]);
 }
 
@@ -96,8 +145,10 @@ __global__ void k_spmvN(int nrows, int nrhs, const int* __restrict__ ia, const i
                         const VY* __restrict__ r, VACC alpha, VACC beta, int mode) {
   int64_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= (int64_t)nrows * nrhs) return;
-  int row = (int)(t / nrhs);
-  int c = (int)(t - (int64_t)row * nrhs);
+  const int row = (int)row_of(t, nrhs);
+  const int c = col_of(t, nrhs);
+  // the element this thread updates is loaded first, in flight with the row's entries
+  const VACC y0 = mode == 1 ? (VACC)r[t] : (beta == (VACC)0 ? VACC(0) : (VACC)y[t]);
   VACC acc = 0;
   const int k1 = ia[row + 1];
   // eight entries at a time: their indices and values, then their x gathers,
@@ -118,9 +169,58 @@ __global__ void k_spmvN(int nrows, int nrhs, const int* __restrict__ ia, const i
     for (int u = 0; u < 8; ++u)
       if (k + u < k1) acc += av[u] * xv[u];
   }
-  if (mode == 1) y[t] = cvt<VY>((VACC)r[t] - acc);
+  if (mode == 1) y[t] = cvt<VY>(y0 - acc);
   else if (beta == (VACC)0) y[t] = cvt<VY>(alpha * acc);
-  else y[t] = cvt<VY>(alpha * acc + beta * (VACC)y[t]);
+  else y[t] = cvt<VY>(alpha * acc + beta * y0);
+}
+
+// Four consecutive columns per thread (nrhs a multiple of 4, x, y and r
+// 16-byte aligned): the row's indices and values are loaded once for four
+// columns and every x gather and y access is 16 bytes or wider; each
+// column's sum runs over the entries in the same order as k_spmvN.
+template <typename VA, typename VX, typename VY, typename VACC>
+__global__ void k_spmvN4(int nrows, int nrhs, const int* __restrict__ ia, const int* __restrict__ ja,
+                         const VA* __restrict__ a, const VX* __restrict__ x, VY* __restrict__ y,
+                         const VY* __restrict__ r, VACC alpha, VACC beta, int mode) {
+  const int64_t e = 4 * (xcd_block() * blockDim.x + threadIdx.x);
+  if (e >= (int64_t)nrows * nrhs) return;
+  const int row = (int)row_of(e, nrhs);
+  const int c = col_of(e, nrhs);
+  using PY = ColPack<VY, 4>;
+  using PX = ColPack<VX, 4>;
+  PY y0;
+  if (mode == 1) y0 = *reinterpret_cast<const PY*>(r + e);
+  else if (!(beta == (VACC)0)) y0 = *reinterpret_cast<const PY*>(y + e);
+  VACC acc[4] = {VACC(0), VACC(0), VACC(0), VACC(0)};
+  const int k1 = ia[row + 1];
+  for (int k = ia[row]; k < k1; k += 8) {
+    int jj[8];
+    VACC av[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const bool in = k + u < k1;
+      jj[u] = in ? ja[k + u] : 0;
+      av[u] = in ? (VACC)a[k + u] : VACC(0);
+    }
+    PX xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k + u < k1) xv[u] = *reinterpret_cast<const PX*>(x + (int64_t)jj[u] * nrhs + c);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k + u < k1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] += av[u] * (VACC)xv[u].v[q];
+      }
+  }
+  PY out;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (mode == 1) out.v[q] = cvt<VY>((VACC)y0.v[q] - acc[q]);
+    else if (beta == (VACC)0) out.v[q] = cvt<VY>(alpha * acc[q]);
+    else out.v[q] = cvt<VY>(alpha * acc[q] + beta * (VACC)y0.v[q]);
+  }
+  *reinterpret_cast<PY*>(y + e) = out;
 }
 
 // (block-diagonal ILU solves: kernels_trsv.hpp)
@@ -175,7 +275,7 @@ __global__ __launch_bounds__(NT) void k_lr_dot_reg(int m, int k, int nrhs, const
   const int64_t total = (int64_t)m * nrhs;
   for (int64_t g = (int64_t)blockIdx.x * NT + tid; g < total; g += (int64_t)gridDim.x * NT) {
     const V zv = z[g];
-    const V* wr = W + (g / nrhs) * k;
+    const V* wr = W + row_of(g, nrhs) * k;
 #pragma unroll
     for (int j = 0; j < KM; ++j)
       if (j < k) acc[j] += sconj(wr[j]) * zv;
@@ -214,8 +314,8 @@ template <typename V>
 __global__ void k_lr_update(int m, int k, int nrhs, const V* __restrict__ WH, const V* __restrict__ t, V* __restrict__ z) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= (int64_t)m * nrhs) return;
-  int64_t i = g / nrhs;
-  int c = (int)(g - i * nrhs);
+  const int64_t i = row_of(g, nrhs);
+  const int c = col_of(g, nrhs);
   V acc = 0;
   for (int j = 0; j < k; ++j) acc += WH[i * k + j] * t[j * nrhs + c];
   z[g] += acc;
@@ -744,7 +844,7 @@ template <typename S>
 __global__ void k_scale_cols(int64_t n, int nrhs, const KState* __restrict__ st, S* __restrict__ v) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * nrhs) return;
-  v[g] *= S(st[g % nrhs].scale);
+  v[g] *= S(st[col_of(g, nrhs)].scale);
 }
 
 // FGMRES start (fgmres.hpp:318-359): normb, normr from squared norms.
@@ -779,7 +879,7 @@ template <typename S>
 __global__ void k_zero_cols(int64_t n, int nrhs, const double* __restrict__ flag, S* __restrict__ x) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * nrhs) return;
-  if (flag[g % nrhs] != 0.0) x[g] = S(0.0);
+  if (flag[col_of(g, nrhs)] != 0.0) x[g] = S(0.0);
 }
 
 // Start of a restart cycle (fgmres.hpp:382-386): rs[0] = normr, v0 /= normr.
@@ -892,7 +992,7 @@ __global__ void k_update_x(int64_t n, int nrhs, int p, const VZ* __restrict__ Z,
   __syncthreads();
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * nrhs) return;
-  const int c = (int)(g % nrhs);
+  const int c = col_of(g, nrhs);
   if (!upd[c]) return;
   S acc = (S)x[g];
   for (int j = 0; j < p; ++j) acc += cs[j * nrhs + c] * (S)Z[j * ldz + g];
