@@ -315,6 +315,46 @@ int pgs_gen_laplacian(int nx, int ny, int nz, double shift_re, double shift_im, 
 int pgs_gen_convdiff(int nx, int ny, int nz, double c, int* n, int64_t* nnz, int** ia, int** ja, double** a);
 void pgs_free(void* p);
 
+/* ---------------- parameter files ---------------- */
+
+#define PGS_INPUTS_SEQUENTIAL 0 /* the sequential drivers' 38-line `inputs` */
+#define PGS_INPUTS_HYPRE 1      /* the hypre interface's 31-line `inputs` */
+/* Reads a parameter file of the reference's drivers into params (a
+ * PGS_IO_SIZE array, typically pgs_params_default's) line by line:
+ * read_inputs_from_file of ParGeMSLR/TESTS/sequential/io.cpp:336-700
+ * (PGS_INPUTS_SEQUENTIAL) or of ParGeMSLR/TESTS/hypre/
+ * pargemslr_hypre_interface.cpp:391-820 (PGS_INPUTS_HYPRE), with the same
+ * keywords, line order and GLOBAL-to-LOCAL copies.  Returns PGS_ERROR_IO for a
+ * missing file or an unknown keyword (the lines before it stay applied).
+ * Options the setup does not implement (B solves other than ILUT, ND
+ * partitions, AMD/ND local orderings, the U-only level solve) are read
+ * faithfully; pgs_gemslr_setup then refuses them with
+ * PGS_ERROR_INVALID_OPTION. */
+int pgs_params_read_inputs(const char* path, int format, double* params);
+
+/* ---------------- hypre-style preconditioner object (one process) ---------------- */
+
+/* The life cycle of the reference's hypre interface
+ * (ParGeMSLR/TESTS/hypre/HYPRE_pargemslr.hpp:23-58: HYPRE_GEMSLRCreate /
+ * CreateFromFile, Setup, Solve, Destroy) without hypre's types.
+ * create reads inputs_file (NULL: "inputs" in the working directory) in the
+ * hypre layout and, like HYPRE_GEMSLRCreateFromFile, keeps the defaults for
+ * what it cannot read.  setup takes the diagonal block of the ParCSR matrix
+ * (hypre_CSRMatrixI/J/Data of hypre_ParCSRMatrixDiag; at one process it is
+ * the whole matrix), runs the GeMSLR setup with those parameters and uploads
+ * the operator to `device`.  solve applies the preconditioner once, x = M^{-1} b,
+ * on host arrays of n values (hypre_VectorData of the local vectors): what
+ * hypre's Krylov solvers call HYPRE_GEMSLRSolve for.  The apply is FP64 unless
+ * set_precision selects PGS_PREC_FP32.  At one process the operator is the
+ * sequential GemslrClass's (RKway levels), not ParallelGemslrClass's. */
+typedef struct pgs_hypre_gemslr_s* pgs_hypre_gemslr_t;
+int pgs_hypre_gemslr_create(const char* inputs_file, pgs_hypre_gemslr_t* out);
+int pgs_hypre_gemslr_params(pgs_hypre_gemslr_t s, double* params);
+int pgs_hypre_gemslr_set_precision(pgs_hypre_gemslr_t s, int prec);
+int pgs_hypre_gemslr_setup(pgs_hypre_gemslr_t s, int n, const int* ia, const int* ja, const double* a, int device);
+int pgs_hypre_gemslr_solve(pgs_hypre_gemslr_t s, const double* b, double* x);
+int pgs_hypre_gemslr_destroy(pgs_hypre_gemslr_t s);
+
 /* ---------------- MatrixMarket ingest ---------------- */
 
 /* CsrMatrixClass::ReadFromMMFile (ParGeMSLR/SRC/matrices/csr_matrix.cpp:

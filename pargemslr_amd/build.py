@@ -14,7 +14,8 @@ LIB = os.path.join(HERE, "libpargemslr_amd.so")
 
 HOST_SRC = ["host/partition.cpp", "host/rcm_ilut.cpp", "host/dense.cpp", "host/gemslr_setup.cpp",
             "host/gen.cpp", "host/trsv_pack.cpp", "host/capi_host.cpp", "host/capi_gen.cpp",
-            "host/mm_io.cpp", "host/op_store.cpp"]
+            "host/mm_io.cpp", "host/op_store.cpp",
+            "host/inputs.cpp", "host/hypre_shim.cpp"]
 HIP_SRC = ["device/dev_op.hip", "device/server.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -93,7 +93,9 @@ EXPORTED = [
     "pgs_memset_zero", "pgs_stream_create", "pgs_stream_destroy", "pgs_stream_synchronize", "pgs_device_synchronize",
     "pgs_memcpy_dtod", "pgs_dev_op_dims", "pgs_ws_estimate", "pgs_server_create", "pgs_server_add_operator", "pgs_server_submit",
     "pgs_server_wait", "pgs_server_destroy", "pgs_gemslr_matrix", "pgs_gemslr_save", "pgs_gemslr_load",
-    "pgs_read_mm_csr", "pgs_read_mm_vector",
+    "pgs_read_mm_csr", "pgs_read_mm_vector", "pgs_params_read_inputs", "pgs_hypre_gemslr_create",
+    "pgs_hypre_gemslr_params", "pgs_hypre_gemslr_set_precision", "pgs_hypre_gemslr_setup", "pgs_hypre_gemslr_solve",
+    "pgs_hypre_gemslr_destroy",
 ]
 
 _lib = None
@@ -123,6 +125,13 @@ def lib():
     L.pgs_read_mm_csr.argtypes = [C.c_char_p, C.c_int, C.c_int, ip, ip, C.POINTER(C.c_int64), C.POINTER(ip),
                                   C.POINTER(ip), C.POINTER(dp)]
     L.pgs_read_mm_vector.argtypes = [C.c_char_p, C.c_int, C.c_int, ip, C.POINTER(dp)]
+    L.pgs_params_read_inputs.argtypes = [C.c_char_p, C.c_int, dp]
+    L.pgs_hypre_gemslr_create.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.pgs_hypre_gemslr_params.argtypes = [vp, dp]
+    L.pgs_hypre_gemslr_set_precision.argtypes = [vp, C.c_int]
+    L.pgs_hypre_gemslr_setup.argtypes = [vp, C.c_int, ip, ip, dp, C.c_int]
+    L.pgs_hypre_gemslr_solve.argtypes = [vp, dp, dp]
+    L.pgs_hypre_gemslr_destroy.argtypes = [vp]
     L.pgs_device_count.argtypes = [ip]
     L.pgs_dev_op_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
     L.pgs_dev_op_destroy.argtypes = [vp]
@@ -248,6 +257,59 @@ def read_mm(path, complex_=False, idxin=1):
             lib().pgs_free(C.cast(p, C.c_void_p))
         raise PgsError(101, f"read_mm: {path} is {n.value} x {m.value}, not square")
     return _take_csr(n, nnz, ia, ja, a, complex_)
+
+
+PGS_INPUTS_SEQUENTIAL = 0
+PGS_INPUTS_HYPRE = 1
+
+
+def read_inputs(path, fmt=PGS_INPUTS_SEQUENTIAL, params=None):
+    """Parameter array from one of the reference drivers' `inputs` files
+    (pgs_params_read_inputs), starting from `params` or the defaults."""
+    p = default_params() if params is None else np.array(params, dtype=np.float64)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    _chk(lib().pgs_params_read_inputs(os.fsencode(path), fmt, _dptr(p)), "pgs_params_read_inputs")
+    return p
+
+
+class HypreGemslr:
+    """The hypre-style preconditioner object (pgs_hypre_gemslr_*): create from
+    an inputs file, setup on a CSR matrix, solve = one apply on host arrays."""
+
+    def __init__(self, inputs_file=None):
+        self._h = C.c_void_p()
+        _chk(lib().pgs_hypre_gemslr_create(None if inputs_file is None else os.fsencode(inputs_file),
+                                           C.byref(self._h)), "pgs_hypre_gemslr_create")
+
+    def params(self):
+        p = np.zeros(180)
+        _chk(lib().pgs_hypre_gemslr_params(self._h, _dptr(p)), "pgs_hypre_gemslr_params")
+        return p
+
+    def set_precision(self, prec):
+        _chk(lib().pgs_hypre_gemslr_set_precision(self._h, prec), "pgs_hypre_gemslr_set_precision")
+
+    def setup(self, A, device=0):
+        self.n = A.n
+        _chk(lib().pgs_hypre_gemslr_setup(self._h, A.n, _iptr(A.ia), _iptr(A.ja), _dptr(np.ascontiguousarray(A.a)),
+                                          device), "pgs_hypre_gemslr_setup")
+
+    def solve(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        _chk(lib().pgs_hypre_gemslr_solve(self._h, _dptr(b), _dptr(x)), "pgs_hypre_gemslr_solve")
+        return x
+
+    def close(self):
+        if self._h:
+            lib().pgs_hypre_gemslr_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def read_mm_vector(path, complex_=False, idxin=1):

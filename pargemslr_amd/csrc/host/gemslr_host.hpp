@@ -21,6 +21,9 @@ namespace pgs {
 // Typed view of the PARGEMSLR_IO_* entries this library reads
 // (ParGeMSLR/SRC/utils/utils.hpp:77-297, defaults utils.cpp:31-127).
 struct GemslrParams {
+  int bsolve_top = 0;    // B solve of the top levels: 0 = ILUT (the only option implemented)
+  int bsolve_lower = 0;  // B solve of the other levels: 0 = ILUT
+  int global_solve = 0;  // SolveLevel variant: 0 = LU (the only option implemented)
   int nlev = 5;
   int ncomp = 4;
   int partition = 1;     // 1 = RKway (the only option implemented)

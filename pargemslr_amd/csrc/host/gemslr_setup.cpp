@@ -83,6 +83,9 @@ void params_default_array(double* p) {
 }
 
 void params_from_array(const double* p, GemslrParams& g) {
+  g.bsolve_top = (int)p[1];
+  g.bsolve_lower = (int)p[3];
+  g.global_solve = (int)p[141];
   g.nlev = (int)p[20];
   g.ncomp = (int)p[22];
   g.partition = (int)p[24];
@@ -639,6 +642,24 @@ int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::s
   }
   if (prm.partition != 1) {
     if (msg) *msg = "only the RKway partition (option 1) is implemented";
+    return 100;
+  }
+  // options of the reference's parameter array this setup does not implement
+  // are refused rather than replaced (gemslr.hpp:2069-2161 reads them)
+  if (prm.bsolve_top != 0 || prm.bsolve_lower != 0) {
+    if (msg) *msg = "only ILUT block solves (B solve option 0) are implemented";
+    return 100;
+  }
+  if (prm.global_solve != 0) {
+    if (msg) *msg = "only the LU variant of the level solve (global solve option 0) is implemented";
+    return 100;
+  }
+  if (prm.perm_option != 0 && prm.perm_option != 1) {
+    if (msg) *msg = "only the NO and RCM local orderings (0, 1) are implemented";
+    return 100;
+  }
+  if ((prm.lr_rank1 > 0 && prm.lr_option1 != 1) || (prm.lr_rank2 > 0 && prm.lr_option2 != 1)) {
+    if (msg) *msg = "only thick-restart Arnoldi low-rank corrections (option 1) are implemented";
     return 100;
   }
   op = HostOp<T>();

@@ -44,6 +44,7 @@ inline uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
 // Calls v(field) for every parameter, in the file's order.
 template <class P, class V>
 void visit_params(P& p, V& v) {
+  v.i32(p.bsolve_top); v.i32(p.bsolve_lower); v.i32(p.global_solve);
   v.i32(p.nlev); v.i32(p.ncomp); v.i32(p.partition); v.i32(p.minsep); v.i32(p.kmin); v.i32(p.kfactor);
   v.flag(p.vertexsep);
   v.f64(p.droptol_B); v.f64(p.droptol_C);

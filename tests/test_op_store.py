@@ -12,10 +12,10 @@ import pargemslr_amd as pg
 from tests import problems as P
 
 # fixed-size parts of the file layout: magic + version + kind, the parameter
-# block (17 int32, 11 double, 2 uint8, 1 uint64), then n, nlev, solve_phase
+# block (20 int32, 11 double, 2 uint8, 1 uint64), then n, nlev, solve_phase
 # and the four setup times
 HEADER = 8 + 4 + 4
-PARAMS = 17 * 4 + 11 * 8 + 2 + 8
+PARAMS = 20 * 4 + 11 * 8 + 2 + 8
 SCALARS = 4 + 4 + 1 + 4 * 8
 
 
