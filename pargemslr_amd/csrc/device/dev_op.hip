@@ -1327,21 +1327,19 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
     S* h = ws->hbuf.as<S>();
     S* h2 = ws->h2buf.as<S>();
     double* t2 = ws->t2.as<double>();
-    if constexpr (!scalar_traits<S>::is_complex) {
-      if (dcgs2) {
-        // delayed CGS2: finalize v_{i-1} (and Hessenberg column i-2, with its
-        // Givens update) from the dots of this sweep, then project w once
-        const int p = i - 1;
-        S* d = ws->dc_d.as<S>();
-        S* coef = ws->dc_coef.as<S>();
-        dcgs2_dots<S>(ws, n, nrhs, p, V, ldv, w, d, ss);
-        k_dcgs2_fix<S><<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef);
-        dcgs2_update<S>(ws, n, nrhs, p, V, ldv, vprev, w, coef, t2, ss);
-        k_dcgs2_norm<S><<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st);
-        k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w);
-        HIPCHK(hipGetLastError());
-        return;
-      }
+    if (dcgs2) {
+      // delayed CGS2: finalize v_{i-1} (and Hessenberg column i-2, with its
+      // Givens update) from the dots of this sweep, then project w once
+      const int p = i - 1;
+      S* d = ws->dc_d.as<S>();
+      S* coef = ws->dc_coef.as<S>();
+      dcgs2_dots<S>(ws, n, nrhs, p, V, ldv, w, d, ss);
+      k_dcgs2_fix<S><<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef);
+      dcgs2_update<S>(ws, n, nrhs, p, V, ldv, vprev, w, coef, t2, ss);
+      k_dcgs2_norm<S><<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st);
+      k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w);
+      HIPCHK(hipGetLastError());
+      return;
     }
     if (o.orth == PGS_ORTH_CGS2 || o.orth == PGS_ORTH_DCGS2) {
       // PargemslrCgs2 (matrixops.cpp:10588-10709): two projection passes, the
@@ -1431,16 +1429,14 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
         }
       }
     }
-    if constexpr (!scalar_traits<S>::is_complex) {
-      if (dcgs2 && !none_active) {
-        // close the cycle: finalize the last basis vector and Hessenberg
-        // column of the columns still running (one more dot sweep)
-        S* d = ws->dc_d.as<S>();
-        dcgs2_dots<S>(ws, n, nrhs, i, V, ldv, nullptr, d, s);
-        k_dcgs2_fix<S><<<1, 64, 0, s>>>(nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st,
-                                        ws->dc_coef.as<S>());
-        HIPCHK(hipGetLastError());
-      }
+    if (dcgs2 && !none_active) {
+      // close the cycle: finalize the last basis vector and Hessenberg
+      // column of the columns still running (one more dot sweep)
+      S* d = ws->dc_d.as<S>();
+      dcgs2_dots<S>(ws, n, nrhs, i, V, ldv, nullptr, d, s);
+      k_dcgs2_fix<S><<<1, 64, 0, s>>>(nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st,
+                                      ws->dc_coef.as<S>());
+      HIPCHK(hipGetLastError());
     }
     // x += Z y
     k_backsolve<S><<<1, 64, 0, s>>>(nrhs, m, H, rs, st, ws->coef.as<S>(), ws->upd.as<int>());

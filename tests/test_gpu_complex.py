@@ -171,6 +171,24 @@ def test_complex_dcgs2_unpreconditioned_matches_oracle(kdim, cplx, gpu):
     assert rr == pytest.approx(rr_o, rel=1e-6)
 
 
+def test_complex_dcgs2_is_its_own_orthogonalization(cplx, gpu):
+    """Delayed CGS2 on a complex basis runs its own sweeps: same iteration
+    count as CGS2 (within one) and the same tolerance, but not the same bits
+    (a silent fallback to CGS2 would give identical solutions)."""
+    A, G, D, W = cplx
+    b = _crandn(43, A.n)
+    out = {}
+    for orth in (pg.PGS_ORTH_CGS2, pg.PGS_ORTH_DCGS2):
+        xd = _dev(np.zeros(A.n), gpu)
+        (st, it, _), = W.fgmres(_dev(b, gpu), xd, kdim=50, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP64, orth=orth)
+        assert st == 0
+        out[orth] = (_host(xd), it)
+    (x1, it1), (x2, it2) = out[pg.PGS_ORTH_CGS2], out[pg.PGS_ORTH_DCGS2]
+    assert abs(it1 - it2) <= 1
+    assert not np.array_equal(x1, x2)
+    assert np.linalg.norm(x1 - x2) <= 1e-6 * np.linalg.norm(x1)
+
+
 def test_complex_dcgs2_batched_equals_single(cplx, gpu):
     """Two complex columns with delayed CGS2 and the complex64 apply: each
     column reaches the tolerance and equals its single solve."""

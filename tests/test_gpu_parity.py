@@ -246,6 +246,24 @@ def test_fgmres_graph_replay_matches_direct(prec, orth, small, gpu):
     assert all(st == 0 for st, _ in outs[0][1]) and max(it for _, it in outs[0][1]) > 10
 
 
+def test_fgmres_dcgs2_is_its_own_orthogonalization(small, gpu):
+    """Delayed CGS2 runs its own sweeps: iteration count within one of CGS2's,
+    solutions within the tolerance of each other but not the same bits (a
+    silent fallback to CGS2 would give identical solutions)."""
+    A, G, D, W = small
+    b = pg.rng_normal(44, A.n)
+    out = {}
+    for orth in (pg.PGS_ORTH_CGS2, pg.PGS_ORTH_DCGS2):
+        xd = _dev(np.zeros(A.n), gpu)
+        (st, it, _), = W.fgmres(_dev(b, gpu), xd, kdim=50, maxits=500, tol=1e-8, prec=pg.PGS_PREC_FP64, orth=orth)
+        assert st == 0
+        out[orth] = (_host(xd), it)
+    (x1, it1), (x2, it2) = out[pg.PGS_ORTH_CGS2], out[pg.PGS_ORTH_DCGS2]
+    assert abs(it1 - it2) <= 1
+    assert not np.array_equal(x1, x2)
+    assert np.linalg.norm(x1 - x2) <= 1e-6 * np.linalg.norm(x1)
+
+
 def test_fgmres_dcgs2_batched_columns_match_single(small, gpu):
     """Delayed CGS2, mixed form, 4 right-hand sides (one zero): every column
     reaches the tolerance and matches its single solve; iterations within one
