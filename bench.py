@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # (FP32 apply), by right-hand sides per launch, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --probe --nrhs N`, each
 # scaled by what the counter reads for a 1 GiB device copy
-TRAFFIC_LEVEL0 = {32: 1704.8e6}
+TRAFFIC_LEVEL0 = {32: 1994.3e6}
 
 
 def log(*a):
