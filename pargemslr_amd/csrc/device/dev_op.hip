@@ -438,6 +438,8 @@ void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
       (T.max_block_ent + 1) * (int64_t)sizeof(TPair<V>) >= (int64_t)INT32_MAX ||
       (std::max(T.max_block_steps, T.max_block_steps_h) + 64) * 16 >= (int64_t)INT32_MAX)
     throw std::runtime_error("diagonal block too large for 32-bit offsets");
+  if (nrhs <= 0 || (nrhs & (nrhs - 1)) != 0)
+    throw std::runtime_error("the ring solve takes a power-of-two number of columns");
   int cg = std::min(nrhs, op->trsv_cg > 0 ? op->trsv_cg : 4);
   // Below one workgroup per CU each group's step chain is the bound, and a
   // narrower group has shorter steps: spread the columns over as many groups

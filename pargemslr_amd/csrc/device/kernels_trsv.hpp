@@ -418,6 +418,9 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   }
   const bool writer = q < CG;
   const int dummy = (wmask + 1) * CG;  // LDS slot written by lanes with nothing to write
+  // row * nrhs as a shift: the launch admits power-of-two column counts only
+  // (a 32-bit multiply is a quarter-rate instruction, two per step otherwise)
+  const int csh = __builtin_ctz((unsigned)nrhs);
 
   TPair<V> pe[D][EPL];
   TPair<V> pr[D];
@@ -438,7 +441,7 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
 #define PGS_RING_RHS(K)                                                           \
   do {                                                                            \
     _Pragma("unroll") for (int j_ = 0; j_ < NCL; ++j_) xr[K][j_] =                \
-        bload_val(rs_src, (pr[K].col * nrhs + c0 + cl[j_]) * VS, V(0));           \
+        bload_val(rs_src, ((pr[K].col << csh) + c0 + cl[j_]) * VS, V(0));         \
   } while (0)
 
   const int rel = s_begin - sbeg;
@@ -519,7 +522,7 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
         V v = xin[j] - a[j];
         if (UPPER) v *= dv;
         xs[wr ? slot * CG + cl[j] : dummy] = v;
-        bstore_val(rs_dst, wr ? (row * nrhs + c0 + cl[j]) * VS : 0x7ffffff0, v);
+        bstore_val(rs_dst, wr ? ((row << csh) + c0 + cl[j]) * VS : 0x7ffffff0, v);
       }
       // refill this stage with step s + D and load the right-hand side of
       // step s + D/2, whose row pair arrived D/2 steps ago.  The refill
