@@ -433,9 +433,11 @@ void launch_trsv_ring_cg(const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStre
 template <typename V>
 void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s) {
   // the kernel addresses a block's rows of all columns, its entries and its
-  // step descriptors with 32-bit byte offsets
-  if ((int64_t)T.max_rows * nrhs * (int64_t)sizeof(V) >= (int64_t)INT32_MAX ||
-      (T.max_block_ent + 1) * (int64_t)sizeof(TPair<V>) >= (int64_t)INT32_MAX ||
+  // step descriptors with 32-bit byte offsets; lanes with nothing to store
+  // aim at offset 0x7ffffff0, which must stay past the block's vector; lanes
+  // past a step's rows read up to 32 rows' slabs beyond the step
+  if ((int64_t)T.max_rows * nrhs * (int64_t)sizeof(V) > (int64_t)0x7ffffff0 ||
+      (T.max_block_ent + 32 * kRingWidth + 1) * (int64_t)sizeof(TPair<V>) >= (int64_t)INT32_MAX ||
       (std::max(T.max_block_steps, T.max_block_steps_h) + 64) * 16 >= (int64_t)INT32_MAX)
     throw std::runtime_error("diagonal block too large for 32-bit offsets");
   if (nrhs <= 0 || (nrhs & (nrhs - 1)) != 0)

@@ -357,7 +357,12 @@ __global__ __launch_bounds__(NT) void k_trsv_pipe(int nblocks, const int* __rest
 // (a row's y is read by the lanes that later write that row).  Rows are
 // prefetched D steps ahead (descriptor, row pair, slots); the right-hand side
 // of a step, whose address depends on the row pair, is loaded D/2 steps ahead.
-// Lane q of a row's LPR lanes finalizes columns q, q + LPR, ...
+// Lane q of a row's LPR lanes finalizes columns q, q + LPR, ...  Lanes past
+// a step's rows gather and write nothing.  With 8- and 16-byte values they
+// re-read the step's last row (cache lines already fetched); with 4-byte
+// values they read the following rows' pairs and slabs (valid ring slots, or
+// zeros past the block from the bounded buffers), which saves the clamp's
+// instructions in a loop bound by instruction issue.
 template <typename V, int CG>
 struct alignas(CG * sizeof(V) < 16 ? CG * sizeof(V) : 16) RingRow {
   V v[CG];
@@ -421,6 +426,7 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   // row * nrhs as a shift: the launch admits power-of-two column counts only
   // (a 32-bit multiply is a quarter-rate instruction, two per step otherwise)
   const int csh = __builtin_ctz((unsigned)nrhs);
+  constexpr bool kClampRows = sizeof(V) > 4;
 
   TPair<V> pe[D][EPL];
   TPair<V> pr[D];
@@ -431,9 +437,10 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
 #define PGS_RING_STAGE(K, DESC)                                                 \
   do {                                                                          \
     const int3 d_ = (DESC);                                                     \
-    const int tt_ = min(t0, max(d_.z - 1, 0));                                  \
-    sw[K] = t0 < d_.z ? ((d_.x + tt_) & wmask) : -1;                            \
-    pr[K] = bload_pair(rs_ri, (d_.x + tt_) * PS, V(0));                         \
+    const int tt_ = kClampRows ? min(t0, max(d_.z - 1, 0)) : t0;                \
+    const int r_ = d_.x + tt_;                                                  \
+    sw[K] = t0 < d_.z ? (r_ & wmask) : -1;                                      \
+    pr[K] = bload_pair(rs_ri, r_ * PS, V(0));                                   \
     const int eb_ = (d_.y + tt_ * RW + q) * PS;                                 \
     _Pragma("unroll") for (int e_ = 0; e_ < EPL; ++e_)                          \
       pe[K][e_] = bload_pair(rs_ent, eb_ + e_ * LPR * PS, V(0));                \
