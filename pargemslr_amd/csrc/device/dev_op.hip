@@ -167,7 +167,7 @@ void upload_trsv(DevTrsv& d, const TrsvPack<T>& P, int64_t* counter) {
   std::vector<TPair<V>> ent(P.col.size() + 1), ri(P.rows.size() + 1);
   for (size_t i = 0; i < P.col.size(); ++i) {
     ent[i] = TPair<V>();
-    ent[i].col = P.col[i];
+    ent[i].col = P.window > 0 ? P.col[i] * ring_unit<V>() : P.col[i];  // ring slots as 4-column row offsets
     ent[i].val = hval<V>(P.val[i]);
   }
   for (size_t i = 0; i < P.rows.size(); ++i) {
@@ -529,22 +529,32 @@ void set_lds_limit(K kern, int lds_max) {
   HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max));
 }
 
+// The ring kernel addresses its dynamic LDS from byte 0, which holds while it
+// has no static LDS.
+template <typename K>
+void set_ring_attrs(K kern, int lds_max) {
+  set_lds_limit(kern, lds_max);
+  hipFuncAttributes a;
+  HIPCHK(hipFuncGetAttributes(&a, (const void*)kern));
+  if (a.sharedSizeBytes != 0) throw std::runtime_error("ring solve kernel with static LDS");
+}
+
 template <typename V>
 void set_trsv_attrs(int lds_max) {
   constexpr int D1 = ring_depth<V, 1>(), D2 = ring_depth<V, 2>(), D4 = ring_depth<V, 4>();
-  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D1, 1, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D2, 2, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D4, 4, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D4, 4, 2>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D1, 1, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D2, 2, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D4, 4, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, (ring_depth<V, 1, 2>()), 1, 1>, lds_max);
-  set_lds_limit(k_trsv_ring<V, 64, 2, kRingWidth / 2, (ring_depth<V, 2, 2>()), 2, 1>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 128, 4, kRingWidth / 4, D1, 1, 1>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 128, 4, kRingWidth / 4, D2, 2, 1>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 128, 4, kRingWidth / 4, D4, 4, 1>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 128, 4, kRingWidth / 4, D4, 4, 2>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 64, 4, kRingWidth / 4, D1, 1, 1>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 64, 4, kRingWidth / 4, D2, 2, 1>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 64, 4, kRingWidth / 4, D4, 4, 1>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 64, 2, kRingWidth / 2, (ring_depth<V, 1, 2>()), 1, 1>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 64, 2, kRingWidth / 2, (ring_depth<V, 2, 2>()), 2, 1>, lds_max);
   if constexpr (sizeof(V) == 4) {
     constexpr int D8 = ring_depth<V, 8>();
-    set_lds_limit(k_trsv_ring<V, 128, 4, kRingWidth / 4, D8, 8, 1>, lds_max);
-    set_lds_limit(k_trsv_ring<V, 64, 4, kRingWidth / 4, D8, 8, 1>, lds_max);
+    set_ring_attrs(k_trsv_ring<V, 128, 4, kRingWidth / 4, D8, 8, 1>, lds_max);
+    set_ring_attrs(k_trsv_ring<V, 64, 4, kRingWidth / 4, D8, 8, 1>, lds_max);
   }
   if constexpr (scalar_traits<V>::is_complex) return;  // no pipelined kernel for complex values
   constexpr int DL2 = sizeof(V) == 4 ? 5 : 4;  // the pipelined kernel's stage depths (launch_trsv_pipe_nt)

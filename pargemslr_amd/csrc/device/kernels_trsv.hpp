@@ -368,6 +368,37 @@ struct alignas(CG * sizeof(V) < 16 ? CG * sizeof(V) : 16) RingRow {
   V v[CG];
 };
 
+// Ring-packed entries hold their ring slot times ring_unit<V>(): the byte
+// offset of the slot in a ring of 4-column rows, so that 4-column groups (the
+// default) gather without scaling the slot and the others scale it by one
+// shift.
+template <typename V>
+constexpr int ring_unit() {
+  return 4 * (int)sizeof(V);
+}
+template <typename V, int CG>
+__device__ __forceinline__ unsigned ring_byte(int c) {
+  constexpr int RB = CG * (int)sizeof(V), U = ring_unit<V>();
+  if constexpr (RB >= U) return (unsigned)c * (RB / U);
+  else return (unsigned)c / (unsigned)(U / RB);
+}
+
+// A ring row read through its LDS byte address.  The ring kernel has no
+// static LDS, so its dynamic LDS starts at address 0 (checked on the host
+// when the kernels are set up): the address is the slot's byte offset plus
+// the group's ring offset, and with one group the gather takes the entry's
+// column as its address unchanged.
+template <typename T>
+__device__ __forceinline__ T lds_read(unsigned addr) {
+  constexpr int N = (int)sizeof(T) / 4;
+  if constexpr (N == 1) {
+    return __builtin_bit_cast(T, *reinterpret_cast<const __attribute__((address_space(3))) unsigned*>((size_t)addr));
+  } else {
+    using W = unsigned __attribute__((ext_vector_type(N)));
+    return __builtin_bit_cast(T, *reinterpret_cast<const __attribute__((address_space(3))) W*>((size_t)addr));
+  }
+}
+
 __device__ __forceinline__ float bload_val(__amdgpu_buffer_rsrc_t r, int off, float) {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
@@ -397,10 +428,32 @@ __device__ __forceinline__ void bstore_val(__amdgpu_buffer_rsrc_t r, int off, cp
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, off, 0, 0);
 }
 
+// An empty asm that takes a value as read and rewritten: the value must have
+// arrived, and it counts as produced here rather than by its load.
+__device__ __forceinline__ void pin_reg(int& v) { __asm__ __volatile__("" : "+v"(v)); }
+__device__ __forceinline__ void pin_reg(float& v) { __asm__ __volatile__("" : "+v"(v)); }
+__device__ __forceinline__ void pin_reg(double& v) { __asm__ __volatile__("" : "+v"(v)); }
+template <typename R>
+__device__ __forceinline__ void pin_reg(cplx<R>& v) {
+  pin_reg(v.re);
+  pin_reg(v.im);
+}
+template <typename V>
+__device__ __forceinline__ void pin_reg(TPair<V>& p) {
+  pin_reg(p.col);
+  pin_reg(p.val);
+}
+__device__ __forceinline__ void pin_reg(int3& d) {
+  pin_reg(d.x);
+  pin_reg(d.y);
+  pin_reg(d.z);
+}
+
 template <typename V, int NT, int LPR, int EPL, int D, int CG, bool UPPER, int G>
 __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int wmask, int nrhs, int c0,
                                            __amdgpu_buffer_rsrc_t rs_desc, __amdgpu_buffer_rsrc_t rs_ri,
                                            __amdgpu_buffer_rsrc_t rs_ent, __amdgpu_buffer_rsrc_t rs_src, V* xs,
+                                           unsigned xsb,
                                            __amdgpu_buffer_rsrc_t rs_dst) {
   constexpr int PS = (int)sizeof(TPair<V>);
   constexpr int VS = (int)sizeof(V);
@@ -427,6 +480,13 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   // (a 32-bit multiply is a quarter-rate instruction, two per step otherwise)
   const int csh = __builtin_ctz((unsigned)nrhs);
   constexpr bool kClampRows = sizeof(V) > 4;
+  // Gathers through LDS byte addresses and the prologue's register pinning
+  // (below) make every geometry faster (level 0 at 128^3: 1 to 64 RHS 3-12 %
+  // less time) except one: the one-wave FP32 kernel with 4-column groups
+  // (the 32-RHS launch) runs 12 % slower with them (1.47 against 1.31 ms on
+  // one device), with or without the other, and at ring depths 8 and 10 as
+  // well.  That kernel keeps the generic gather and the plain prologue.
+  constexpr bool kLdsPath = !(NT == 64 && LPR == 4 && CG == 4 && sizeof(V) == 4);
 
   TPair<V> pe[D][EPL];
   TPair<V> pr[D];
@@ -434,16 +494,27 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   int sw[D];   // ring slot of this lane's row in the step held by stage k, -1 past the step's rows
   int3 dn[D];  // descriptor of the step stage k loads next
 
-#define PGS_RING_STAGE(K, DESC)                                                 \
+#define PGS_RING_PAIR(K, DESC)                                                  \
   do {                                                                          \
     const int3 d_ = (DESC);                                                     \
     const int tt_ = kClampRows ? min(t0, max(d_.z - 1, 0)) : t0;                \
     const int r_ = d_.x + tt_;                                                  \
     sw[K] = t0 < d_.z ? (r_ & wmask) : -1;                                      \
     pr[K] = bload_pair(rs_ri, r_ * PS, V(0));                                   \
+  } while (0)
+#define PGS_RING_ENT(K, DESC)                                                   \
+  do {                                                                          \
+    const int3 d_ = (DESC);                                                     \
+    const int tt_ = kClampRows ? min(t0, max(d_.z - 1, 0)) : t0;                \
     const int eb_ = (d_.y + tt_ * RW + q) * PS;                                 \
     _Pragma("unroll") for (int e_ = 0; e_ < EPL; ++e_)                          \
       pe[K][e_] = bload_pair(rs_ent, eb_ + e_ * LPR * PS, V(0));                \
+  } while (0)
+#define PGS_RING_STAGE(K, DESC) \
+  do {                          \
+    const int3 ds_ = (DESC);    \
+    PGS_RING_PAIR(K, ds_);      \
+    PGS_RING_ENT(K, ds_);       \
   } while (0)
 #define PGS_RING_RHS(K)                                                           \
   do {                                                                            \
@@ -459,6 +530,23 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   }
 #pragma unroll
   for (int k = 0; k < D / 2; ++k) PGS_RING_RHS(k);
+  // The compiler gives the loop's first steps the load waits of the worst
+  // path into them, and the path from here is the worst: the prologue's last
+  // loads are right-hand sides and slabs of the first steps, and every
+  // iteration would wait for nearly all loads in flight once per 2 D steps.
+  // Taking the prologue's registers as redefined here (one wait for its
+  // loads, once per sweep) leaves the loop the waits of its own back edge.
+#pragma unroll
+  for (int k = 0; k < (kLdsPath ? D : 0); ++k) {
+    if (k < D / 2) {
+#pragma unroll
+      for (int j = 0; j < NCL; ++j) pin_reg(xr[k][j]);
+    }
+    pin_reg(pr[k]);
+    pin_reg(dn[k]);
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) pin_reg(pe[k][e]);
+  }
 
   for (int base = s_begin; base < s_end; base += 2 * D) {
 #pragma unroll
@@ -471,7 +559,11 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
       V wgt[EPL];
 #pragma unroll
       for (int e = 0; e < EPL; ++e) {
-        xv[e] = reinterpret_cast<const RingRow<V, CG>*>(xs)[pe[k][e].col];
+        if constexpr (kLdsPath)
+          xv[e] = lds_read<RingRow<V, CG>>(xsb + ring_byte<V, CG>(pe[k][e].col));
+        else
+          xv[e] = *reinterpret_cast<const RingRow<V, CG>*>(reinterpret_cast<const unsigned char*>(xs) +
+                                                            ring_byte<V, CG>(pe[k][e].col));
         wgt[e] = pe[k][e].val;  // padded slots weigh zero
       }
       const int slot = sw[k];
@@ -552,6 +644,8 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
     }
   }
 #undef PGS_RING_STAGE
+#undef PGS_RING_PAIR
+#undef PGS_RING_ENT
 #undef PGS_RING_RHS
 }
 
@@ -570,6 +664,7 @@ __global__ __launch_bounds__(NT * G) void k_trsv_ring(int nblocks, int wmask, co
   extern __shared__ unsigned char smem_raw[];
   const int grp = threadIdx.x / NT;
   V* xs = reinterpret_cast<V*>(smem_raw) + grp * (wmask + 2) * CG;
+  const unsigned xsb = (unsigned)(grp * (wmask + 2) * CG * (int)sizeof(V));  // xs as an LDS byte address
   const int b = blockIdx.x % nblocks;
   const int c0 = (blockIdx.x / nblocks) * CG * G + grp * CG;
   const int r0 = blk_row0[b], nb = blk_row0[b + 1] - r0;
@@ -585,10 +680,10 @@ __global__ __launch_bounds__(NT * G) void k_trsv_ring(int nblocks, int wmask, co
   for (int i = threadIdx.x; i < (wmask + 2) * CG * G; i += NT * G) xall[i] = 0;  // rings + dummy slots
   __syncthreads();
   ring_sweep<V, NT, LPR, EPL, D, CG, false, G>(sbeg, sU, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
-                                               make_rsrc(rhs + (int64_t)r0 * nrhs, vbytes), xs,
+                                               make_rsrc(rhs + (int64_t)r0 * nrhs, vbytes), xs, xsb,
                                                make_rsrc(xb, vbytes));
   ring_sweep<V, NT, LPR, EPL, D, CG, true, G>(sU, send, sbeg, wmask, nrhs, c0, rs_desc, rs_ri, rs_ent,
-                                              make_rsrc(xb, vbytes), xs, make_rsrc(xb, vbytes));
+                                              make_rsrc(xb, vbytes), xs, xsb, make_rsrc(xb, vbytes));
 }
 
 }  // namespace dev
