@@ -110,6 +110,18 @@ int pgs_params_default(double* params);
 int pgs_gemslr_setup(int n, const int* ia, const int* ja, const double* a, int is_complex, const double* params,
                      uint64_t seed, pgs_gemslr_t* out);
 int pgs_gemslr_destroy(pgs_gemslr_t op);
+/* The same setup with the low-rank corrections built on a device (the setup
+ * acceleration of the reference's low-rank build, GemslrClass::Setup ->
+ * SetupLowRank*, ParGeMSLR/SRC/preconditioners/gemslr.cpp:2782-3257, which runs
+ * on the host there): partition, orderings and ILUT on the host, then per
+ * level, bottom first, the thick-restart Arnoldi process with its basis, the
+ * applies of G (E B^-1 F and C against the device solve of the levels below)
+ * and the orthogonalization on `device`, its small Hessenberg / Schur work on
+ * the host.  FP64 (complex<double>) arithmetic.  The result is a host operator
+ * like pgs_gemslr_setup's (its low-rank bases equal to rounding, not bit for
+ * bit); upload it with pgs_dev_op_create.  PGS_ERR_NO_DEVICE without a device. */
+int pgs_gemslr_setup_device(int n, const int* ia, const int* ja, const double* a, int is_complex,
+                            const double* params, uint64_t seed, int device, pgs_gemslr_t* out);
 
 typedef struct {
   int n;

@@ -85,7 +85,7 @@ class RequestInfo(C.Structure):
 
 # every extern "C" entry point of include/pargemslr_amd.h
 EXPORTED = [
-    "pgs_params_default", "pgs_gemslr_setup", "pgs_gemslr_destroy", "pgs_gemslr_info", "pgs_gemslr_level_view",
+    "pgs_params_default", "pgs_gemslr_setup", "pgs_gemslr_setup_device", "pgs_gemslr_destroy", "pgs_gemslr_info", "pgs_gemslr_level_view",
     "pgs_gemslr_perm", "pgs_gemslr_host_apply", "pgs_device_count", "pgs_dev_op_create", "pgs_dev_op_destroy",
     "pgs_dev_op_bytes", "pgs_ws_create", "pgs_ws_destroy", "pgs_spmv", "pgs_apply", "pgs_fgmres", "pgs_strerror",
     "pgs_last_hip_error", "pgs_gen_laplacian", "pgs_gen_convdiff", "pgs_free", "pgs_rng_uniform", "pgs_rng_normal",
@@ -114,6 +114,7 @@ def lib():
     dp = C.POINTER(C.c_double)
     L.pgs_params_default.argtypes = [dp]
     L.pgs_gemslr_setup.argtypes = [C.c_int, ip, ip, dp, C.c_int, dp, C.c_uint64, C.POINTER(vp)]
+    L.pgs_gemslr_setup_device.argtypes = [C.c_int, ip, ip, dp, C.c_int, dp, C.c_uint64, C.c_int, C.POINTER(vp)]
     L.pgs_gemslr_destroy.argtypes = [vp]
     L.pgs_gemslr_info.argtypes = [vp, C.POINTER(GemslrInfo)]
     L.pgs_gemslr_level_view.argtypes = [vp, C.c_int, C.POINTER(LevelView)]
@@ -364,15 +365,22 @@ def rng_normal(seed, n):
 class Gemslr:
     """Host-resident GeMSLR operator (GemslrClass after Setup)."""
 
-    def __init__(self, A, params, seed=0):
+    def __init__(self, A, params, seed=0, lowrank_device=None):
+        """lowrank_device: None builds the low-rank corrections on the host,
+        a device index builds them on that device (pgs_gemslr_setup_device)."""
         self.A = A
         self._h = C.c_void_p()
         params = np.ascontiguousarray(params, dtype=np.float64)
         self.params = params.copy()
         a = A.a.view(np.float64) if A.is_complex else A.a
         a = np.ascontiguousarray(a)
-        _chk(lib().pgs_gemslr_setup(A.n, _iptr(A.ia), _iptr(A.ja), _dptr(a), int(A.is_complex), _dptr(params),
-                                    C.c_uint64(seed), C.byref(self._h)), "pgs_gemslr_setup")
+        if lowrank_device is None:
+            _chk(lib().pgs_gemslr_setup(A.n, _iptr(A.ia), _iptr(A.ja), _dptr(a), int(A.is_complex),
+                                        _dptr(params), C.c_uint64(seed), C.byref(self._h)), "pgs_gemslr_setup")
+        else:
+            _chk(lib().pgs_gemslr_setup_device(A.n, _iptr(A.ia), _iptr(A.ja), _dptr(a), int(A.is_complex),
+                                               _dptr(params), C.c_uint64(seed), int(lowrank_device),
+                                               C.byref(self._h)), "pgs_gemslr_setup_device")
 
     @classmethod
     def load(cls, path):

@@ -812,7 +812,7 @@ void apply_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const VI* v, VZ* z, int nrhs, hi
 // solves, E, F, W, WH) in the precisions of prec_mask: V64 for PGS_PREC_FP64,
 // V32 for PGS_PREC_FP32.
 template <typename V64, typename V32, typename T>
-void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask) {
+void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask, bool all_c = false) {
   op->n = H.n;
   op->nlev = H.nlev;
   op->lev_ptr = H.lev_ptr;
@@ -855,7 +855,9 @@ void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask) {
           upload(L->W, to_row_major<V>(HL.W, L->nrem, HL.rank), &op->bytes);
           upload(L->WH, to_row_major<V>(HL.WH, L->nrem, HL.rank), &op->bytes);
         }
-        if (l == 0 && op->inner_maxits > 0) upload_csr<V>(L->C, HL.C, &op->bytes);
+        // C of level 0 for the inner Schur iteration; every level's C for the
+        // device low-rank build of a setup
+        if ((l == 0 && op->inner_maxits > 0) || (all_c && HL.C.nrows > 0)) upload_csr<V>(L->C, HL.C, &op->bytes);
       }
       P.levels.push_back(std::move(L));
     }
@@ -1629,3 +1631,176 @@ extern "C" int pgs_ws_profile_read(pgs_ws_t w, pgs_kernel_profile_t* out) {
   out->total_bytes = ws->prof_bytes;
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Low-rank build of the setup on the device (f2 of the solve path's scope:
+// GemslrClass::SetupLowRank*, gemslr.cpp:2782-3257).  The thick-restart
+// Arnoldi driver (Hessenberg matrix, Schur forms, restart choices) runs on
+// the host as in the host setup; the basis and every m-length operation live
+// on the device: G x = (I - C Ct^{-1}) x + E B^{-1} F Ct^{-1} x with Ct^{-1}
+// the device SolveLevel of the levels below (whose corrections were built
+// and uploaded first), CGS2 by multidot / multi_axpy, and the restart and
+// final bases W = V Y, WH = W R.  The arithmetic is FP64 (complex<double>).
+
+namespace {
+
+// out(:, j) = sum_q X(:, q) Y(q, j), j < p, q < k; X column-major with
+// leading dimension m, Y with leading dimension ldy
+template <typename V>
+__global__ __launch_bounds__(256) void k_lr_combine(int64_t m, int k, int p, const V* __restrict__ X,
+                                                    const V* __restrict__ Y, int ldy, V* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int j = 0; j < p; ++j) {
+    V s = V(0);
+    for (int q = 0; q < k; ++q) s += X[i + q * m] * Y[q + (int64_t)j * ldy];
+    out[i + j * m] = s;
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_lr_divide(int64_t m, V* __restrict__ x, double d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) x[i] = x[i] / V(d);
+}
+
+template <typename T, typename V>
+struct DevLrBackend final : LrBackend<T> {
+  static_assert(sizeof(T) == sizeof(V), "host and device values share their layout");
+  static constexpr int kChunk = 64;  // basis vectors per multidot / multi_axpy launch
+  pgs_dev_op_s* op;
+  pgs_ws_t wsh = nullptr;
+  pgs_ws_s* ws = nullptr;
+  hipStream_t s = nullptr;
+  int l = 0, ncols = 0;
+  int64_t m = 0;
+  DevMem Vb, Tb, zl, zu, wu, hd, yd, nd;
+  explicit DevLrBackend(pgs_dev_op_s* o) : op(o) {}
+  ~DevLrBackend() override {
+    if (wsh) pgs_ws_destroy(wsh);
+  }
+  V* col(int j) const { return Vb.as<V>() + (int64_t)j * m; }
+  const DevLevel& level() const { return *prec_of<V>(op).levels[l]; }
+  void begin(int lev, int mm, int nc) override {
+    l = lev;
+    m = mm;
+    ncols = nc;
+    const int e = pgs_ws_create(reinterpret_cast<pgs_dev_op_t>(op), 1, 1, &wsh);
+    if (e) throw HipError{e == PGS_ERR_OOM ? hipErrorOutOfMemory : hipErrorUnknown};
+    ws = reinterpret_cast<pgs_ws_s*>(wsh);
+    Vb.alloc(sizeof(V) * (size_t)m * nc);
+    Tb.alloc(sizeof(V) * (size_t)m * nc);
+    zl.alloc(sizeof(V) * (size_t)m);
+    zu.alloc(sizeof(V) * (size_t)std::max(1, level().nloc));
+    wu.alloc(sizeof(V) * (size_t)std::max(1, level().nloc));
+    hd.alloc(sizeof(V) * (size_t)nc);
+    yd.alloc(sizeof(V) * (size_t)nc * nc);
+    nd.alloc(sizeof(double));
+  }
+  void set_col(int j, const T* x) override {
+    HIPCHK(hipMemcpy(col(j), x, sizeof(V) * m, hipMemcpyHostToDevice));
+  }
+  void apply(int j) override {
+    const DevLevel& L = level();
+    const V* x = col(j);
+    V* y = col(j + 1);
+    solve_level<V>(op, ws, prec_of<V>(op), l + 1, x, zl.as<V>(), 1, s);
+    launch_spmv<V, V, V, V>(L.C, zl.as<V>(), y, x, V(1.0), V(0.0), 1, 1, s);  // y = x - C zl
+    launch_spmv<V, V, V, V>(L.F, zl.as<V>(), zu.as<V>(), (const V*)nullptr, V(1.0), V(0.0), 0, 1, s);
+    launch_trsv<V>(op, ws, l, L.B, zu.as<V>(), wu.as<V>(), 1, s);
+    launch_spmv<V, V, V, V>(L.E, wu.as<V>(), y, (const V*)nullptr, V(1.0), V(1.0), 0, 1, s);  // y += E wu
+  }
+  void dots(int j, T* h) override {
+    for (int q0 = 0; q0 <= j; q0 += kChunk) {
+      const int pc = std::min(kChunk, j + 1 - q0);
+      multidot<V, V, V>(ws, m, 1, pc, col(q0), m, col(j + 1), hd.as<V>() + q0, 0, s);
+    }
+    HIPCHK(hipMemcpy(h, hd.p, sizeof(V) * (size_t)(j + 1), hipMemcpyDeviceToHost));
+  }
+  void update(int j, const T* h) override {
+    HIPCHK(hipMemcpy(hd.p, h, sizeof(V) * (size_t)(j + 1), hipMemcpyHostToDevice));
+    for (int q0 = 0; q0 <= j; q0 += kChunk) {
+      const int pc = std::min(kChunk, j + 1 - q0);
+      multi_axpy<V, V>(ws, m, 1, pc, col(q0), m, hd.as<V>() + q0, col(j + 1), nullptr, s);
+    }
+  }
+  double norm(int j) override {
+    colnorm2<V>(ws, m, 1, col(j), nd.as<double>(), s);
+    double v = 0.0;
+    HIPCHK(hipMemcpy(&v, nd.p, sizeof(double), hipMemcpyDeviceToHost));
+    return std::sqrt(v);
+  }
+  void divide(int j, double d) override {
+    k_lr_divide<V><<<nblocks_for(m, 256), 256, 0, s>>>(m, col(j), d);
+    HIPCHK(hipGetLastError());
+  }
+  void combine(const V* X, int k, int p, const T* Y, int ldy, V* out) {
+    HIPCHK(hipMemcpy(yd.p, Y, sizeof(V) * (size_t)ldy * p, hipMemcpyHostToDevice));
+    k_lr_combine<V><<<nblocks_for(m, 256), 256, 0, s>>>(m, k, p, X, yd.as<V>(), ldy, out);
+    HIPCHK(hipGetLastError());
+  }
+  void restart(int mcur, int p, const T* Y) override {
+    combine(Vb.as<V>(), mcur, p, Y, mcur, Tb.as<V>());
+    HIPCHK(hipMemcpy(col(p), col(mcur), sizeof(V) * m, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(Vb.p, Tb.p, sizeof(V) * m * p, hipMemcpyDeviceToDevice));
+  }
+  void finish(int mcur, int r, const T* Y, const T* R, std::vector<T>& W, std::vector<T>& WH) override {
+    combine(Vb.as<V>(), mcur, r, Y, mcur, Tb.as<V>());  // W
+    combine(Tb.as<V>(), r, r, R, r, Vb.as<V>());       // WH = W R
+    W.resize((size_t)m * r);
+    WH.resize((size_t)m * r);
+    HIPCHK(hipMemcpy(W.data(), Tb.p, sizeof(V) * m * r, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(WH.data(), Vb.p, sizeof(V) * m * r, hipMemcpyDeviceToHost));
+  }
+};
+
+template <typename T, typename V64, typename V32>
+int lowrank_build_device(HostOp<T>& H, int device, std::string* msg) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) {
+    if (msg) *msg = "no such device for the low-rank build";
+    return PGS_ERR_NO_DEVICE;
+  }
+  try {
+    std::unique_ptr<pgs_dev_op_s> op(new pgs_dev_op_s());
+    HIPCHK(hipSetDevice(device));
+    op->device = device;
+    int lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    op->lds_max = lds;
+    HIPCHK(hipDeviceGetAttribute(&op->num_cus, hipDeviceAttributeMultiprocessorCount, device));
+    set_kernel_attrs(lds);
+    op->is_complex = scalar_traits<V64>::is_complex ? 1 : 0;
+    upload_op<V64, V32>(op.get(), H, PGS_PREC_FP64, true);
+    for (int l = H.nlev - 2; l >= 0; --l) {
+      {
+        DevLrBackend<T, V64> be(op.get());
+        lowrank_level_with(H, l, be);
+      }
+      // the correction of level l takes part in the operator of level l - 1
+      const HostLevel<T>& HL = H.levels[l];
+      DevLevel& L = *prec_of<V64>(op.get()).levels[l];
+      L.rank = HL.rank;
+      if (HL.rank > 0) {
+        upload(L.W, to_row_major<V64>(HL.W, L.nrem, HL.rank), &op->bytes);
+        upload(L.WH, to_row_major<V64>(HL.WH, L.nrem, HL.rank), &op->bytes);
+      }
+    }
+    HIPCHK(hipDeviceSynchronize());
+  } catch (const HipError& e) {
+    if (msg) *msg = std::string("device low-rank build: ") + hipGetErrorString(e.e);
+    return set_err(e);
+  } catch (const std::bad_alloc&) {
+    return PGS_ERR_OOM;
+  }
+  return 0;
+}
+
+struct LowrankDeviceInstall {
+  LowrankDeviceInstall() {
+    g_lowrank_device = &lowrank_build_device<double, double, float>;
+    g_lowrank_device_z = &lowrank_build_device<zdouble, cplx<double>, cplx<float>>;
+  }
+} g_lowrank_device_install;
+
+}  // namespace

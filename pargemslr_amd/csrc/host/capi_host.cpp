@@ -17,8 +17,8 @@ int pgs_params_default(double* params) {
   return 0;
 }
 
-int pgs_gemslr_setup(int n, const int* ia, const int* ja, const double* a, int is_complex, const double* params,
-                     uint64_t seed, pgs_gemslr_t* out) {
+static int gemslr_setup_impl(int n, const int* ia, const int* ja, const double* a, int is_complex,
+                             const double* params, uint64_t seed, int lowrank_device, pgs_gemslr_t* out) {
   if (!out) return PGS_ERROR_INVALID_PARAM;
   *out = nullptr;
   if (n < 0 || (n > 0 && (!ia || !ja || !a)) || !params) return PGS_ERROR_INVALID_PARAM;
@@ -31,6 +31,7 @@ int pgs_gemslr_setup(int n, const int* ia, const int* ja, const double* a, int i
   GemslrParams prm;
   params_from_array(params, prm);
   prm.seed = seed;
+  prm.lowrank_device = lowrank_device;
   // inner Schur iteration: at least one and at most kSchurMaxIts steps (the
   // device keeps its Krylov basis in one workspace allocation)
   if (prm.schur_enable != 0 && (prm.schur_maxits < 1 || prm.schur_maxits > PGS_SCHUR_MAXITS_LIMIT ||
@@ -72,6 +73,20 @@ int pgs_gemslr_setup(int n, const int* ia, const int* ja, const double* a, int i
   }
   *out = reinterpret_cast<pgs_gemslr_t>(h);
   return 0;
+}
+
+int pgs_gemslr_setup(int n, const int* ia, const int* ja, const double* a, int is_complex, const double* params,
+                     uint64_t seed, pgs_gemslr_t* out) {
+  return gemslr_setup_impl(n, ia, ja, a, is_complex, params, seed, -1, out);
+}
+
+int pgs_gemslr_setup_device(int n, const int* ia, const int* ja, const double* a, int is_complex,
+                            const double* params, uint64_t seed, int device, pgs_gemslr_t* out) {
+  if (device < 0) {
+    if (out) *out = nullptr;
+    return PGS_ERROR_INVALID_PARAM;
+  }
+  return gemslr_setup_impl(n, ia, ja, a, is_complex, params, seed, device, out);
 }
 
 int pgs_gemslr_destroy(pgs_gemslr_t op) {

@@ -49,6 +49,7 @@ struct GemslrParams {
   double schur_tol = 1e-2;
   int schur_maxits = 5;
   uint64_t seed = 0;
+  int lowrank_device = -1;  // >= 0: the low-rank corrections are built on this device
 };
 
 template <typename T>
@@ -85,6 +86,37 @@ struct HostOp {
 // Returns 0 or an error code (PARGEMSLR_ERROR_* numbering).
 template <typename T>
 int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::string* msg = nullptr);
+
+// The m-length vector work of the thick-restart Arnoldi low-rank build of
+// level l: basis columns V(:, 0..ncols-1) of length m = nrem(l) and the
+// operator G x = (I - C Ct^{-1}) x + E B^{-1} F Ct^{-1} x
+// (GemslrClass::EBFCMatVec, gemslr.cpp:3737-3792).  The setup's host code and
+// the device build supply it; lowrank_level_with drives it (the Hessenberg
+// matrix, Schur forms and restart choices stay on the host).
+template <typename T>
+struct LrBackend {
+  virtual ~LrBackend() = default;
+  virtual void begin(int l, int m, int ncols) = 0;
+  virtual void set_col(int j, const T* x) = 0;            // V(:,j) = x (host array of m values)
+  virtual void apply(int j) = 0;                          // V(:,j+1) = G V(:,j)
+  virtual void dots(int j, T* h) = 0;                     // h[q] = V(:,q)^H V(:,j+1), q = 0..j (host array)
+  virtual void update(int j, const T* h) = 0;             // V(:,j+1) -= V(:,0:j) h
+  virtual double norm(int j) = 0;                         // ||V(:,j)||
+  virtual void divide(int j, double d) = 0;               // V(:,j) /= d
+  virtual void restart(int mcur, int p, const T* Y) = 0;  // V(:,0:p) = V(:,0:mcur) Y, then V(:,p) = V(:,mcur)
+  // W = V(:,0:mcur) Y and WH = W R, column-major m x r host arrays
+  virtual void finish(int mcur, int r, const T* Y, const T* R, std::vector<T>& W, std::vector<T>& WH) = 0;
+};
+template <typename T>
+int lowrank_level_with(HostOp<T>& op, int l, LrBackend<T>& be);
+
+// Low-rank build on a device (installed by the device code when the library
+// has it): the corrections of every level of a setup with lowrank_device >= 0,
+// bottom level first.  Returns 0 or an error code.
+using LowrankDeviceFn = int (*)(HostOp<double>&, int, std::string*);
+using LowrankDeviceFnZ = int (*)(HostOp<zdouble>&, int, std::string*);
+extern LowrankDeviceFn g_lowrank_device;
+extern LowrankDeviceFnZ g_lowrank_device_z;
 
 // Lcat / Ucat / Dcat of level l from its per-block factors (part of the setup;
 // also run when an operator is read back from a file).

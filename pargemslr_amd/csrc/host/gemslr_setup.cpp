@@ -417,8 +417,78 @@ void pick(const std::vector<dense::cplx>& lam, const std::vector<int>& cand, int
   }
 }
 
+// The m-length vector work of the low-rank build on the host: the basis in
+// host memory, G applied by EbfcOp.
 template <typename T>
-int lowrank_level(HostOp<T>& op, int l) {
+struct HostLrBackend final : LrBackend<T> {
+  HostOp<T>& op;
+  int l = 0, m = 0;
+  std::unique_ptr<EbfcOp<T>> G;
+  std::vector<T> V;
+  explicit HostLrBackend(HostOp<T>& o) : op(o) {}
+  void begin(int lev, int mm, int ncols) override {
+    l = lev;
+    m = mm;
+    G.reset(new EbfcOp<T>(op, l));
+    V.assign((size_t)m * ncols, T(0));
+  }
+  T* col(int j) { return &V[(size_t)j * m]; }
+  void set_col(int j, const T* x) override { std::copy(x, x + m, col(j)); }
+  void apply(int j) override { G->apply(col(j), col(j + 1)); }
+  void dots(int j, T* h) override {
+    const T* w = col(j + 1);
+    parallel_for(j + 1, [&](int q) { h[q] = vdot(m, col(q), w); });
+  }
+  void update(int j, const T* h) override {
+    T* w = col(j + 1);
+    for (int q = 0; q <= j; ++q) {
+      const T* vq = col(q);
+      T c = h[q];
+      for (int i = 0; i < m; ++i) w[i] -= c * vq[i];
+    }
+  }
+  double norm(int j) override { return std::sqrt(sabs(vdot(m, col(j), col(j)))); }
+  void divide(int j, double d) override {
+    T* w = col(j);
+    for (int i = 0; i < m; ++i) w[i] /= d;
+  }
+  // X Y with X the first k columns of V (zero coefficients skipped)
+  void combine(const T* X, int k, int p, const T* Y, int ldy, std::vector<T>& out) {
+    out.assign((size_t)m * p, T(0));
+    for (int j = 0; j < p; ++j)
+      for (int q = 0; q < k; ++q) {
+        T c = Y[q + (size_t)j * ldy];
+        if (c == T(0)) continue;
+        const T* xq = X + (size_t)q * m;
+        T* dst = &out[(size_t)j * m];
+        for (int i = 0; i < m; ++i) dst[i] += c * xq[i];
+      }
+  }
+  void restart(int mcur, int p, const T* Y) override {
+    std::vector<T> Vn;
+    combine(V.data(), mcur, p, Y, mcur, Vn);
+    std::memmove(col(p), col(mcur), sizeof(T) * m);
+    std::memcpy(V.data(), Vn.data(), sizeof(T) * m * p);
+  }
+  void finish(int mcur, int r, const T* Y, const T* R, std::vector<T>& W, std::vector<T>& WH) override {
+    combine(V.data(), mcur, r, Y, mcur, W);
+    WH.assign((size_t)m * r, T(0));
+    for (int j = 0; j < r; ++j)
+      for (int q = 0; q < r; ++q) {
+        T c = R[q + (size_t)j * r];
+        const T* wq = &W[(size_t)q * m];
+        T* dst = &WH[(size_t)j * m];
+        for (int i = 0; i < m; ++i) dst[i] += wq[i] * c;
+      }
+  }
+};
+
+}  // namespace
+
+// Thick-restart Arnoldi on G of level l (GemslrClass::SetupLowRankSubspace /
+// thick restart, gemslr.cpp:2782-3257), the m-length work done by `be`.
+template <typename T>
+int lowrank_level_with(HostOp<T>& op, int l, LrBackend<T>& be) {
   HostLevel<T>& L = op.levels[l];
   const GemslrParams& P = op.prm;
   const int m = op.nrem(l);
@@ -443,37 +513,33 @@ int lowrank_level(HostOp<T>& op, int l) {
   if (maxsteps <= 0) return 0;
   const double tol_orth = std::max(P.lr_tol_orth, 2.220446049250313e-16);
 
-  EbfcOp<T> G(op, l);
-  std::vector<T> V((size_t)m * (maxsteps + 1), T(0));
+  be.begin(l, m, maxsteps + 1);
   std::vector<T> H((size_t)(maxsteps + 1) * maxsteps, T(0));
   const int ldh = maxsteps + 1;
   auto Hat = [&](int i, int j) -> T& { return H[i + (size_t)j * ldh]; };
   // starting vector
   {
+    std::vector<T> v0(m);
     uint64_t s = P.seed ^ (0xA5A5ULL + 977ULL * (uint64_t)l);
-    for (int i = 0; i < m; ++i) V[i] = P.lr_rand_init ? T(unif01(s)) : T(1.0);
-    double nv = std::sqrt(sabs(vdot(m, V.data(), V.data())));
-    for (int i = 0; i < m; ++i) V[i] /= nv;
+    for (int i = 0; i < m; ++i) v0[i] = P.lr_rand_init ? T(unif01(s)) : T(1.0);
+    double nv = std::sqrt(sabs(vdot(m, v0.data(), v0.data())));
+    for (int i = 0; i < m; ++i) v0[i] /= nv;
+    be.set_col(0, v0.data());
   }
   std::vector<T> h(maxsteps + 1), h2(maxsteps + 1);
+  // CGS2: two rounds of dots and updates per step
   auto arnoldi = [&](int j0, int j1) -> int {
     for (int j = j0; j < j1; ++j) {
-      T* w = &V[(size_t)(j + 1) * m];
-      G.apply(&V[(size_t)j * m], w);
-      for (int pass = 0; pass < 2; ++pass) {
-        std::vector<T>& hh = pass == 0 ? h : h2;
-        parallel_for(j + 1, [&](int q) { hh[q] = vdot(m, &V[(size_t)q * m], w); });
-        for (int q = 0; q <= j; ++q) {
-          const T* vq = &V[(size_t)q * m];
-          T c = hh[q];
-          for (int i = 0; i < m; ++i) w[i] -= c * vq[i];
-        }
-      }
+      be.apply(j);
+      be.dots(j, h.data());
+      be.update(j, h.data());
+      be.dots(j, h2.data());
+      be.update(j, h2.data());
       for (int q = 0; q <= j; ++q) Hat(q, j) = h[q] + h2[q];
-      double beta = std::sqrt(sabs(vdot(m, w, w)));
+      double beta = be.norm(j + 1);
       Hat(j + 1, j) = T(beta);
       if (beta < tol_orth) return j + 1;
-      for (int i = 0; i < m; ++i) w[i] /= beta;
+      be.divide(j + 1, beta);
     }
     return j1;
   };
@@ -522,17 +588,7 @@ int lowrank_level(HostOp<T>& op, int l) {
     int p = invariant_basis<T>(mcur, Tm, Qm, sel, Y);
     if (p <= 0 || p >= mcur) break;
     // V(:,0:p) = Vm Y ; V(:,p) = v_{m+1}
-    std::vector<T> Vn((size_t)m * p, T(0));
-    for (int j = 0; j < p; ++j)
-      for (int q = 0; q < mcur; ++q) {
-        T c = Y[q + (size_t)j * mcur];
-        if (c == T(0)) continue;
-        const T* vq = &V[(size_t)q * m];
-        T* dst = &Vn[(size_t)j * m];
-        for (int i = 0; i < m; ++i) dst[i] += c * vq[i];
-      }
-    std::memmove(&V[(size_t)p * m], &V[(size_t)mcur * m], sizeof(T) * m);
-    std::memcpy(V.data(), Vn.data(), sizeof(T) * m * p);
+    be.restart(mcur, p, Y.data());
     // H = [Y^H Hm Y ; h_last * Y(m-1,:)]
     std::vector<T> HY((size_t)mcur * p, T(0));
     for (int j = 0; j < p; ++j)
@@ -569,15 +625,6 @@ int lowrank_level(HostOp<T>& op, int l) {
   int r = invariant_basis<T>(mcur, Tm, Qm, sel, Y);
   if (r <= 0) return 0;
   // W = Vm Y ; Tr = Y^H Hm Y ; Hk = (I - Tr)^{-1} - I ; WH = W Hk
-  L.W.assign((size_t)m * r, T(0));
-  for (int j = 0; j < r; ++j)
-    for (int q = 0; q < mcur; ++q) {
-      T c = Y[q + (size_t)j * mcur];
-      if (c == T(0)) continue;
-      const T* vq = &V[(size_t)q * m];
-      T* dst = &L.W[(size_t)j * m];
-      for (int i = 0; i < m; ++i) dst[i] += c * vq[i];
-    }
   std::vector<T> R((size_t)r * r, T(0));
   for (int j = 0; j < r; ++j)
     for (int i = 0; i < r; ++i) {
@@ -592,16 +639,17 @@ int lowrank_level(HostOp<T>& op, int l) {
   if (dense::invert(r, R) != 0) return 0;
   for (int i = 0; i < r; ++i) R[i + (size_t)i * r] -= T(1);
   L.Hk = R;
-  L.WH.assign((size_t)m * r, T(0));
-  for (int j = 0; j < r; ++j)
-    for (int q = 0; q < r; ++q) {
-      T c = R[q + (size_t)j * r];
-      const T* wq = &L.W[(size_t)q * m];
-      T* dst = &L.WH[(size_t)j * m];
-      for (int i = 0; i < m; ++i) dst[i] += wq[i] * c;
-    }
+  be.finish(mcur, r, Y.data(), R.data(), L.W, L.WH);
   L.rank = r;
   return r;
+}
+
+namespace {
+
+template <typename T>
+int lowrank_level(HostOp<T>& op, int l) {
+  HostLrBackend<T> be(op);
+  return lowrank_level_with(op, l, be);
 }
 
 }  // namespace
@@ -632,6 +680,25 @@ void build_level_cat(HostOp<T>& op, int l) {
     std::copy(L.ilu[b].Dinv.begin(), L.ilu[b].Dinv.end(), L.Dcat.begin() + off);
   }
 }
+
+namespace {
+
+int lowrank_device(HostOp<double>& op, int device, std::string* msg) {
+  if (!g_lowrank_device) {
+    if (msg) *msg = "no device low-rank build in this library";
+    return 102;
+  }
+  return g_lowrank_device(op, device, msg);
+}
+int lowrank_device(HostOp<zdouble>& op, int device, std::string* msg) {
+  if (!g_lowrank_device_z) {
+    if (msg) *msg = "no device low-rank build in this library";
+    return 102;
+  }
+  return g_lowrank_device_z(op, device, msg);
+}
+
+}  // namespace
 
 template <typename T>
 int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::string* msg) {
@@ -754,12 +821,16 @@ int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::s
   }
   for (int l = 0; l < nlev; ++l) build_level_cat(op, l);
   const double t2 = now_s();
-  // 4. low-rank corrections, bottom level first
-  for (int l = nlev - 2; l >= 0; --l) {
-    lowrank_level(op, l);
-    // the level-0 block C stays for the inner Schur iteration of the solve phase
-    if (!(l == 0 && prm.schur_enable)) op.levels[l].C = Csr<T>();
+  // 4. low-rank corrections, bottom level first, on the host or on a device
+  if (prm.lowrank_device >= 0) {
+    const int e = lowrank_device(op, prm.lowrank_device, msg);
+    if (e) return e;
+  } else {
+    for (int l = nlev - 2; l >= 0; --l) lowrank_level(op, l);
   }
+  // the level-0 block C stays for the inner Schur iteration of the solve phase
+  for (int l = 0; l + 1 < nlev; ++l)
+    if (!(l == 0 && prm.schur_enable)) op.levels[l].C = Csr<T>();
   op.solve_phase = true;
   const double t3 = now_s();
   op.setup_seconds[0] = t1 - t0;
@@ -769,7 +840,12 @@ int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::s
   return 0;
 }
 
+LowrankDeviceFn g_lowrank_device = nullptr;
+LowrankDeviceFnZ g_lowrank_device_z = nullptr;
+
 template void build_level_cat<double>(HostOp<double>&, int);
+template int lowrank_level_with<double>(HostOp<double>&, int, LrBackend<double>&);
+template int lowrank_level_with<zdouble>(HostOp<zdouble>&, int, LrBackend<zdouble>&);
 template void build_level_cat<zdouble>(HostOp<zdouble>&, int);
 template int gemslr_setup<double>(const Csr<double>&, const GemslrParams&, HostOp<double>&, std::string*);
 template int gemslr_setup<zdouble>(const Csr<zdouble>&, const GemslrParams&, HostOp<zdouble>&, std::string*);
