@@ -112,6 +112,24 @@ def main():
     log(f"[bench] setup {inf.setup_seconds[3]:.1f}s (partition {inf.setup_seconds[0]:.1f}, ilut {inf.setup_seconds[1]:.1f},"
         f" lowrank {inf.setup_seconds[2]:.1f}); nlev={inf.nlev} nnz_ilu={inf.nnz_ilu} nnz_lr={inf.nnz_lr}")
     levels = G.levels()
+    # the same setup with the low-rank corrections built on the device
+    # (pgs_gemslr_setup_device), timed beside the host build; its ranks and
+    # bases are compared with the host build's (the solves below use the
+    # host-built operator)
+    Gd = pg.Gemslr(A, params, seed=0, lowrank_device=0)
+    infd = Gd.info()
+    lr_dist, ranks_equal = 0.0, True
+    for lh, ld in zip(levels, Gd.levels()):
+        ranks_equal = ranks_equal and lh["rank"] == ld["rank"]
+        if lh["rank"] and ld["rank"]:
+            Wh, Wd = lh["W"], ld["W"]
+            lr_dist = max(lr_dist, float(np.linalg.norm(Wd - Wh @ (Wh.conj().T @ Wd))))
+    del Gd
+    setup_lr = {"host_lowrank_s": round(inf.setup_seconds[2], 3), "device_lowrank_s": round(infd.setup_seconds[2], 3),
+                "device_setup_s": round(infd.setup_seconds[3], 2), "ranks_equal": ranks_equal,
+                "max_subspace_dist": lr_dist}
+    log(f"[bench] low-rank build on the device {infd.setup_seconds[2]:.2f}s (host {inf.setup_seconds[2]:.2f}s), "
+        f"ranks equal {ranks_equal}, subspace distance {lr_dist:.1e}")
     D = pg.DevOp(G, prec_mask=pg.PGS_PREC_FP64 | pg.PGS_PREC_FP32)
     W1 = pg.Workspace(D, max_nrhs=1, kdim=cfg["kdim"])
     WB = pg.Workspace(D, max_nrhs=nr, kdim=cfg["kdim"])
@@ -259,6 +277,7 @@ def main():
         "err_vs_xstar": err_vs_xstar,
         "converged": ok,
         "setup_s": round(inf.setup_seconds[3], 2),
+        "setup_lowrank": setup_lr,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

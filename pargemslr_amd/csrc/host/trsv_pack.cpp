@@ -124,12 +124,14 @@ TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_ro
   P.nblocks = L.ncomp;
   P.nloc = nloc;
   std::vector<SweepOrder> oL(L.ncomp), oU(L.ncomp);
-  for (int b = 0; b < L.ncomp; ++b) {
-    const IluFactor<T>& F = L.ilu[b];
-    oL[b] = best_order(F.L, false, max_rows);
-    oU[b] = best_order(F.U, true, max_rows);
+  // the blocks' schedules are independent: one task per block and factor
+  parallel_for(2 * L.ncomp, [&](int t) {
+    const IluFactor<T>& F = L.ilu[t / 2];
+    if (t % 2 == 0) oL[t / 2] = best_order(F.L, false, max_rows);
+    else oU[t / 2] = best_order(F.U, true, max_rows);
+  });
+  for (int b = 0; b < L.ncomp; ++b)
     P.window_needed = std::max(P.window_needed, std::max(oL[b].window, oU[b].window));
-  }
   int W = 1;
   while (W < P.window_needed) W <<= 1;
   int row_len = 0;
