@@ -165,16 +165,22 @@ void upload_trsv(DevTrsv& d, const TrsvPack<T>& P, int64_t* counter) {
     upload(d.stepU_h, suh, counter);
   }
   std::vector<TPair<V>> ent(P.col.size() + 1), ri(P.rows.size() + 1);
-  for (size_t i = 0; i < P.col.size(); ++i) {
-    ent[i] = TPair<V>();
-    ent[i].col = P.window > 0 ? P.col[i] * ring_unit<V>() : P.col[i];  // ring slots as 4-column row offsets
-    ent[i].val = hval<V>(P.val[i]);
-  }
-  for (size_t i = 0; i < P.rows.size(); ++i) {
-    ri[i] = TPair<V>();
-    ri[i].col = P.rows[i];
-    ri[i].val = hval<V>(P.dinv[i]);
-  }
+  constexpr int64_t kChunk = 1 << 20;
+  const int64_t ne = (int64_t)P.col.size(), nr = (int64_t)P.rows.size();
+  parallel_for((int)((ne + kChunk - 1) / kChunk), [&](int c) {
+    for (int64_t i = c * kChunk, i1 = std::min(ne, (c + 1) * kChunk); i < i1; ++i) {
+      ent[i] = TPair<V>();
+      ent[i].col = P.window > 0 ? P.col[i] * ring_unit<V>() : P.col[i];  // ring slots as 4-column row offsets
+      ent[i].val = hval<V>(P.val[i]);
+    }
+  });
+  parallel_for((int)((nr + kChunk - 1) / kChunk), [&](int c) {
+    for (int64_t i = c * kChunk, i1 = std::min(nr, (c + 1) * kChunk); i < i1; ++i) {
+      ri[i] = TPair<V>();
+      ri[i].col = P.rows[i];
+      ri[i].val = hval<V>(P.dinv[i]);
+    }
+  });
   ent.back() = TPair<V>();
   ri.back() = TPair<V>();
   upload(d.ent, ent, counter);

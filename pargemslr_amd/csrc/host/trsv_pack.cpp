@@ -81,37 +81,47 @@ SweepOrder best_order(const Csr<T>& M, bool upper, int max_rows) {
   return b.window < a.window ? b : a;
 }
 
+// Slab width of each step of a sweep (fixed_width when the rows are
+// ring-packed, else the longest row of the step).
 template <typename T>
-void emit_sweep(const Csr<T>& M, const std::vector<T>* dinv, const SweepOrder& o, int base, int wmask,
-                int fixed_width, int64_t ent0, int64_t ri0, TrsvPack<T>& P) {
-  for (size_t s = 0; s + 1 < o.step.size(); ++s) {
-    const int k0 = o.step[s], nr = o.step[s + 1] - k0;
-    int width = fixed_width;
-    for (int t = 0; t < nr && !fixed_width; ++t) {
-      const int i = o.order[k0 + t];
-      width = std::max(width, M.ia[i + 1] - M.ia[i]);
+std::vector<int> step_widths(const Csr<T>& M, const SweepOrder& o, int fixed_width) {
+  std::vector<int> w(o.step.size() - 1, fixed_width);
+  if (fixed_width) return w;
+  for (size_t s = 0; s + 1 < o.step.size(); ++s)
+    for (int k = o.step[s]; k < o.step[s + 1]; ++k) {
+      const int i = o.order[k];
+      w[s] = std::max(w[s], M.ia[i + 1] - M.ia[i]);
     }
-    const int64_t row_off = (int64_t)P.rows.size() - ri0, ent_off = (int64_t)P.col.size() - ent0;
-    P.desc.push_back((int)row_off);
-    P.desc.push_back((int)ent_off);
-    P.desc.push_back(nr);
-    P.desc.push_back(width);
-    P.max_width = std::max(P.max_width, width);
-    const size_t e = P.col.size();
-    P.col.resize(e + (size_t)width * nr, 0);
-    P.val.resize(e + (size_t)width * nr, T(0));
+  return w;
+}
+
+// Writes the steps of one sweep at the given offsets of P's arrays (steps
+// from step0, entries from ent, rows from row); entry and row offsets in the
+// descriptors are relative to the block's first entry and row (ent0, ri0).
+template <typename T>
+void emit_sweep(const Csr<T>& M, const std::vector<T>* dinv, const SweepOrder& o, const std::vector<int>& width,
+                int base, int wmask, int64_t ent0, int64_t ri0, int64_t step0, int64_t ent, int64_t row,
+                TrsvPack<T>& P) {
+  for (size_t s = 0; s + 1 < o.step.size(); ++s) {
+    const int k0 = o.step[s], nr = o.step[s + 1] - k0, w = width[s];
+    int* d = &P.desc[(size_t)(step0 + (int64_t)s) * 4];
+    d[0] = (int)(row - ri0);
+    d[1] = (int)(ent - ent0);
+    d[2] = nr;
+    d[3] = w;
     for (int t = 0; t < nr; ++t) {
       const int i = o.order[k0 + t];
-      P.rows.push_back(i);
-      P.dinv.push_back(dinv ? (*dinv)[i] : T(0));
+      P.rows[row + t] = i;
+      P.dinv[row + t] = dinv ? (*dinv)[i] : T(0);
       int j = 0;
       for (int k = M.ia[i]; k < M.ia[i + 1]; ++k, ++j) {
         const int c = M.ja[k];
-        P.col[e + (size_t)t * width + j] = wmask ? ((base + o.pos[c]) & wmask) : c;
-        P.val[e + (size_t)t * width + j] = M.a[k];
+        P.col[ent + (int64_t)t * w + j] = wmask ? ((base + o.pos[c]) & wmask) : c;
+        P.val[ent + (int64_t)t * w + j] = M.a[k];
       }
-      P.nnz_real += M.ia[i + 1] - M.ia[i];
     }
+    row += nr;
+    ent += (int64_t)w * nr;
   }
 }
 
@@ -146,25 +156,60 @@ TrsvPack<T> trsv_pack(const HostLevel<T>& L, int lev_start, int nloc, int max_ro
   P.blk_stepU.resize(L.ncomp);
   P.blk_ent0.resize(L.ncomp + 1);
   P.blk_ri0.resize(L.ncomp + 1);
+  // sizes of every block, then one allocation and the blocks emitted in
+  // parallel at their offsets
+  std::vector<std::vector<int>> wL(L.ncomp), wU(L.ncomp);
+  std::vector<int64_t> nst(L.ncomp), nstL(L.ncomp), nent(L.ncomp), nrow(L.ncomp), nreal(L.ncomp);
+  std::vector<int> maxw(L.ncomp, 0);
+  parallel_for(L.ncomp, [&](int b) {
+    const IluFactor<T>& F = L.ilu[b];
+    wL[b] = step_widths(F.L, oL[b], fixed_width);
+    wU[b] = step_widths(F.U, oU[b], fixed_width);
+    int64_t e = 0;
+    for (size_t s = 0; s < wL[b].size(); ++s) e += (int64_t)wL[b][s] * (oL[b].step[s + 1] - oL[b].step[s]);
+    for (size_t s = 0; s < wU[b].size(); ++s) e += (int64_t)wU[b][s] * (oU[b].step[s + 1] - oU[b].step[s]);
+    for (int w : wL[b]) maxw[b] = std::max(maxw[b], w);
+    for (int w : wU[b]) maxw[b] = std::max(maxw[b], w);
+    nstL[b] = (int64_t)wL[b].size();
+    nst[b] = nstL[b] + (int64_t)wU[b].size();
+    nent[b] = e;
+    nrow[b] = (int64_t)F.L.nrows + F.U.nrows;
+    nreal[b] = (int64_t)F.L.nnz() + F.U.nnz();
+  });
+  int64_t st = 0, en = 0, rw = 0;
   for (int b = 0; b < L.ncomp; ++b) {
     P.blk_row0[b] = L.dom_ptr[b] - lev_start;
-    const IluFactor<T>& F = L.ilu[b];
-    P.max_block_rows = std::max(P.max_block_rows, F.n);
-    const int64_t ent0 = (int64_t)P.col.size(), ri0 = (int64_t)P.rows.size();
-    P.blk_ent0[b] = ent0;
-    P.blk_ri0[b] = ri0;
-    P.blk_step[b] = (int)P.desc.size() / 4;
-    emit_sweep<T>(F.L, nullptr, oL[b], 0, wmask, fixed_width, ent0, ri0, P);
-    P.blk_stepU[b] = (int)P.desc.size() / 4;
-    emit_sweep<T>(F.U, &F.Dinv, oU[b], F.n, wmask, fixed_width, ent0, ri0, P);
-    P.max_steps = std::max(P.max_steps, (int)P.desc.size() / 4 - P.blk_step[b]);
-    oL[b] = SweepOrder();
-    oU[b] = SweepOrder();
+    P.max_block_rows = std::max(P.max_block_rows, L.ilu[b].n);
+    P.blk_ent0[b] = en;
+    P.blk_ri0[b] = rw;
+    P.blk_step[b] = (int)st;
+    P.blk_stepU[b] = (int)(st + nstL[b]);
+    P.max_steps = std::max(P.max_steps, (int)nst[b]);
+    P.max_width = std::max(P.max_width, maxw[b]);
+    P.nnz_real += nreal[b];
+    st += nst[b];
+    en += nent[b];
+    rw += nrow[b];
   }
   P.blk_row0[L.ncomp] = nloc;
-  P.blk_step[L.ncomp] = (int)P.desc.size() / 4;
-  P.blk_ent0[L.ncomp] = (int64_t)P.col.size();
-  P.blk_ri0[L.ncomp] = (int64_t)P.rows.size();
+  P.blk_step[L.ncomp] = (int)st;
+  P.blk_ent0[L.ncomp] = en;
+  P.blk_ri0[L.ncomp] = rw;
+  P.desc.assign((size_t)st * 4, 0);
+  P.col.assign((size_t)en, 0);
+  P.val.assign((size_t)en, T(0));
+  P.rows.assign((size_t)rw, 0);
+  P.dinv.assign((size_t)rw, T(0));
+  parallel_for(L.ncomp, [&](int b) {
+    const IluFactor<T>& F = L.ilu[b];
+    const int64_t e0 = P.blk_ent0[b], r0 = P.blk_ri0[b];
+    int64_t eU = e0;
+    for (size_t s = 0; s < wL[b].size(); ++s) eU += (int64_t)wL[b][s] * (oL[b].step[s + 1] - oL[b].step[s]);
+    emit_sweep<T>(F.L, nullptr, oL[b], wL[b], 0, wmask, e0, r0, P.blk_step[b], e0, r0, P);
+    emit_sweep<T>(F.U, &F.Dinv, oU[b], wU[b], F.n, wmask, e0, r0, P.blk_stepU[b], eU, r0 + F.L.nrows, P);
+    oL[b] = SweepOrder();
+    oU[b] = SweepOrder();
+  });
   return P;
 }
 
