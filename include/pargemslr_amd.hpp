@@ -275,6 +275,9 @@ class GemslrClass {
     return 0;
   }
   void SetSeed(uint64_t seed) { seed_ = seed; }
+  // Builds the low-rank corrections of the next Setup on this device
+  // (pgs_gemslr_setup_device); -1 (the default) builds them on the host.
+  void SetLowRankDevice(int device) { lr_device_ = device; }
   // Writes the host operator (after Setup or Load) to an operator file.
   int Save(const char* path) const {
     if (!host_) return PGS_ERROR_FUNCTION_CALL;
@@ -309,9 +312,12 @@ class GemslrClass {
     if (ready_) return 0;
     if (!matrix_) return PGS_ERROR_INVALID_PARAM;
     int err = 0;
-    if (!host_)
+    if (!host_ && lr_device_ < 0)
       err = pgs_gemslr_setup(matrix_->n_, matrix_->ia_.data(), matrix_->ja_.data(), VT::raw(matrix_->a_.data()),
                              VT::is_complex, params_, seed_, &host_);
+    else if (!host_)
+      err = pgs_gemslr_setup_device(matrix_->n_, matrix_->ia_.data(), matrix_->ja_.data(),
+                                    VT::raw(matrix_->a_.data()), VT::is_complex, params_, seed_, lr_device_, &host_);
     if (err) return err;
     if (location_ == kMemoryDevice) {
       err = pgs_dev_op_create(host_, 0, PGS_PREC_FP64 | PGS_PREC_FP32, &dop_);
@@ -356,6 +362,7 @@ class GemslrClass {
   int prec_ = PGS_PREC_FP64;
   int kdim_ws_ = 50;
   uint64_t seed_ = 0;
+  int lr_device_ = -1;
   bool ready_ = false;
   pgs_gemslr_t host_ = nullptr;
   pgs_dev_op_t dop_ = nullptr;

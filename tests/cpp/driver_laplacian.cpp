@@ -40,6 +40,8 @@ int run(int nx, int ny, int nz, const double* params, int prec, T shift) {
   precond.SetWithParameterArray(params);
   precond.SetSolveLocation(kMemoryDevice);
   precond.SetApplyPrecision(prec == 2 ? PGS_PREC_FP32 : PGS_PREC_FP64);
+  // PGS_LOWRANK_DEVICE=d: the low-rank corrections are built on device d
+  if (const char* d = std::getenv("PGS_LOWRANK_DEVICE")) precond.SetLowRankDevice(std::atoi(d));
   solver.SetPreconditioner(precond);
   int err = solver.Setup(x, b);
   if (err) {

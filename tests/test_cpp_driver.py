@@ -32,3 +32,21 @@ def test_driver_solves(args, gpu):
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "Number of iterations" in out.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args", [
+    ["32", "32", "32", "3", "8", "1e-3", "20", "16", "50", "1e-8", "2"],
+    ["24", "24", "24", "3", "8", "1e-3", "20", "16", "50", "1e-8", "1", "0.2", "0.02", "1"],
+], ids=["real-mixed", "complex-fp64"])
+def test_driver_with_device_low_rank_build(args, gpu):
+    """GemslrClass::SetLowRankDevice: the same solve with the low-rank
+    corrections built on the device takes the same iteration count."""
+    its = []
+    for env in ({}, {"PGS_LOWRANK_DEVICE": "0"}):
+        out = subprocess.run([_driver()] + args, capture_output=True, text=True, timeout=120,
+                             env=dict(os.environ, **env))
+        print(out.stdout)
+        assert out.returncode == 0, out.stdout + out.stderr
+        its.append([l for l in out.stdout.splitlines() if "Number of iterations" in l])
+    assert its[0] == its[1]
