@@ -39,6 +39,7 @@ extern "C" {
 #define PGS_ERR_BREAKDOWN 203
 #define PGS_ERR_NOT_CONVERGED 204
 #define PGS_ERR_UNSUPPORTED 205
+#define PGS_ERR_NONFINITE 206 /* a column met NaN or Inf and was retired; the other columns are unaffected */
 
 /* parameter array layout: the reference's PARGEMSLR_IO_* indices
  * (ParGeMSLR/SRC/utils/utils.hpp:77-297); only the entries listed here are read */
@@ -239,7 +240,9 @@ typedef struct {
 } pgs_fgmres_opts_t;
 
 typedef struct {
-  int status;       /* 0 converged, PGS_ERR_NOT_CONVERGED, PGS_ERR_BREAKDOWN */
+  int status;       /* 0 converged, PGS_ERR_NOT_CONVERGED, PGS_ERR_BREAKDOWN, PGS_ERR_NONFINITE
+                       (NaN or Inf in b, x0 or the iteration: the column stops at once,
+                       x keeps its value from the start of the cycle) */
   int iters;
   double rel_res;   /* Givens estimate ||r||/||b|| at exit (FlexGmresClass::_rel_res) */
 } pgs_fgmres_result_t;

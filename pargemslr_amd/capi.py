@@ -20,6 +20,7 @@ PGS_ORTH_CGS2 = 1
 PGS_ORTH_DCGS2 = 2
 PGS_ERR_BREAKDOWN = 203
 PGS_ERR_NOT_CONVERGED = 204
+PGS_ERR_NONFINITE = 206
 
 # parameter indices (reference PARGEMSLR_IO_* numbering)
 IO = dict(

@@ -1478,6 +1478,7 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
     res[c].rel_res = hs[c].normb > 0 ? hs[c].normr / hs[c].normb : 0.0;
     if (hs[c].status == 1 || hs[c].status == 4) res[c].status = 0;
     else if (hs[c].status == 2) res[c].status = PGS_ERR_BREAKDOWN;
+    else if (hs[c].status == 5) res[c].status = PGS_ERR_NONFINITE;
     else res[c].status = PGS_ERR_NOT_CONVERGED;
     if (hs[c].status == 4) res[c].rel_res = 0.0;
   }

@@ -325,10 +325,16 @@ __global__ void k_lr_update(int m, int k, int nrhs, const V* __restrict__ WH, co
 // Krylov kernels (FP64).  A block of p vectors V_j (each n x nrhs) at
 // V + j*ldv.  State per column in KState.
 // ---------------------------------------------------------------------------
+// Delayed CGS2: a tentative basis vector whose squared distance from the
+// span of the final ones, q.q - |s|^2, is at most this fraction of q.q is
+// taken as dependent (it lies in the span to working precision).
+constexpr double kDcgs2DependentTol = 1e-10;
+
 struct KState {
   double normb, tolr, normr, scale;
   int active;   // running in the current cycle
-  int status;   // 0 running, 1 converged, 2 breakdown, 3 max iterations, 4 zero rhs/residual
+  int status;   // 0 running, 1 converged, 2 breakdown, 3 max iterations, 4 zero rhs/residual,
+                // 5 non-finite value (NaN or Inf in b, x0 or the iteration): retired at once
   int iters;    // total inner iterations (FlexGmresClass::_iter)
   int icur;     // inner steps of the current cycle
 };
@@ -731,9 +737,32 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
   const S qw = final_pass ? S(0.0) : d[(2 * p + 1) * nrhs + c];
   double nu = 1.0;
   S* Hc = H + (int64_t)c * (m + 1) * m;
+  auto retire = [&](int status) {
+    s.status = status;
+    s.active = 0;
+    s.scale = 0.0;
+    st[c] = s;
+    zero();
+  };
+  if (!isfinite(qq) || !isfinite(sabs_d(qw))) {
+    retire(5);
+    return;
+  }
+  // the tentative v_p lies in the span of v_0..v_{p-1} to working precision
+  // when q.q - |s|^2 cancels (|s| = 1 - 5e-11 and closer): nu and the final
+  // v_p = (q - V s) / nu are then rounding noise, so the column's cycle ends
+  // with Hessenberg column p - 1 (whose subdiagonal nu * h is negligible
+  // there, a lucky breakdown) and restarts from its true residual instead of
+  // continuing on a basis vector that is not orthogonal
+  bool dependent = false;
   if (p > 0) {
     double ss = 0.0;
     for (int j = 0; j < p; ++j) ss += (double)sabs2(d[j * nrhs + c]);
+    if (!isfinite(ss)) {
+      retire(5);
+      return;
+    }
+    dependent = !(qq - ss > kDcgs2DependentTol * qq);
     nu = sqrt(fmax(qq - ss, 0.0));
     const int col = p - 1;
     const S hq = Hc[p + col * (m + 1)];
@@ -754,12 +783,12 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
     const S hii = Hc[(p - 1) + col * (m + 1)];
     const S hii1 = Hc[p + col * (m + 1)];
     const double g = sqrt((double)sabs2(hii) + (double)sabs2(hii1));
+    if (!isfinite(g)) {
+      retire(5);
+      return;
+    }
     if (fabs(g) < eps) {  // breakdown: the reference leaves x untouched (fgmres.hpp:440-443)
-      s.status = 2;
-      s.active = 0;
-      s.scale = 0.0;
-      st[c] = s;
-      zero();
+      retire(2);
       return;
     }
     csc[p - 1] = hii / S(g);
@@ -771,7 +800,7 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
     if (s.normr <= s.tolr) {
       s.status = 1;
       s.active = 0;
-    } else if (p >= m || s.iters >= maxits) {
+    } else if (p >= m || s.iters >= maxits || dependent) {
       s.active = 0;  // cycle ends for this column
     }
     st[c] = s;
@@ -831,6 +860,10 @@ __global__ void k_dcgs2_norm(int nrhs, int m, int p, const double* __restrict__ 
   KState s = st[c];
   if (!s.active) {
     s.scale = 0.0;
+  } else if (!isfinite(t2[c])) {
+    s.status = 5;
+    s.active = 0;
+    s.scale = 0.0;
   } else {
     const double t = sqrt(t2[c]);
     H[(int64_t)c * (m + 1) * m + (p + 1) + p * (m + 1)] = S(t);
@@ -863,7 +896,9 @@ __global__ void k_fgmres_init(int nrhs, const double* __restrict__ nb2, const do
   s.active = 0;
   s.status = 0;
   x_zero_flag[c] = 0.0;
-  if (s.normb < eps) {  // zero right-hand side: x = 0 (fgmres.hpp:321-328)
+  if (!isfinite(s.normb) || !isfinite(s.normr)) {  // NaN or Inf in b or x0: x stays as given
+    s.status = 5;
+  } else if (s.normb < eps) {  // zero right-hand side: x = 0 (fgmres.hpp:321-328)
     s.status = 4;
     x_zero_flag[c] = 1.0;
   } else if (s.normr < eps) {  // initial guess already exact (fgmres.hpp:341-348)
@@ -936,6 +971,13 @@ __global__ void k_givens(int nrhs, int m, int i, int maxits, const S* __restrict
   const S hii = Hc[(i - 1) + col * (m + 1)];
   const S hii1 = Hc[i + col * (m + 1)];
   const double gam = sqrt((double)sabs2(hii) + (double)sabs2(hii1));
+  if (!isfinite(gam) || !isfinite(t)) {  // NaN or Inf from the apply or the matrix product
+    s.status = 5;
+    s.active = 0;
+    s.scale = 0.0;
+    st[c] = s;
+    return;
+  }
   if (fabs(gam) < eps) {  // breakdown: the reference leaves x untouched (fgmres.hpp:440-443)
     s.status = 2;
     s.active = 0;
@@ -968,7 +1010,7 @@ __global__ void k_backsolve(int nrhs, int m, const S* __restrict__ H, S* __restr
   KState s = st[c];
   for (int j = 0; j < m; ++j) coef[j * nrhs + c] = S(0.0);
   upd[c] = 0;
-  if (s.status == 2 || s.status == 4 || s.icur == 0) return;
+  if (s.status == 2 || s.status == 4 || s.status == 5 || s.icur == 0) return;
   const S* Hc = H + (int64_t)c * (m + 1) * m;
   S* rsc = rs + (int64_t)c * (m + 1);
   const int i = s.icur;

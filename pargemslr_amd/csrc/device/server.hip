@@ -112,10 +112,12 @@ bool memory_fits(pgs_server_s* S, int64_t need) {
   return (int64_t)fr - S->opts.mem_reserve >= need;
 }
 
-// Runs one batch; fills status/iters of its requests.
-void run_batch(pgs_server_s* S, void* stream, std::map<int, Slot>& cache, std::vector<Request*>& batch) {
+// Runs one batch on operator `op` (a copy taken under the server's lock:
+// pgs_server_add_operator may grow the operator list meanwhile); fills
+// status/iters of its requests.
+void run_batch(pgs_server_s* S, void* stream, std::map<int, Slot>& cache, const Operator op,
+               std::vector<Request*>& batch) {
   const int op_id = batch[0]->op;
-  const Operator& op = S->ops[op_id];
   const int n = op.n;
   const int dpv = op.dpv;
   int rc = 0;
@@ -199,12 +201,14 @@ void worker_main(pgs_server_s* S) {
   std::map<int, Slot> cache;
   for (;;) {
     std::vector<Request*> batch;
+    Operator opv;
     {
       std::unique_lock<std::mutex> lk(S->m);
       S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
       if (S->queue.empty()) break;  // stop requested and nothing left
       // the oldest request, then later ones for the same operator that fit
       const int op = S->queue.front()->op;
+      opv = S->ops[op];
       int cols = 0;
       for (auto q = S->queue.begin(); q != S->queue.end();) {
         Request* r = *q;
@@ -223,7 +227,7 @@ void worker_main(pgs_server_s* S) {
     if (src) {
       for (Request* r : batch) r->status = src;
     } else {
-      run_batch(S, stream, cache, batch);
+      run_batch(S, stream, cache, opv, batch);
     }
     const auto t1 = Clock::now();
     {

@@ -266,6 +266,7 @@ const char* pgs_strerror(int code) {
     case PGS_ERR_BREAKDOWN: return "FGMRES breakdown";
     case PGS_ERR_NOT_CONVERGED: return "FGMRES did not converge";
     case PGS_ERR_UNSUPPORTED: return "unsupported operation";
+    case PGS_ERR_NONFINITE: return "NaN or Inf in the right-hand side, the initial guess or the iteration";
     default: return "unknown error";
   }
 }

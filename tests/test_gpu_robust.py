@@ -1,0 +1,93 @@
+"""Failure isolation on the device: a column with NaN or Inf stops at once
+without changing any other column of its batch, and a Krylov basis that
+becomes dependent (a lucky breakdown) ends the cycle instead of feeding a
+rounding-noise vector into the delayed-CGS2 basis.
+
+The reference has no batches; its FGMRES (fgmres.hpp:263-558) on a NaN
+right-hand side runs to maxits.  Here the column reports PGS_ERR_NONFINITE
+after zero iterations and the other columns give the same bits as in a batch
+with a finite column in its place."""
+import numpy as np
+import pytest
+
+import pargemslr_amd as pg
+from oracle import oracle as orc
+from tests import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(x, dev):
+    return pg.DeviceArray.from_numpy(np.ascontiguousarray(x, dtype=np.float64), device=dev)
+
+
+def _host(t):
+    pg.synchronize(t.device)
+    return t.numpy()
+
+
+@pytest.fixture(scope="module")
+def small(gpu):
+    A, G = P.build(P.C2_SMALL)
+    D = pg.DevOp(G)
+    W = pg.Workspace(D, max_nrhs=4, kdim=50)
+    return A, G, D, W
+
+
+@pytest.mark.parametrize("orth", [pg.PGS_ORTH_MGS, pg.PGS_ORTH_CGS2, pg.PGS_ORTH_DCGS2])
+@pytest.mark.parametrize("bad", ["nan_b", "inf_b", "nan_x0"])
+def test_nonfinite_column_is_retired(small, gpu, orth, bad):
+    A, G, D, W = small
+    nr = 4
+    B = np.stack([pg.rng_normal(40 + c, A.n) for c in range(nr)], axis=1)
+    X0 = np.zeros((A.n, nr))
+    Bbad, X0bad = B.copy(), X0.copy()
+    if bad == "nan_b":
+        Bbad[A.n // 3, 2] = np.nan
+    elif bad == "inf_b":
+        Bbad[7, 2] = np.inf
+    else:
+        X0bad[A.n // 2, 2] = np.nan
+    runs = []
+    for b, x0 in ((B, X0), (Bbad, X0bad)):
+        xd = _dev(x0, gpu)
+        res = W.fgmres(_dev(b, gpu), xd, nrhs=nr, kdim=50, maxits=300, tol=1e-8, prec=pg.PGS_PREC_FP32, orth=orth)
+        runs.append((res, _host(xd)))
+    (res_ok, x_ok), (res_bad, x_bad) = runs
+    print(f"[robust] {bad} orth={orth}: {res_bad}", flush=True)
+    st, it, _ = res_bad[2]
+    assert st == pg.PGS_ERR_NONFINITE and it == 0, res_bad[2]
+    for c in (0, 1, 3):
+        assert res_bad[c] == res_ok[c], (c, res_bad[c], res_ok[c])
+        assert res_ok[c][0] == 0
+        assert np.array_equal(x_bad[:, c], x_ok[:, c]), c
+    # the retired column keeps its initial guess
+    assert np.array_equal(x_bad[:, 2], X0bad[:, 2], equal_nan=True)
+
+
+@pytest.mark.parametrize("grid", [(4, 4, 4), (5, 5, 1), (6, 6, 1)])
+@pytest.mark.parametrize("orth", [pg.PGS_ORTH_MGS, pg.PGS_ORTH_DCGS2])
+def test_exhausted_krylov_space(gpu, grid, orth):
+    """b = A 1 on a small Laplacian: its Krylov space is exhausted after as
+    many steps as b has distinct eigenvalue components (7, 12, 25 here),
+    where the next basis vector is rounding noise.  With tol = 0 no step
+    meets the stopping test, so the solver must pass that point: every
+    column stays finite and ends at x = 1 to rounding, converged (a zero
+    Givens residual) or at maxits, never as a breakdown that drops the
+    cycle's progress; the oracle's MGS FGMRES gives the same solution."""
+    A = pg.laplacian(*grid)
+    G = pg.Gemslr(A, pg.config_params(2, 2, 1e-3, 20, 0))
+    D = pg.DevOp(G)
+    W = pg.Workspace(D, max_nrhs=1, kdim=30)
+    b = A.matvec(np.ones(A.n))
+    xo, st_o, it_o, _, _ = orc.fgmres(A, b, prec=None, kdim=30, maxits=40, tol=0.0)
+    xd = _dev(np.zeros(A.n), gpu)
+    (status, it, rr), = W.fgmres(_dev(b, gpu), xd, kdim=30, maxits=40, tol=0.0, prec=pg.PGS_PREC_FP64, orth=orth,
+                                 use_precond=False)
+    x = _host(xd)
+    print(f"[robust] exhausted {grid} orth={orth}: status {status} iters {it} (oracle {st_o}, {it_o})", flush=True)
+    assert status in (0, pg.PGS_ERR_NOT_CONVERGED), status
+    assert np.isfinite(x).all()
+    assert np.linalg.norm(x - 1.0) <= 1e-10 * np.sqrt(A.n)
+    assert np.linalg.norm(b - A.matvec(x)) <= 1e-12 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
