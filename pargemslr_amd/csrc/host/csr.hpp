@@ -11,7 +11,9 @@
 #include <complex>
 #include <cstdint>
 #include <cmath>
+#include <exception>
 #include <functional>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -53,19 +55,31 @@ inline void parallel_for(int n, const std::function<void(int)>& f, int nthreads 
   }
   std::vector<std::thread> pool;
   std::atomic<int> next{0};
+  // the first exception a worker throws (std::bad_alloc from a body's
+  // allocation, say) stops the remaining chunks and is rethrown on the
+  // calling thread after every worker has joined
+  std::exception_ptr first;
+  std::mutex first_m;
   int nt = std::min(nthreads, n);
   int chunk = std::max(1, n / (nt * 32));
   for (int t = 0; t < nt; ++t) {
     pool.emplace_back([&]() {
-      for (;;) {
-        int i0 = next.fetch_add(chunk);
-        if (i0 >= n) break;
-        int i1 = std::min(n, i0 + chunk);
-        for (int i = i0; i < i1; ++i) f(i);
+      try {
+        for (;;) {
+          int i0 = next.fetch_add(chunk);
+          if (i0 >= n) break;
+          int i1 = std::min(n, i0 + chunk);
+          for (int i = i0; i < i1; ++i) f(i);
+        }
+      } catch (...) {
+        next.store(n);
+        std::lock_guard<std::mutex> g(first_m);
+        if (!first) first = std::current_exception();
       }
     });
   }
   for (auto& th : pool) th.join();
+  if (first) std::rethrow_exception(first);
 }
 
 template <typename T>

@@ -75,7 +75,9 @@ void ring_solve(const TrsvPack<T>& P, const std::vector<int>& desc, int sbeg, in
         T acc = T(0.0);
         for (int j = 0; j < w; ++j) {
           const int64_t e = e0 + eo + (int64_t)t * w + j;
-          acc += P.val[e] * ring[P.col[e]];
+          // col holds the slot in the packed window; a shorter ring (the
+          // control) takes the same position modulo its own size
+          acc += P.val[e] * ring[P.col[e] & (W - 1)];
         }
         const T src = upper ? x[row[t]] : rhs[row[t]];
         v[t] = src - acc;
