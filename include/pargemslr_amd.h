@@ -199,6 +199,35 @@ int pgs_stream_destroy(int device, void* stream);
 int pgs_stream_synchronize(int device, void* stream);
 int pgs_device_synchronize(int device);
 
+/* ---------------- device vectors and matrices of the class surface ---------------- */
+
+/* Single-vector operations on device arrays of n values (double, or (re, im)
+ * pairs with is_complex = 1); alpha / value are one double or one pair.
+ * SequentialVectorClass::Axpy / Scale / Fill / Dot / Norm2
+ * (ParGeMSLR/SRC/vectors/sequential_vector.cpp:706-732, 1037-1172 over
+ * vectorops.cpp:93-146, 633-680): y += alpha x; x *= alpha; x = value;
+ * result = sum_i conj(x_i) y_i (one double or one pair, written to host
+ * memory after the stream has drained; summed in a fixed order, so repeated
+ * calls give the same bits). */
+int pgs_vec_axpy(int device, int64_t n, int is_complex, const double* alpha, const double* x, double* y,
+                 void* stream);
+int pgs_vec_scale(int device, int64_t n, int is_complex, const double* alpha, double* x, void* stream);
+int pgs_vec_fill(int device, int64_t n, int is_complex, const double* value, double* x, void* stream);
+int pgs_vec_dot(int device, int64_t n, int is_complex, const double* x, const double* y, double* result,
+                void* stream);
+
+/* A square CSR matrix resident on the device by itself (CsrMatrixClass::
+ * MoveData(kMemoryDevice), csr_matrix.cpp), for MatVec without a
+ * preconditioner: y = alpha A x + beta y with the SpMV kernels of the solve
+ * (CsrMatrixClass::MatVec, csr_matrix.cpp:1340-1365).  ia (n + 1, from 0),
+ * ja (0-based columns < n) and a (double or (re, im) pairs) are copied. */
+typedef struct pgs_dev_csr_s* pgs_dev_csr_t;
+int pgs_dev_csr_create(int device, int n, const int* ia, const int* ja, const double* a, int is_complex,
+                       pgs_dev_csr_t* out);
+int pgs_dev_csr_destroy(pgs_dev_csr_t m);
+int pgs_dev_csr_spmv(pgs_dev_csr_t m, const double* alpha, const double* x, const double* beta, double* y,
+                     void* stream);
+
 /* Uploads the operator once (A, permutation, level factors in level-set
  * order, E, F, W, WH) for the precisions in prec_mask (PGS_PREC_* bits). */
 int pgs_dev_op_create(pgs_gemslr_t op, int device, int prec_mask, pgs_dev_op_t* out);

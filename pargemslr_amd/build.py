@@ -73,7 +73,7 @@ def build(verbose=False):
     _run(["g++"] + SAN_FLAGS + [os.path.join(cpp, "parallel_for_check.cpp"), "-o",
                                 os.path.join(cpp, "parallel_for_check_san")])
     # C++ driver on the class surface (include/pargemslr_amd.hpp), run by the tests
-    for name in ("driver_laplacian", "driver_gen", "pack_check"):
+    for name in ("driver_laplacian", "driver_gen", "driver_surface", "pack_check"):
         drv_src = os.path.join(root, "tests", "cpp", name + ".cpp")
         if os.path.exists(drv_src):
             drv = os.path.join(root, "tests", "cpp", name)

@@ -18,6 +18,7 @@ PGS_PREC_FP32 = 2
 PGS_ORTH_MGS = 0
 PGS_ORTH_CGS2 = 1
 PGS_ORTH_DCGS2 = 2
+PGS_ERROR_INVALID_PARAM = 101
 PGS_ERR_BREAKDOWN = 203
 PGS_ERR_NOT_CONVERGED = 204
 PGS_ERR_NONFINITE = 206
@@ -96,7 +97,8 @@ EXPORTED = [
     "pgs_server_wait", "pgs_server_destroy", "pgs_gemslr_matrix", "pgs_gemslr_save", "pgs_gemslr_load",
     "pgs_read_mm_csr", "pgs_read_mm_vector", "pgs_params_read_inputs", "pgs_hypre_gemslr_create",
     "pgs_hypre_gemslr_params", "pgs_hypre_gemslr_set_precision", "pgs_hypre_gemslr_setup", "pgs_hypre_gemslr_solve",
-    "pgs_hypre_gemslr_destroy",
+    "pgs_hypre_gemslr_destroy", "pgs_vec_axpy", "pgs_vec_scale", "pgs_vec_fill", "pgs_vec_dot",
+    "pgs_dev_csr_create", "pgs_dev_csr_destroy", "pgs_dev_csr_spmv",
 ]
 
 _lib = None
@@ -173,6 +175,13 @@ def lib():
     L.pgs_server_submit.argtypes = [vp, C.c_int, C.c_int, dp, dp, ip, C.POINTER(C.c_int64)]
     L.pgs_server_wait.argtypes = [vp, C.c_int64, C.POINTER(RequestInfo)]
     L.pgs_server_destroy.argtypes = [vp]
+    L.pgs_vec_axpy.argtypes = [C.c_int, C.c_int64, C.c_int, dp, vp, vp, vp]
+    L.pgs_vec_scale.argtypes = [C.c_int, C.c_int64, C.c_int, dp, vp, vp]
+    L.pgs_vec_fill.argtypes = [C.c_int, C.c_int64, C.c_int, dp, vp, vp]
+    L.pgs_vec_dot.argtypes = [C.c_int, C.c_int64, C.c_int, vp, vp, dp, vp]
+    L.pgs_dev_csr_create.argtypes = [C.c_int, C.c_int, ip, ip, dp, C.c_int, C.POINTER(vp)]
+    L.pgs_dev_csr_destroy.argtypes = [vp]
+    L.pgs_dev_csr_spmv.argtypes = [vp, dp, vp, dp, vp, vp]
     _lib = L
     return L
 

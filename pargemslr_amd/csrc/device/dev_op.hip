@@ -1608,6 +1608,191 @@ extern "C" int pgs_device_synchronize(int device) {
   return hip_rc(hipDeviceSynchronize());
 }
 
+// ---------------------------------------------------------------------------
+// device vectors and stand-alone device matrices of the class surface
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kVecBlocks = 1024;  // grid cap of the single-vector kernels (grid-stride loops)
+
+unsigned vec_grid(int64_t n) { return (unsigned)std::min<int64_t>(kVecBlocks, std::max<int64_t>(1, (n + 255) / 256)); }
+
+bool vec_args_ok(int64_t n, const void* a, const void* b) { return n >= 0 && (n == 0 || (a && b)); }
+
+}  // namespace
+
+struct pgs_dev_csr_s {
+  int device = 0;
+  int is_complex = 0;
+  DevCsr A;
+};
+
+extern "C" int pgs_vec_axpy(int device, int64_t n, int is_complex, const double* alpha, const double* x, double* y,
+                            void* stream) {
+  if (!alpha || !vec_args_ok(n, x, y)) return PGS_ERROR_INVALID_PARAM;
+  try {
+    HIPCHK(hipSetDevice(device));
+    if (n == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (is_complex) {
+      using Z = cplx<double>;
+      k_vec_axpy<Z><<<vec_grid(n), 256, 0, s>>>(n, Z(alpha[0], alpha[1]), reinterpret_cast<const Z*>(x),
+                                                reinterpret_cast<Z*>(y));
+    } else {
+      k_vec_axpy<double><<<vec_grid(n), 256, 0, s>>>(n, alpha[0], x, y);
+    }
+    HIPCHK(hipGetLastError());
+  } catch (const HipError& e) {
+    return set_err(e);
+  }
+  return 0;
+}
+
+extern "C" int pgs_vec_scale(int device, int64_t n, int is_complex, const double* alpha, double* x, void* stream) {
+  if (!alpha || !vec_args_ok(n, x, x)) return PGS_ERROR_INVALID_PARAM;
+  try {
+    HIPCHK(hipSetDevice(device));
+    if (n == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (is_complex) {
+      using Z = cplx<double>;
+      k_vec_scale<Z><<<vec_grid(n), 256, 0, s>>>(n, Z(alpha[0], alpha[1]), reinterpret_cast<Z*>(x));
+    } else {
+      k_vec_scale<double><<<vec_grid(n), 256, 0, s>>>(n, alpha[0], x);
+    }
+    HIPCHK(hipGetLastError());
+  } catch (const HipError& e) {
+    return set_err(e);
+  }
+  return 0;
+}
+
+extern "C" int pgs_vec_fill(int device, int64_t n, int is_complex, const double* value, double* x, void* stream) {
+  if (!value || !vec_args_ok(n, x, x)) return PGS_ERROR_INVALID_PARAM;
+  try {
+    HIPCHK(hipSetDevice(device));
+    if (n == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (is_complex) {
+      using Z = cplx<double>;
+      k_vec_fill<Z><<<vec_grid(n), 256, 0, s>>>(n, Z(value[0], value[1]), reinterpret_cast<Z*>(x));
+    } else {
+      k_vec_fill<double><<<vec_grid(n), 256, 0, s>>>(n, value[0], x);
+    }
+    HIPCHK(hipGetLastError());
+  } catch (const HipError& e) {
+    return set_err(e);
+  }
+  return 0;
+}
+
+extern "C" int pgs_vec_dot(int device, int64_t n, int is_complex, const double* x, const double* y, double* result,
+                           void* stream) {
+  if (!result || !vec_args_ok(n, x, y)) return PGS_ERROR_INVALID_PARAM;
+  result[0] = 0.0;
+  if (is_complex) result[1] = 0.0;
+  void* buf = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  try {
+    HIPCHK(hipSetDevice(device));
+    if (n == 0) return 0;
+    const unsigned g = vec_grid(n);
+    // partials (2 per workgroup) and the two sums, stream-ordered so that
+    // concurrent calls on other streams never share them
+    HIPCHK(hipMallocAsync(&buf, sizeof(double) * (2 * (size_t)g + 2), s));
+    double* part = static_cast<double*>(buf);
+    double* out = part + 2 * (size_t)g;
+    if (is_complex) {
+      using Z = cplx<double>;
+      k_vec_dot<Z><<<g, 256, 0, s>>>(n, reinterpret_cast<const Z*>(x), reinterpret_cast<const Z*>(y), part);
+    } else {
+      k_vec_dot<double><<<g, 256, 0, s>>>(n, x, y, part);
+    }
+    HIPCHK(hipGetLastError());
+    k_dot_final_wave<<<1, 256, 0, s>>>((int)g, 2, 1, part, out, 0);
+    HIPCHK(hipGetLastError());
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipFreeAsync(buf, s));
+    buf = nullptr;
+    HIPCHK(hipStreamSynchronize(s));
+    result[0] = h[0];
+    if (is_complex) result[1] = h[1];
+  } catch (const HipError& e) {
+    if (buf) (void)hipFreeAsync(buf, s);
+    return set_err(e);
+  }
+  return 0;
+}
+
+extern "C" int pgs_dev_csr_create(int device, int n, const int* ia, const int* ja, const double* a, int is_complex,
+                                  pgs_dev_csr_t* out) {
+  if (!out || n < 0 || !ia) return PGS_ERROR_INVALID_PARAM;
+  *out = nullptr;
+  if (ia[0] != 0) return PGS_ERROR_INVALID_PARAM;
+  for (int i = 0; i < n; ++i)
+    if (ia[i + 1] < ia[i]) return PGS_ERROR_INVALID_PARAM;
+  const int64_t nnz = ia[n];
+  if (nnz > 0 && (!ja || !a)) return PGS_ERROR_INVALID_PARAM;
+  for (int64_t k = 0; k < nnz; ++k)
+    if (ja[k] < 0 || ja[k] >= n) return PGS_ERROR_INVALID_PARAM;
+  std::unique_ptr<pgs_dev_csr_s> m;
+  try {
+    m.reset(new pgs_dev_csr_s);
+    m->device = device;
+    m->is_complex = is_complex ? 1 : 0;
+    HIPCHK(hipSetDevice(device));
+    if (is_complex) {
+      Csr<zdouble> H;
+      H.nrows = H.ncols = n;
+      H.ia.assign(ia, ia + n + 1);
+      H.ja.assign(ja, ja + nnz);
+      H.a.resize((size_t)nnz);
+      for (int64_t k = 0; k < nnz; ++k) H.a[k] = zdouble(a[2 * k], a[2 * k + 1]);
+      upload_csr<cplx<double>>(m->A, H, nullptr);
+    } else {
+      Csr<double> H;
+      H.nrows = H.ncols = n;
+      H.ia.assign(ia, ia + n + 1);
+      H.ja.assign(ja, ja + nnz);
+      H.a.assign(a, a + nnz);
+      upload_csr<double>(m->A, H, nullptr);
+    }
+  } catch (const HipError& e) {
+    return set_err(e);
+  } catch (const std::bad_alloc&) {
+    return PGS_ERR_OOM;
+  }
+  *out = m.release();
+  return 0;
+}
+
+extern "C" int pgs_dev_csr_destroy(pgs_dev_csr_t m) {
+  if (!m) return 0;
+  (void)hipSetDevice(m->device);
+  delete m;
+  return 0;
+}
+
+extern "C" int pgs_dev_csr_spmv(pgs_dev_csr_t m, const double* alpha, const double* x, const double* beta, double* y,
+                                void* stream) {
+  if (!m || !alpha || !beta || (m->A.nrows > 0 && (!x || !y))) return PGS_ERROR_INVALID_PARAM;
+  try {
+    HIPCHK(hipSetDevice(m->device));
+    if (m->is_complex) {
+      using Z = cplx<double>;
+      launch_spmv<Z, Z, Z, Z>(m->A, reinterpret_cast<const Z*>(x), reinterpret_cast<Z*>(y), (const Z*)nullptr,
+                              Z(alpha[0], alpha[1]), Z(beta[0], beta[1]), 0, 1, (hipStream_t)stream);
+    } else {
+      launch_spmv<double, double, double, double>(m->A, x, y, (const double*)nullptr, alpha[0], beta[0], 0, 1,
+                                                  (hipStream_t)stream);
+    }
+  } catch (const HipError& e) {
+    return set_err(e);
+  }
+  return 0;
+}
+
 extern "C" int pgs_ws_profile(pgs_ws_t w, int enable) {
   if (!w) return PGS_ERROR_INVALID_PARAM;
   pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);

@@ -1072,5 +1072,62 @@ __global__ void k_copy_conv(int64_t m, const S* __restrict__ in, VZ* __restrict_
   if (t < m) out[t] = (VZ)in[t];
 }
 
+// ---------------------------------------------------------------------------
+// single-vector operations of SequentialVectorClass on the device
+// (Axpy / Scale / Fill / Dot, ParGeMSLR/SRC/vectors/sequential_vector.cpp:
+// 706-732, 1037-1172; vectorops.cpp:93-146, 633-680)
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ void k_vec_axpy(int64_t n, S alpha, const S* __restrict__ x, S* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += alpha * x[i];
+}
+
+template <typename S>
+__global__ void k_vec_scale(int64_t n, S alpha, S* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= alpha;
+}
+
+template <typename S>
+__global__ void k_vec_fill(int64_t n, S v, S* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+// Per-workgroup partials of sum_i conj(x_i) y_i: part[blk * 2] (real part),
+// part[blk * 2 + 1] (imaginary part, 0 for real values); summed in a fixed
+// order by k_dot_final_wave with p = 2.  256 threads.
+template <typename S>
+__global__ __launch_bounds__(256) void k_vec_dot(int64_t n, const S* __restrict__ x, const S* __restrict__ y,
+                                                 double* __restrict__ part) {
+  __shared__ double red[2][4];
+  double re = 0.0, im = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const S t = sconj(x[i]) * y[i];
+    if constexpr (scalar_traits<S>::is_complex) {
+      re += t.re;
+      im += t.im;
+    } else {
+      re += t;
+    }
+  }
+#pragma unroll
+  for (int m = kWave / 2; m > 0; m >>= 1) {
+    re += __shfl_xor(re, m);
+    im += __shfl_xor(im, m);
+  }
+  const int wv = threadIdx.x / kWave;
+  if (threadIdx.x % kWave == 0) {
+    red[0][wv] = re;
+    red[1][wv] = im;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const double* r = red[threadIdx.x];
+    part[(int64_t)blockIdx.x * 2 + threadIdx.x] = (r[0] + r[1]) + (r[2] + r[3]);
+  }
+}
+
 }  // namespace dev
 }  // namespace pgs

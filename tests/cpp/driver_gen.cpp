@@ -21,7 +21,7 @@
 using namespace pargemslr_amd;
 
 template <typename T>
-int run(const char* matfile, const char* rhsfile, const double* params, int prec, const char* mode,
+int run(const char* matfile, const char* rhsfile, double* params, int prec, const char* mode,
         const char* opfile) {
   CsrMatrixClass<T> A;
   int err = A.ReadFromMMFile(matfile, 1);

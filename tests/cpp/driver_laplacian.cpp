@@ -17,7 +17,7 @@
 using namespace pargemslr_amd;
 
 template <typename T>
-int run(int nx, int ny, int nz, const double* params, int prec, T shift) {
+int run(int nx, int ny, int nz, double* params, int prec, T shift) {
   CsrMatrixClass<T> A;
   if (A.Laplacian(nx, ny, nz, T(0.0), T(0.0), T(0.0), shift)) return 3;
   const int n = A.GetNumRowsLocal();
