@@ -209,11 +209,18 @@ def test_c5_four_operators_server_mix_equals_alone(gpu):
     for (op, B), (X, its, info) in zip(plan, out):
         A, G, D = ops[op]
         assert info["status"] == 0 and info["nrhs"] == B.shape[1]
-        W = pg.Workspace(D, max_nrhs=B.shape[1], kdim=50)
-        Xa = _dev(np.zeros_like(B), gpu)
-        res = W.fgmres(_dev(B, gpu), Xa, nrhs=B.shape[1], kdim=50, maxits=1000, tol=1e-8, prec=pg.PGS_PREC_FP32,
+        # alone: the request's columns as a block of their own, padded with
+        # zero columns to the next power of two (a zero column takes no
+        # iterations and changes no other column)
+        nr = B.shape[1]
+        npad = 1 << (nr - 1).bit_length()
+        Bp = np.zeros((B.shape[0], npad), dtype=B.dtype)
+        Bp[:, :nr] = B
+        W = pg.Workspace(D, max_nrhs=npad, kdim=50)
+        Xa = _dev(np.zeros_like(Bp), gpu)
+        res = W.fgmres(_dev(Bp, gpu), Xa, nrhs=npad, kdim=50, maxits=1000, tol=1e-8, prec=pg.PGS_PREC_FP32,
                        orth=pg.PGS_ORTH_DCGS2)
-        Xa = _host(Xa)
+        Xa = _host(Xa)[:, :nr]
         W.close()
         for c in range(B.shape[1]):
             b = B[:, c]
