@@ -58,9 +58,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--nrhs", type=int, default=32)
-    ap.add_argument("--fp64-solves", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU time budget of the oracle baseline")
+    ap.add_argument("--single-solves", type=int, default=5, help="timed single-RHS solves per form")
+    ap.add_argument("--fp64-batch-steps", type=int, default=3, help="timed batched solves in the plain FP64 form")
+    ap.add_argument("--profile-steps", type=int, default=2,
+                    help="batched solves with the level-0 solve timed per launch (roofline), after the timed steps")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU time budget of each oracle baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the OpenMP-over-blocks baseline (0: the host's CPUs, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the short serving leg of the default run")
+    ap.add_argument("--serve-requests", type=int, default=12, help="requests of the default run's serving leg")
+    ap.add_argument("--serve-rate", type=float, default=3.0, help="arrivals per second of that leg")
     ap.add_argument("--serve", action="store_true",
                     help="run the serving stream (resident operators, Poisson arrivals) instead")
     ap.add_argument("--requests", type=int, default=32, help="requests of the serving stream")
@@ -72,6 +80,9 @@ def main():
                     help="inner Schur iteration 'maxits,tol' (e.g. 5,1e-2, the reference's default); off if empty")
     ap.add_argument("--probe", action="store_true",
                     help="counter probe: --nrhs FP32 applies of the 128^3 operator and a 1 GiB copy")
+    ap.add_argument("--spmv-probe", action="store_true",
+                    help="one-column SpMV of the 128^3 Laplacian with 8, 4, 2 lanes per row and one thread per row"
+                         " (for a kernel trace; each variant is its own kernel instantiation)")
     ap.add_argument("--restart", default="",
                     help="restart path: save the --config operator to this file, load it, upload it and compare"
                          " the time with the setup")
@@ -85,6 +96,9 @@ def main():
         return
     if args.probe:
         probe(args)
+        return
+    if args.spmv_probe:
+        spmv_probe(args)
         return
     if args.restart:
         restart(args)
@@ -153,15 +167,15 @@ def main():
         x1.zero()
         return W1.fgmres(b1, x1, kdim=kd, maxits=1000, tol=tol, prec=prec, orth=orth)[0]
 
-    def solveB():
+    def solveB(prec=pg.PGS_PREC_FP32, graph=False):
         XB.zero()
-        return WB.fgmres(B, XB, nrhs=nr, kdim=kd, maxits=1000, tol=tol, prec=pg.PGS_PREC_FP32, orth=orth)
+        return WB.fgmres(B, XB, nrhs=nr, kdim=kd, maxits=1000, tol=tol, prec=prec, orth=orth, use_graph=graph)
 
     # plain FP64 form, single RHS
     r64 = solve1(pg.PGS_PREC_FP64)
     pg.synchronize(dev)
     lat64 = []
-    for _ in range(args.fp64_solves):
+    for _ in range(args.single_solves):
         pg.synchronize(dev)
         ts = time.perf_counter()
         r64 = solve1(pg.PGS_PREC_FP64)
@@ -176,7 +190,7 @@ def main():
     r32 = solve1(pg.PGS_PREC_FP32)
     pg.synchronize(dev)
     lat32 = []
-    for _ in range(args.fp64_solves):
+    for _ in range(args.single_solves):
         pg.synchronize(dev)
         ts = time.perf_counter()
         r32 = solve1(pg.PGS_PREC_FP32)
@@ -187,11 +201,10 @@ def main():
     log(f"[bench] mixed single: status {r32[0]} iters {r32[1]} true rel res {res32:.2e} "
         f"median {np.median(lat32) * 1e3:.1f} ms")
 
-    # headline: mixed, batched
+    # headline: mixed, batched, direct launches, no per-launch timing
     for _ in range(args.warmup):
         solveB()
     pg.synchronize(dev)
-    WB.profile(True)
     step_t = []
     pg.synchronize(dev)
     t_start = time.perf_counter()
@@ -203,9 +216,63 @@ def main():
         step_t.append(time.perf_counter() - ts)
     pg.synchronize(dev)
     T = time.perf_counter() - t_start
-    prof = WB.profile_read()
-    WB.profile(False)
     XBh = XB.numpy()
+
+    # the same steps replaying HIP graphs of the inner FGMRES steps (captured
+    # in the first, untimed solve)
+    solveB(graph=True)
+    pg.synchronize(dev)
+    graph_t = []
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        resG = solveB(graph=True)
+        pg.synchronize(dev)
+        graph_t.append(time.perf_counter() - ts)
+    graph_same = [r[:2] for r in resG] == [r[:2] for r in resB] and bool(np.array_equal(XB.numpy(), XBh))
+    log(f"[bench] mixed batched with graph replay: {nr * len(graph_t) / sum(graph_t):.2f} solves/s, "
+        f"same iterations and bits as direct launches: {graph_same}")
+
+    # plain FP64 form, batched (the same columns, FP64 apply): splits the
+    # speedup into batching and precision
+    solveB(prec=pg.PGS_PREC_FP64)
+    pg.synchronize(dev)
+    f64b_t = []
+    for _ in range(args.fp64_batch_steps):
+        ts = time.perf_counter()
+        res64B = solveB(prec=pg.PGS_PREC_FP64)
+        pg.synchronize(dev)
+        f64b_t.append(time.perf_counter() - ts)
+    X64B = XB.numpy()
+    true64B = max(float(np.linalg.norm(Bh[:, c] - A.matvec(X64B[:, c])) / np.linalg.norm(Bh[:, c])) for c in range(nr))
+    ok64B = all(r[0] == 0 for r in res64B) and true64B <= tol
+    fp64_batched = nr * len(f64b_t) / sum(f64b_t)
+    log(f"[bench] fp64 batched nrhs={nr}: {fp64_batched:.2f} solves/s, iters {min(r[1] for r in res64B)}-"
+        f"{max(r[1] for r in res64B)}, max true rel res {true64B:.2e}, ok={ok64B}")
+
+    # per-launch timing (HIP events on the launch stream) of every kernel
+    # family over separate mixed batched solves: the roofline figures
+    WB.profile(True)
+    for _ in range(args.profile_steps):
+        solveB()
+    pg.synchronize(dev)
+    prof = WB.profile_read()
+    fams = WB.profile_families()
+    WB.profile(False)
+    tot_ms = sum(f["total_ms"] for f in fams.values())
+    kernel_table = {}
+    log(f"[bench] kernel families over {args.profile_steps} batched solves (direct launches, events per launch):")
+    for name, f in fams.items():
+        if f["launches"] == 0:
+            continue
+        gbs = f["total_bytes"] / (f["total_ms"] * 1e-3) / 1e9 if f["total_ms"] > 0 and f["total_bytes"] > 0 else None
+        kernel_table[name] = {"launches": f["launches"], "ms": round(f["total_ms"], 3),
+                              "share": round(f["total_ms"] / tot_ms, 4) if tot_ms > 0 else None,
+                              "gb": round(f["total_bytes"] / 1e9, 3),
+                              "gbs": round(gbs, 1) if gbs else None,
+                              "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+        log(f"[bench]   {name:16s} {f['launches']:7d} launches {f['total_ms']:9.2f} ms "
+            f"({100 * f['total_ms'] / tot_ms:5.1f} %)  {f['total_bytes'] / 1e9:8.2f} GB  "
+            f"{(gbs or 0):7.0f} GB/s  {(gbs or 0) / HBM_PEAK_GBS:5.3f} of peak")
     true_res = [float(np.linalg.norm(Bh[:, c] - A.matvec(XBh[:, c])) / np.linalg.norm(Bh[:, c])) for c in range(nr)]
     err_vs_fp64 = float(np.linalg.norm(XBh[:, 0] - x64) / np.linalg.norm(x64))
     err_vs_xstar = float(np.linalg.norm(XBh - Xs) / np.linalg.norm(Xs)) if Xs is not None else None
@@ -228,22 +295,39 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        # the oracle (FP64, 1 thread, modified Gram-Schmidt as the reference's
+        # the oracle (FP64, modified Gram-Schmidt as the reference's
         # FlexGmresClass) solving the same columns to the same tolerance, whole
-        # solves until about --cpu-seconds of CPU work have been timed
+        # solves until about --cpu-seconds have been timed: once on one thread,
+        # once with the diagonal blocks' ILU solves spread over OpenMP threads
+        # (as the reference's SolveB, gemslr.cpp:3653-3655; the results are
+        # the same bits)
         from oracle import oracle as orc
         O = orc.GemslrOracle(n, G.perm(), levels)
-        t_cpu, its_cpu, solves = 0.0, [], 0
-        while solves < nr and (solves == 0 or t_cpu < args.cpu_seconds):
-            ts = time.perf_counter()
-            _, st_c, it_c, _, _ = orc.fgmres(A, Bh[:, solves], prec=O, kdim=kd, maxits=1000, tol=tol, orth=0)
-            t_cpu += time.perf_counter() - ts
-            its_cpu.append(it_c)
-            solves += 1
-        cpu = {"value": round(solves / t_cpu, 5), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"oracle FP64 GeMSLR-FGMRES({kd}), {solves} whole solves of columns 0-{solves - 1} to "
-                         f"{tol} on the same operator ({t_cpu:.1f} s, iterations {its_cpu}, 1 thread)"}
-        log(f"[bench] cpu baseline {solves} solves in {t_cpu:.1f} s -> {cpu['value']} solves/s (iterations {its_cpu})")
+        host_cpus = os.cpu_count() or 1
+        nthreads = args.cpu_threads if args.cpu_threads > 0 else min(16, host_cpus)
+
+        def cpu_leg(threads):
+            orc.set_threads(threads)
+            t_cpu, its_cpu, solves = 0.0, [], 0
+            while solves < nr and (solves == 0 or t_cpu < args.cpu_seconds):
+                ts = time.perf_counter()
+                _, st_c, it_c, _, _ = orc.fgmres(A, Bh[:, solves], prec=O, kdim=kd, maxits=1000, tol=tol, orth=0)
+                t_cpu += time.perf_counter() - ts
+                its_cpu.append(it_c)
+                solves += 1
+            orc.set_threads(1)
+            log(f"[bench] cpu baseline, {threads} thread(s): {solves} solves in {t_cpu:.1f} s -> "
+                f"{solves / t_cpu:.5f} solves/s (iterations {its_cpu})")
+            return solves, t_cpu, its_cpu
+
+        s1, t1, i1 = cpu_leg(1)
+        sN, tN, iN = cpu_leg(nthreads)
+        cpu = {"value": round(sN / tN, 5), "unit": "solves/s", "cores": nthreads, "kind": "port",
+               "sample": f"oracle FP64 GeMSLR-FGMRES({kd}), {sN} whole solves of columns 0-{sN - 1} to {tol} on the "
+                         f"same operator with the diagonal blocks' ILU solves over {nthreads} OpenMP threads "
+                         f"({tN:.1f} s, iterations {iN}); host has {host_cpus} CPUs",
+               "value_1thread": round(s1 / t1, 5), "sample_1thread": f"{s1} solves in {t1:.1f} s on 1 thread",
+               "host_cpus": host_cpus}
 
     out = {
         "metric": "FGMRES solves/s at 1e-8 true relative residual (mixed FP32-apply/FP64-Krylov, batched)",
@@ -267,20 +351,45 @@ def main():
         "latency_ms": {"batched_step_p50": round(float(np.percentile(step_t, 50)) * 1e3, 2),
                        "batched_step_p99": round(float(np.percentile(step_t, 99)) * 1e3, 2),
                        "fp64_single_p50": round(float(np.percentile(lat64, 50)) * 1e3, 2),
-                       "mixed_single_p50": round(float(np.percentile(lat32, 50)) * 1e3, 2)},
+                       "fp64_single_p99": round(float(np.percentile(lat64, 99)) * 1e3, 2),
+                       "mixed_single_p50": round(float(np.percentile(lat32, 50)) * 1e3, 2),
+                       "mixed_single_p99": round(float(np.percentile(lat32, 99)) * 1e3, 2),
+                       "fp64_batched_step_p50": round(float(np.percentile(f64b_t, 50)) * 1e3, 2)},
         "fp64_single_solves_per_s": round(fp64_solves_per_s, 3),
         "mixed_single_solves_per_s": round(1.0 / float(np.median(lat32)), 3),
+        "fp64_batched_solves_per_s": round(fp64_batched, 3),
+        "mixed_batched_graph_solves_per_s": round(nr * len(graph_t) / sum(graph_t), 3),
+        "graph_replay_same_bits": graph_same,
         "speedup_vs_fp64": round(solves_per_s / fp64_solves_per_s, 2),
-        "iters": {"fp64": r64[1], "mixed_single": r32[1], "batched_min": min(iters), "batched_max": max(iters)},
-        "true_rel_res": {"fp64": float(res64), "mixed_single": float(res32), "batched_max": max(true_res)},
+        "speedup_split": {"batching_fp64": round(fp64_batched / fp64_solves_per_s, 2),
+                          "precision_batched": round(solves_per_s / fp64_batched, 2)},
+        "timed_solves": {"fp64_single": len(lat64), "mixed_single": len(lat32), "mixed_batched_steps": args.steps,
+                         "fp64_batched_steps": len(f64b_t)},
+        "iters": {"fp64": r64[1], "mixed_single": r32[1], "batched_min": min(iters), "batched_max": max(iters),
+                  "fp64_batched_min": min(r[1] for r in res64B), "fp64_batched_max": max(r[1] for r in res64B)},
+        "true_rel_res": {"fp64": float(res64), "mixed_single": float(res32), "batched_max": max(true_res),
+                         "fp64_batched_max": true64B},
         "err_mixed_vs_fp64_solution": err_vs_fp64,
         "err_vs_xstar": err_vs_xstar,
-        "converged": ok,
+        "converged": ok and ok64B,
         "setup_s": round(inf.setup_seconds[3], 2),
         "setup_lowrank": setup_lr,
         "roofline": roofline,
+        "kernel_table": kernel_table,
         "cpu_baseline": cpu,
     }
+    if not args.no_serve and args.config == "c2":
+        # a short stream of the serving workload (four resident operators,
+        # Poisson arrivals, 1-16 columns per request): its solves/s and request
+        # latency percentiles beside the batched figures
+        for D_ in (W1, WB, D):
+            D_.close()
+        sv = serving_stream(args.serve_requests, args.serve_rate, bool(args.graphs), orth, None)
+        out["serving"] = {k: sv[k] for k in ("value", "unit", "latency_ms", "batch_nrhs_mean", "max_true_rel_res",
+                                             "converged", "wall_s")}
+        out["serving"]["requests"] = sv["config"]["requests"]
+        out["serving"]["rhs_total"] = sv["config"]["rhs_total"]
+        out["serving"]["workload"] = sv["config"]["workload"]
     print(json.dumps(out), flush=True)
 
 
@@ -333,6 +442,36 @@ def restart(args):
         sys.exit(1)
 
 
+def spmv_probe(args):
+    """y = A x with the 128^3 Laplacian held by itself on the device
+    (pgs_dev_csr), 30 products per lanes-per-row setting; the products of
+    each setting are checked against the first setting's."""
+    import ctypes as C
+
+    import pargemslr_amd as pg
+
+    A = pg.laplacian(128, 128, 128)
+    L = pg.lib()
+    x = pg.DeviceArray.from_numpy(pg.rng_normal(1, A.n))
+    y = pg.DeviceArray(A.n)
+    one = (C.c_double * 2)(1.0, 0.0)
+    zero = (C.c_double * 2)(0.0, 0.0)
+    ref = None
+    for lpr in ("8", "4", "2", "1"):
+        os.environ["PGS_SPMV_LPR"] = lpr
+        h = C.c_void_p()
+        pg.capi._chk(L.pgs_dev_csr_create(0, A.n, pg.capi._iptr(A.ia), pg.capi._iptr(A.ja),
+                                          A.a.ctypes.data_as(C.POINTER(C.c_double)), 0, C.byref(h)), "create")
+        for _ in range(30):
+            pg.capi._chk(L.pgs_dev_csr_spmv(h, one, x.ptr, zero, y.ptr, None), "spmv")
+        pg.synchronize()
+        out = y.numpy()
+        ref = out if ref is None else ref
+        log(f"[spmv-probe] lanes per row {lpr}: max |y - y(8 lanes)| {np.abs(out - ref).max():.2e}")
+        L.pgs_dev_csr_destroy(h)
+    del os.environ["PGS_SPMV_LPR"]
+
+
 def probe(args):
     """Workload for hardware-counter passes (rocprofv3 --pmc): one warm-up
     apply of the 128^3 operator (FP32, --nrhs right-hand sides), a 1 GiB
@@ -361,6 +500,16 @@ def probe(args):
 
 
 def serve(args):
+    """The serving stream alone (--serve): one JSON line."""
+    orth = {"mgs": 0, "cgs2": 1, "dcgs2": 2}[args.orth]
+    schur = None
+    if args.schur:
+        k, t = args.schur.split(",")
+        schur = (int(k), float(t))
+    print(json.dumps(serving_stream(args.requests, args.rate, bool(args.graphs), orth, schur)), flush=True)
+
+
+def serving_stream(requests, rate, graphs, orth, schur):
     """Serving stream: four resident operators (2D Laplacian 1024^2, 3D
     Laplacian 64^3, upwind convection-diffusion 96^3 with c = 0.3, complex-
     shifted 3D Laplacian 80^3 with shift 0.2+0.02i; GeMSLR 3 levels, 64
@@ -383,10 +532,6 @@ def serve(args):
             return pg.rng_normal(seed, 2 * A.n).view(np.complex128)
         return pg.rng_normal(seed, A.n)
 
-    schur = None
-    if args.schur:
-        k, t = args.schur.split(",")
-        schur = (int(k), float(t))
     params = pg.config_params(3, 64, 1e-3, 20, 16, kdim=50, schur=schur)
     ops = []
     for name, A in mats:
@@ -395,19 +540,17 @@ def serve(args):
         D = pg.DevOp(G, prec_mask=pg.PGS_PREC_FP32)
         ops.append((name, A, G, D))
         log(f"[serve] {name}: n={A.n} setup {time.time() - t0:.1f}s, device {D.bytes() / 1e9:.2f} GB")
-    orth = {"mgs": pg.PGS_ORTH_MGS, "cgs2": pg.PGS_ORTH_CGS2, "dcgs2": pg.PGS_ORTH_DCGS2}[args.orth]
-    S = pg.Server(max_batch=16, slots=3, mem_reserve=8 << 30, kdim=50, tol=1e-8, orth=orth,
-                  use_graph=bool(args.graphs))
+    S = pg.Server(max_batch=16, slots=3, mem_reserve=8 << 30, kdim=50, tol=1e-8, orth=orth, use_graph=graphs)
     ids = [S.add_operator(D) for _, _, _, D in ops]
     # warm-up: one request per operator (workspaces, graphs)
     for i, (_, A, _, _) in enumerate(ops):
         S.wait(S.submit(ids[i], rhs(7, A)))
     plan = []
-    for k in range(args.requests):
+    for k in range(requests):
         op = int(rng.integers(0, len(ops)))
         nr = int(rng.integers(1, 17))
         B = np.stack([rhs(100000 + 64 * k + c, ops[op][1]) for c in range(nr)], axis=1)
-        plan.append((op, B, float(rng.exponential(1.0 / args.rate))))
+        plan.append((op, B, float(rng.exponential(1.0 / rate))))
     reqs = [None] * len(plan)
 
     def feed():
@@ -445,15 +588,17 @@ def serve(args):
         "value": round(cols / T, 3), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32-apply/f64-krylov (complex64/complex128 for the complex operator)", "data": "synthetic",
         "config": {"workload": "serving stream: " + ", ".join(n for n, _, _, _ in ops),
-                   "requests": len(plan), "rhs_total": cols, "arrival_rate_per_s": args.rate,
-                   "max_batch": 16, "slots": 3, "graphs": bool(args.graphs),
+                   "requests": len(plan), "rhs_total": cols, "arrival_rate_per_s": rate,
+                   "max_batch": 16, "slots": 3, "graphs": graphs,
                    "inner_schur": list(schur) if schur else None},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
                        "mean_queue": round(float(np.mean([i["queue_ms"] for _, _, i in out])), 2)},
         "batch_nrhs_mean": round(float(np.mean([i["batch_nrhs"] for _, _, i in out])), 2),
         "per_operator": per_op, "max_true_rel_res": worst, "converged": ok, "wall_s": round(T, 2),
     }
-    print(json.dumps(line), flush=True)
+    for _, _, _, D in ops:
+        D.close()
+    return line
 
 
 if __name__ == "__main__":

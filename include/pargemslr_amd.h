@@ -285,18 +285,43 @@ int pgs_fgmres(pgs_dev_op_t dop, pgs_ws_t ws, const double* b, double* x, int nr
                pgs_fgmres_result_t* res, void* stream);
 
 /* Kernel timing for the roofline report (no counterpart in the reference).
- * When enabled, the workspace records a HIP event pair around every launch of
- * the level-0 block triangular solve it issues (on the launch stream) and
- * accumulates the algorithmic bytes of each launch:
- *   (nnz(L_0) + nnz(U_0)) * (s + 4) + 2 * (n_0 + nblocks) * 4 + n_0 * s + 2 * n_0 * nrhs * s
- * (s = value bytes).  Reading synchronizes the recorded events. */
+ * When enabled, the workspace records a HIP event pair (on the launch stream)
+ * around every launch of the solve path it issues and accumulates, per kernel
+ * family, the launches, their time and their algorithmic bytes (s = value
+ * bytes of the launch, vb = n * nrhs * s of the Krylov scalar):
+ *   PGS_PROF_TRSV0 / PGS_PROF_TRSV   block triangular solves of level 0 / the levels below:
+ *       (nnz(L) + nnz(U)) * (s + 4) + 2 * (n_l + 1) * 4 + n_l * s + 2 * n_l * nrhs * s
+ *   PGS_PROF_SPMV_A / PGS_PROF_SPMV_EF   y = A x (+ r) / the coupling blocks E, F:
+ *       nnz * (s + 4) + (rows + 1) * 4 + cols * nrhs * s_x + rows * nrhs * s_y (x2 when y or r is read)
+ *   PGS_PROF_LOWRANK    (2 k + 3 nrhs) * m * s
+ *   PGS_PROF_PERM       n * nrhs * (s_in + s_out) + 4 n (gather or scatter)
+ *   PGS_PROF_ORTH_DOTS / PGS_PROF_ORTH_UPDATE   basis sweeps: (vectors read + written) * vb
+ *   PGS_PROF_SCALE      2 vb;  PGS_PROF_UPDATE_X  (i * s_z / s + 2) * vb
+ *   PGS_PROF_STATE      the one-workgroup Givens / Hessenberg kernels (0 bytes)
+ * Timed solves launch directly (no graph replay).  Reading synchronizes the
+ * recorded events. */
+#define PGS_PROF_TRSV0 0
+#define PGS_PROF_TRSV 1
+#define PGS_PROF_SPMV_A 2
+#define PGS_PROF_SPMV_EF 3
+#define PGS_PROF_LOWRANK 4
+#define PGS_PROF_PERM 5
+#define PGS_PROF_ORTH_DOTS 6
+#define PGS_PROF_ORTH_UPDATE 7
+#define PGS_PROF_SCALE 8
+#define PGS_PROF_UPDATE_X 9
+#define PGS_PROF_STATE 10
+#define PGS_PROFILE_FAMILIES 11
 typedef struct {
   int64_t launches;
   double total_ms;
   double total_bytes;
 } pgs_kernel_profile_t;
 int pgs_ws_profile(pgs_ws_t ws, int enable);
+/* the level-0 block triangular solve (family PGS_PROF_TRSV0) */
 int pgs_ws_profile_read(pgs_ws_t ws, pgs_kernel_profile_t* out);
+/* every family: out[PGS_PROFILE_FAMILIES], indexed by PGS_PROF_* */
+int pgs_ws_profile_families(pgs_ws_t ws, pgs_kernel_profile_t* out);
 
 const char* pgs_strerror(int code);
 /* message of the last HIP error seen by this thread */
@@ -308,11 +333,15 @@ const char* pgs_last_hip_error(void);
  * right-hand sides are queued in arrival order.  Each of `slots` workers (one
  * stream and one host thread each) takes the oldest request plus every later
  * queued request for the same operator that fits in max_batch columns and
- * solves them together with pgs_fgmres, so work of several batches overlaps on
- * the device and requests arriving meanwhile join the next free worker.  A
- * worker keeps one workspace per operator it has served; it admits a new one
- * only when the device's free memory minus mem_reserve holds it, dropping its
- * own idle workspaces and then waiting for other batches if needed. */
+ * runs them as one batch of columns, one FGMRES restart cycle (pgs_fgmres,
+ * at most kdim iterations) at a time: at each restart, columns that have
+ * converged or failed leave (a request completes when all of its columns
+ * have), and queued requests for the same operator take the free columns.
+ * Work of several batches overlaps on the device.  The iteration cap maxits
+ * is applied at restart boundaries.  A worker keeps one workspace per
+ * operator it has served; it admits a new one only when the device's free
+ * memory minus mem_reserve holds it, dropping its own idle workspaces and
+ * then waiting for other batches if needed. */
 typedef struct pgs_server_s* pgs_server_t;
 typedef struct {
   int max_batch;       /* most right-hand sides per batch (power of two <= 64) */
@@ -331,6 +360,7 @@ typedef struct {
   double queue_ms;     /* submit to batch start */
   double solve_ms;     /* batch start to results written */
   double total_ms;     /* submit to results written */
+  int joined_cycle;    /* restart cycle of its batch at which it joined (0: the batch started with it) */
 } pgs_request_info_t;
 int pgs_server_create(int device, const pgs_server_opts_t* opts, pgs_server_t* out);
 /* the operator must outlive the server */

@@ -50,6 +50,8 @@ typedef struct {
   const void *W, *WH; /* nrem x rank, column-major */
   const int *C_ia, *C_ja; /* level 0 with the inner iteration: nrem x nrem lower-right block */
   const void* C_a;
+  int ncomp;             /* diagonal blocks of the level (0: solve the level as one block) */
+  const int* dom_ptr;    /* ncomp + 1 block row offsets within the level */
 } orc_level;
 
 typedef struct {
@@ -59,6 +61,12 @@ typedef struct {
   int inner_maxits;  /* > 0: inner FGMRES on the level-0 Schur complement */
   double inner_tol;
 } orc_gemslr;
+
+/* threads over which a level's diagonal blocks are solved (the CPU
+ * baseline's OpenMP-over-blocks variant); 1 = serial.  Every row is computed
+ * in the same order either way, so the results are the same bits. */
+static int orc_threads = 1;
+void orc_set_threads(int t) { orc_threads = t < 1 ? 1 : t; }
 
 /* workspace size (values) of orc_gemslr_apply / orc_gemslr_apply_z */
 long orc_gemslr_work_size(const orc_gemslr* op) {

@@ -22,7 +22,8 @@ class OrcLevel(C.Structure):
                 ("L_a", C.c_void_p), ("U_a", C.c_void_p), ("Dinv", C.c_void_p),
                 ("E_ia", C.c_void_p), ("E_ja", C.c_void_p), ("F_ia", C.c_void_p), ("F_ja", C.c_void_p),
                 ("E_a", C.c_void_p), ("F_a", C.c_void_p), ("W", C.c_void_p), ("WH", C.c_void_p),
-                ("C_ia", C.c_void_p), ("C_ja", C.c_void_p), ("C_a", C.c_void_p)]
+                ("C_ia", C.c_void_p), ("C_ja", C.c_void_p), ("C_a", C.c_void_p),
+                ("ncomp", C.c_int), ("dom_ptr", C.c_void_p)]
 
 
 class OrcGemslr(C.Structure):
@@ -46,6 +47,7 @@ def lib():
         vp = C.c_void_p
         L.orc_gemslr_work_size.argtypes = [C.POINTER(OrcGemslr)]
         L.orc_gemslr_work_size.restype = C.c_long
+        L.orc_set_threads.argtypes = [C.c_int]
         for sfx in ("", "_z"):
             getattr(L, "orc_csr_matvec" + sfx).argtypes = [C.c_int, vp, vp, vp, C.c_double, vp, C.c_double, vp]
             getattr(L, "orc_ilu_solve" + sfx).argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -140,6 +142,9 @@ class GemslrOracle:
                 setattr(o, nm + "_ja", _p(self._k(_i32(ja))))
                 setattr(o, nm + "_a", _p(self._k(_val(a, cz))))
             o.Dinv = _p(self._k(_val(d["Dinv"], cz)))
+            if d.get("dom_ptr") is not None:
+                o.ncomp = int(d["ncomp"])
+                o.dom_ptr = _p(self._k(_i32(d["dom_ptr"])))
             if d["rank"] > 0:
                 o.W = C.c_void_p(self._k(np.asfortranarray(d["W"], dtype=self.dtype)).ctypes.data)
                 o.WH = C.c_void_p(self._k(np.asfortranarray(d["WH"], dtype=self.dtype)).ctypes.data)
@@ -179,6 +184,13 @@ def fgmres(A, b, x0=None, prec=None, kdim=50, maxits=1000, tol=1e-8, absolute=Fa
     st = _fn("orc_fgmres", cz)(A.n, _p(ia), _p(ja), _p(a), C.byref(prec.s) if prec is not None else None, _p(b),
                                _p(x), kdim, maxits, tol, int(absolute), orth, C.byref(it), C.byref(rr), _p(hist))
     return x, st, it.value, rr.value, hist[: it.value + 1]
+
+
+def set_threads(t):
+    """Threads over which the oracle solves a level's diagonal blocks (the
+    CPU baseline's OpenMP-over-blocks variant; the results are the same
+    bits); 1 = serial, the default."""
+    lib().orc_set_threads(int(t))
 
 
 def mgs(V, k, H, cgs2=False):

@@ -39,6 +39,35 @@ void ORC_FN(orc_ilu_solve)(int n, const int* L_ia, const int* L_ja, const ORC_T*
   }
 }
 
+/* The block-diagonal ILU solve of one level, one diagonal block at a time
+ * (GemslrClass::SolveB, SRC/preconditioners/gemslr.cpp:3632-3674, each block
+ * by IluClass::SolveHost, ilu.cpp:3677-3786), the blocks spread over
+ * orc_threads OpenMP threads as SolveB's loop (gemslr.cpp:3653-3655).  A
+ * block's rows reference only its own rows, and every row is computed as in
+ * orc_ilu_solve. */
+static void ORC_FN(level_ilu_solve)(const orc_level* L, const ORC_T* rhs, ORC_T* x) {
+  const ORC_T* La = (const ORC_T*)L->L_a;
+  const ORC_T* Ua = (const ORC_T*)L->U_a;
+  const ORC_T* Dinv = (const ORC_T*)L->Dinv;
+  int b;
+  if (!L->dom_ptr || L->ncomp <= 1 || orc_threads <= 1) {
+    ORC_FN(orc_ilu_solve)(L->nloc, L->L_ia, L->L_ja, La, L->U_ia, L->U_ja, Ua, Dinv, rhs, x);
+    return;
+  }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(orc_threads)
+  for (b = 0; b < L->ncomp; ++b) {
+    const int r0 = L->dom_ptr[b], r1 = L->dom_ptr[b + 1];
+    int i, j;
+    for (i = r0; i < r1; ++i) x[i] = rhs[i];
+    for (i = r0; i < r1; ++i)
+      for (j = L->L_ia[i]; j < L->L_ia[i + 1]; ++j) x[i] -= La[j] * x[L->L_ja[j]];
+    for (i = r1 - 1; i >= r0; --i) {
+      for (j = L->U_ia[i]; j < L->U_ia[i + 1]; ++j) x[i] -= Ua[j] * x[L->U_ja[j]];
+      x[i] *= Dinv[i];
+    }
+  }
+}
+
 /* x = WH * (W^H * z): DenseMatrixPMatVecTemplate 'C' then 'N'
  * (SRC/matrices/matrixops.cpp:101-332; the 'C' product conjugates W,
  * matrixops.cpp:260), as GemslrClass::SolveApplyLowRankLevel
@@ -194,13 +223,10 @@ out:
 static void ORC_FN(solve_level)(const orc_gemslr* op, int l, const ORC_T* rhs, ORC_T* x, ORC_T* work) {
   const orc_level* L = &op->levels[l];
   const int nl = L->nloc, m = L->nrem;
-  const ORC_T* La = (const ORC_T*)L->L_a;
-  const ORC_T* Ua = (const ORC_T*)L->U_a;
-  const ORC_T* Dinv = (const ORC_T*)L->Dinv;
   ORC_T *z, *zl, *xlr, *t;
   int i;
   if (l == op->nlev - 1) {
-    ORC_FN(orc_ilu_solve)(nl, L->L_ia, L->L_ja, La, L->U_ia, L->U_ja, Ua, Dinv, rhs, x);
+    ORC_FN(level_ilu_solve)(L, rhs, x);
     return;
   }
   z = work;
@@ -208,7 +234,7 @@ static void ORC_FN(solve_level)(const orc_gemslr* op, int l, const ORC_T* rhs, O
   xlr = z + nl + m;
   t = xlr + m;
   /* step 1: z_u = B^{-1} rhs_u (block-diagonal ILU, blocks are independent) */
-  ORC_FN(orc_ilu_solve)(nl, L->L_ia, L->L_ja, La, L->U_ia, L->U_ja, Ua, Dinv, rhs, z);
+  ORC_FN(level_ilu_solve)(L, rhs, z);
   /* steps 2-3: z_l = -E z_u; z_l += rhs_l */
   ORC_FN(orc_csr_matvec)(m, L->E_ia, L->E_ja, (const ORC_T*)L->E_a, -1.0, z, 0.0, zl);
   for (i = 0; i < m; ++i) zl[i] += rhs[nl + i];
@@ -226,7 +252,7 @@ static void ORC_FN(solve_level)(const orc_gemslr* op, int l, const ORC_T* rhs, O
   ORC_FN(orc_csr_matvec)(nl, L->F_ia, L->F_ja, (const ORC_T*)L->F_a, -1.0, x + nl, 0.0, z);
   for (i = 0; i < nl; ++i) z[i] += rhs[i];
   /* step 7: x_u = B^{-1} z_u */
-  ORC_FN(orc_ilu_solve)(nl, L->L_ia, L->L_ja, La, L->U_ia, L->U_ja, Ua, Dinv, z, x);
+  ORC_FN(level_ilu_solve)(L, z, x);
 }
 
 /* z = M^{-1} v: GatherPerm, SolveLevel(0), ScatterRperm

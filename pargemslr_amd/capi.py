@@ -82,7 +82,7 @@ class ServerOpts(C.Structure):
 
 class RequestInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("nrhs", C.c_int), ("batch_nrhs", C.c_int), ("queue_ms", C.c_double),
-                ("solve_ms", C.c_double), ("total_ms", C.c_double)]
+                ("solve_ms", C.c_double), ("total_ms", C.c_double), ("joined_cycle", C.c_int)]
 
 
 # every extern "C" entry point of include/pargemslr_amd.h
@@ -98,8 +98,12 @@ EXPORTED = [
     "pgs_read_mm_csr", "pgs_read_mm_vector", "pgs_params_read_inputs", "pgs_hypre_gemslr_create",
     "pgs_hypre_gemslr_params", "pgs_hypre_gemslr_set_precision", "pgs_hypre_gemslr_setup", "pgs_hypre_gemslr_solve",
     "pgs_hypre_gemslr_destroy", "pgs_vec_axpy", "pgs_vec_scale", "pgs_vec_fill", "pgs_vec_dot",
-    "pgs_dev_csr_create", "pgs_dev_csr_destroy", "pgs_dev_csr_spmv",
+    "pgs_dev_csr_create", "pgs_dev_csr_destroy", "pgs_dev_csr_spmv", "pgs_ws_profile_families",
 ]
+
+# kernel families of pgs_ws_profile_families, by index
+PROFILE_FAMILIES = ["trsv level 0", "trsv levels 1+", "spmv A", "spmv E/F", "low-rank", "permute", "orth dots",
+                    "orth update", "scale", "x update", "state"]
 
 _lib = None
 
@@ -182,6 +186,7 @@ def lib():
     L.pgs_dev_csr_create.argtypes = [C.c_int, C.c_int, ip, ip, dp, C.c_int, C.POINTER(vp)]
     L.pgs_dev_csr_destroy.argtypes = [vp]
     L.pgs_dev_csr_spmv.argtypes = [vp, dp, vp, dp, vp, vp]
+    L.pgs_ws_profile_families.argtypes = [vp, C.POINTER(KernelProfile)]
     _lib = L
     return L
 
@@ -656,6 +661,14 @@ class Workspace:
         _chk(lib().pgs_ws_profile_read(self._h, C.byref(p)), "pgs_ws_profile_read")
         return dict(launches=p.launches, total_ms=p.total_ms, total_bytes=p.total_bytes)
 
+    def profile_families(self):
+        """Per kernel family (PROFILE_FAMILIES): launches, ms and algorithmic
+        bytes recorded since profile(True)."""
+        arr = (KernelProfile * len(PROFILE_FAMILIES))()
+        _chk(lib().pgs_ws_profile_families(self._h, arr), "pgs_ws_profile_families")
+        return {name: dict(launches=p.launches, total_ms=p.total_ms, total_bytes=p.total_bytes)
+                for name, p in zip(PROFILE_FAMILIES, arr)}
+
     def spmv(self, x, y, alpha=1.0, beta=0.0, nrhs=1, stream=None):
         """y = alpha A x + beta y (alpha, beta complex for a complex operator)."""
         alpha, beta = complex(alpha), complex(beta)
@@ -718,7 +731,7 @@ class Server:
         _chk(lib().pgs_server_wait(self._h, req, C.byref(info)), "pgs_server_wait")
         _, X, it = self._live.pop(req)
         return X, it, dict(status=info.status, nrhs=info.nrhs, batch_nrhs=info.batch_nrhs, queue_ms=info.queue_ms,
-                           solve_ms=info.solve_ms, total_ms=info.total_ms)
+                           solve_ms=info.solve_ms, total_ms=info.total_ms, joined_cycle=info.joined_cycle)
 
     def close(self):
         if self._h:

@@ -9,6 +9,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../../include/pargemslr_amd.h"
@@ -108,6 +109,12 @@ void upload_csr(DevCsr& d, const Csr<T>& A, int64_t* counter) {
   upload(d.a, convert_vec<V>(A.a), counter);
   double avg = A.nrows > 0 ? (double)A.nnz() / A.nrows : 0.0;
   d.lpr = avg <= 2 ? 2 : avg <= 4 ? 4 : avg <= 8 ? 8 : avg <= 16 ? 16 : avg <= 32 ? 32 : 64;
+  // PGS_SPMV_LPR=1, 2, 4, 8 ... fixes the lanes per row of one-column
+  // products (1: one thread per row, eight entries in flight)
+  if (const char* e = std::getenv("PGS_SPMV_LPR")) {
+    const int v = std::atoi(e);
+    if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32 || v == 64) d.lpr = v;
+  }
 }
 
 // Pipelined triangular-solve geometry: 512 threads, 4 lanes per row, 5 slots
@@ -223,6 +230,7 @@ struct pgs_dev_op_s {
   // half steps); 128 / 64 with ring_lpr 4 or 2 force one geometry
   int ring_nt = 0;
   int ring_lpr = 4;
+  int lr_mfma = 1;          // low-rank term of FP32 blocks of 16, 32 or 64 columns on the matrix cores
   int inner_maxits = 0;     // > 0: inner FGMRES on the level-0 Schur complement in every apply
   double inner_tol = 0.0;
   int64_t bytes = 0;
@@ -252,12 +260,15 @@ struct pgs_ws_s {
   int part_blocks = 0;
   bool fused_cgs2 = true;  // CGS2 as three k_basis_reg sweeps instead of four
   int64_t v_bytes = 0;
-  // kernel timing of the level-0 triangular solve
+  // kernel timing by family (pgs_ws_profile): HIP events on the launch
+  // stream around every launch, and each launch's algorithmic bytes
   bool prof = false;
   std::vector<hipEvent_t> ev;         // pool, used in start/stop pairs
+  std::vector<int> ev_fam;            // family of each recorded pair
   size_t ev_used = 0;
-  double prof_bytes = 0.0, prof_ms = 0.0;
-  int64_t prof_launches = 0;
+  double fam_ms[PGS_PROFILE_FAMILIES] = {};
+  double fam_bytes[PGS_PROFILE_FAMILIES] = {};
+  int64_t fam_launches[PGS_PROFILE_FAMILIES] = {};
   // captured inner FGMRES steps (index = step), the settings they were
   // recorded with, and the stream they are recorded on
   std::vector<hipGraphExec_t> graph_exec;
@@ -280,7 +291,7 @@ namespace {
 constexpr int kNT = 256;
 constexpr int kTrsvNT = 256;
 constexpr int kPMAX = 128;  // Krylov dimension + 1 limit of the shared-memory coefficient tables
-constexpr int kLrBlocks = 256;  // most workgroups of the low-rank partial sums
+constexpr int kLrBlocks = 1024;  // most workgroups (partial sums) of the low-rank stage 1
 constexpr int kBasisNT = 256;
 constexpr int kBasisPM = 64;  // most basis vectors k_basis_reg takes
 constexpr int kPartRowsDcgs2 = 2 * 32 + 2;  // outputs per column of one k_dcgs2_dots chunk
@@ -303,7 +314,11 @@ template <typename VA, typename VX, typename VY, typename VACC>
 void launch_spmv(const DevCsr& M, const VX* x, VY* y, const VY* r, VACC alpha, VACC beta, int mode, int nrhs,
                  hipStream_t s) {
   if (M.nrows == 0) return;
-  if (nrhs == 1) {
+  if (nrhs == 1 && M.lpr == 1) {
+    const unsigned g = (nblocks_for(M.nrows, kNT) + kXcds - 1) / kXcds * kXcds;
+    k_spmvN<VA, VX, VY, VACC><<<g, kNT, 0, s>>>(M.nrows, 1, M.ia.as<int>(), M.ja.as<int>(), M.a.as<VA>(), x, y, r,
+                                                alpha, beta, mode);
+  } else if (nrhs == 1) {
     const int64_t threads = (int64_t)M.nrows * M.lpr;
     const unsigned g = (nblocks_for(threads, kNT) + kXcds - 1) / kXcds * kXcds;  // see k_spmvN
 #define PGS_SPMV_CASE(L) \
@@ -334,48 +349,71 @@ void launch_spmv(const DevCsr& M, const VX* x, VY* y, const VY* r, VACC alpha, V
   HIPCHK(hipGetLastError());
 }
 
-// Folds the recorded event pairs into the totals (synchronizes them).
+// Folds the recorded event pairs into the family totals (synchronizes them).
 void prof_flush(pgs_ws_s* ws) {
   for (size_t i = 0; i + 1 < ws->ev_used; i += 2) {
     HIPCHK(hipEventSynchronize(ws->ev[i + 1]));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ws->ev[i], ws->ev[i + 1]));
-    ws->prof_ms += ms;
+    ws->fam_ms[ws->ev_fam[i / 2]] += ms;
   }
   ws->ev_used = 0;
+  ws->ev_fam.clear();
 }
 
-hipEvent_t prof_event(pgs_ws_s* ws) {
-  if (ws->ev_used == ws->ev.size()) {
+// Start of a timed launch: an event pair is reserved whole, so a flush of a
+// full pool never splits a pair.
+void prof_begin(pgs_ws_s* ws, hipStream_t s) {
+  if (ws->ev_used + 2 > ws->ev.size()) {
     if (ws->ev.size() >= 4096) {
       prof_flush(ws);
     } else {
-      hipEvent_t e;
-      HIPCHK(hipEventCreate(&e));
-      ws->ev.push_back(e);
+      for (int k = 0; k < 2; ++k) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        ws->ev.push_back(e);
+      }
     }
   }
-  return ws->ev[ws->ev_used++];
+  HIPCHK(hipEventRecord(ws->ev[ws->ev_used++], s));
+}
+
+void prof_end(pgs_ws_s* ws, hipStream_t s, int fam, double bytes) {
+  HIPCHK(hipEventRecord(ws->ev[ws->ev_used++], s));
+  ws->ev_fam.push_back(fam);
+  ws->fam_bytes[fam] += bytes;
+  ws->fam_launches[fam] += 1;
+}
+
+// A launch (or a few launches of one operation) timed into family `fam` when
+// the workspace profiler is on.
+template <typename F>
+void timed(pgs_ws_s* ws, hipStream_t s, int fam, double bytes, F&& launch) {
+  const bool on = ws && ws->prof;
+  if (on) prof_begin(ws, s);
+  launch();
+  if (on) prof_end(ws, s, fam, bytes);
+}
+
+// algorithmic bytes of y = M x (+ y or r) over nrhs columns
+template <typename VA, typename VX, typename VY>
+double spmv_bytes(const DevCsr& M, int nrhs, bool reads_y) {
+  return (double)M.nnz * (sizeof(VA) + 4.0) + (M.nrows + 1.0) * 4.0 + (double)M.ncols * nrhs * sizeof(VX) +
+         (double)M.nrows * nrhs * sizeof(VY) * (reads_y ? 2.0 : 1.0);
 }
 
 template <typename V>
 void launch_trsv_impl(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, int nrhs, hipStream_t s);
 
-// Level-set triangular solves of one GeMSLR level; the level-0 launches are
-// timed when the workspace profiler is on.
+// Level-set triangular solves of one GeMSLR level (family 0 on level 0, 1
+// below it when the workspace profiler is on).
 template <typename V>
 void launch_trsv(pgs_dev_op_s* op, pgs_ws_s* ws, int level, const DevTrsv& T, const V* rhs, V* x, int nrhs,
                  hipStream_t s) {
-  const bool timed = ws && ws->prof && level == 0;
-  if (timed) HIPCHK(hipEventRecord(prof_event(ws), s));
-  launch_trsv_impl<V>(op, T, rhs, x, nrhs, s);
-  if (timed) {
-    HIPCHK(hipEventRecord(prof_event(ws), s));
-    const double sv = sizeof(V);
-    ws->prof_bytes += (double)T.nnz_real * (sv + 4.0) + 2.0 * (T.nloc + 1) * 4.0 + T.nloc * sv +
-                      2.0 * (double)T.nloc * nrhs * sv;
-    ws->prof_launches += 1;
-  }
+  const double sv = sizeof(V);
+  const double bytes = (double)T.nnz_real * (sv + 4.0) + 2.0 * (T.nloc + 1) * 4.0 + T.nloc * sv +
+                       2.0 * (double)T.nloc * nrhs * sv;
+  timed(ws, s, level == 0 ? PGS_PROF_TRSV0 : PGS_PROF_TRSV, bytes, [&] { launch_trsv_impl<V>(op, T, rhs, x, nrhs, s); });
 }
 
 template <typename V, bool XLDS, int NT, int LPR>
@@ -614,10 +652,44 @@ void set_kernel_attrs(int lds_max) {
   });
 }
 
+// The low-rank term of a block of 16, 32 or 64 FP32 columns on the matrix
+// cores (k_lr_dot_mfma, k_lr_final, k_lr_update_mfma); k <= 32.
+template <int KT, int CT>
+void launch_lowrank_mfma_kt(const DevLevel& L, float* zl, float* part, float* t, int nrhs, hipStream_t s) {
+  const int m = L.nrem, k = L.rank;
+  // stage 1: at most kLrBlocks workgroups of four waves, rows per wave a
+  // multiple of 4 and at least 64
+  int64_t rpw = std::max<int64_t>(64, ((int64_t)m + 4LL * kLrBlocks - 1) / (4LL * kLrBlocks));
+  rpw = (rpw + 3) / 4 * 4;
+  const int nblk = (int)(((int64_t)m + 4 * rpw - 1) / (4 * rpw));
+  k_lr_dot_mfma<KT, CT><<<nblk, 256, 0, s>>>(m, k, nrhs, L.W.as<float>(), zl, part, (int)rpw);
+  k_lr_final<float><<<(k * nrhs + 3) / 4, 256, 0, s>>>(nblk, k, nrhs, part, t);
+  const int64_t tiles = ((int64_t)m + 15) / 16;
+  k_lr_update_mfma<KT, CT><<<(unsigned)((tiles + 3) / 4), 256, 0, s>>>(m, k, nrhs, L.WH.as<float>(), t, zl);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_lowrank_mfma(pgs_ws_s* ws, const DevLevel& L, float* zl, int nrhs, hipStream_t s) {
+  float* part = ws->lrpart.as<float>();
+  float* t = ws->lrt.as<float>();
+  const bool k16 = L.rank <= 16;
+  switch (nrhs) {
+    case 16: k16 ? launch_lowrank_mfma_kt<1, 1>(L, zl, part, t, nrhs, s) : launch_lowrank_mfma_kt<2, 1>(L, zl, part, t, nrhs, s); break;
+    case 32: k16 ? launch_lowrank_mfma_kt<1, 2>(L, zl, part, t, nrhs, s) : launch_lowrank_mfma_kt<2, 2>(L, zl, part, t, nrhs, s); break;
+    default: k16 ? launch_lowrank_mfma_kt<1, 4>(L, zl, part, t, nrhs, s) : launch_lowrank_mfma_kt<2, 4>(L, zl, part, t, nrhs, s); break;
+  }
+}
+
 template <typename V>
 void launch_lowrank(pgs_ws_s* ws, const DevLevel& L, V* zl, int nrhs, hipStream_t s) {
   const int m = L.nrem, k = L.rank;
   if (k <= 0 || m <= 0) return;
+  if constexpr (std::is_same<V, float>::value) {
+    if (ws->op->lr_mfma && k <= 32 && (nrhs == 16 || nrhs == 32 || nrhs == 64)) {
+      launch_lowrank_mfma(ws, L, zl, nrhs, s);
+      return;
+    }
+  }
   V* part = ws->lrpart.as<V>();
   V* t = ws->lrt.as<V>();
   int nblk;
@@ -778,18 +850,24 @@ void solve_level(pgs_dev_op_s* op, pgs_ws_s* ws, DevPrec<V>& P, int l, const V* 
   // z_u = B^{-1} rhs_u
   launch_trsv<V>(op, ws, l, L.B, rhs, z, nrhs, s);
   // z_l = rhs_l - E z_u
-  launch_spmv<V, V, V, V>(L.E, z, zl, rhs + (int64_t)nl * nrhs, (V)1, (V)0, 1, nrhs, s);
+  timed(ws, s, PGS_PROF_SPMV_EF, spmv_bytes<V, V, V>(L.E, nrhs, true), [&] {
+    launch_spmv<V, V, V, V>(L.E, z, zl, rhs + (int64_t)nl * nrhs, (V)1, (V)0, 1, nrhs, s);
+  });
   if (l == 0 && op->inner_maxits > 0) {
     // x_l = inner FGMRES on the Schur complement (gemslr.cpp:3556-3563)
     inner_schur<V>(op, ws, P, zl, x + (int64_t)nl * nrhs, nrhs, s);
   } else {
     // z_l += W H W^H z_l
-    launch_lowrank<V>(ws, L, zl, nrhs, s);
+    if (L.rank > 0 && L.nrem > 0)
+      timed(ws, s, PGS_PROF_LOWRANK,
+            (2.0 * L.rank + 3.0 * nrhs) * (double)L.nrem * sizeof(V), [&] { launch_lowrank<V>(ws, L, zl, nrhs, s); });
     // x_l = SolveLevel(l+1, z_l)
     solve_level<V>(op, ws, P, l + 1, zl, x + (int64_t)nl * nrhs, nrhs, s);
   }
   // z_u = rhs_u - F x_l
-  launch_spmv<V, V, V, V>(L.F, x + (int64_t)nl * nrhs, z, rhs, (V)1, (V)0, 1, nrhs, s);
+  timed(ws, s, PGS_PROF_SPMV_EF, spmv_bytes<V, V, V>(L.F, nrhs, true), [&] {
+    launch_spmv<V, V, V, V>(L.F, x + (int64_t)nl * nrhs, z, rhs, (V)1, (V)0, 1, nrhs, s);
+  });
   // x_u = B^{-1} z_u
   launch_trsv<V>(op, ws, l, L.B, z, x, nrhs, s);
 }
@@ -805,12 +883,16 @@ void apply_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const VI* v, VZ* z, int nrhs, hi
   V* x0 = ws->x0.as<V>();
   auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
   const bool wide = nrhs % 4 == 0 && al16(v) && al16(z);  // workspace buffers are allocation-aligned
-  if (wide) k_gather4<VI, V><<<nblocks_for(n * nrhs / 4, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
-  else k_gather<VI, V><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
+  timed(ws, s, PGS_PROF_PERM, (double)n * nrhs * (sizeof(VI) + sizeof(V)) + 4.0 * n, [&] {
+    if (wide) k_gather4<VI, V><<<nblocks_for(n * nrhs / 4, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
+    else k_gather<VI, V><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
+  });
   HIPCHK(hipGetLastError());
   solve_level<V>(op, ws, P, 0, r0, x0, nrhs, s);
-  if (wide) k_scatter4<V, VZ><<<nblocks_for(n * nrhs / 4, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), x0, z);
-  else k_scatter<V, VZ><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), x0, z);
+  timed(ws, s, PGS_PROF_PERM, (double)n * nrhs * (sizeof(V) + sizeof(VZ)) + 4.0 * n, [&] {
+    if (wide) k_scatter4<V, VZ><<<nblocks_for(n * nrhs / 4, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), x0, z);
+    else k_scatter<V, VZ><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), x0, z);
+  });
   HIPCHK(hipGetLastError());
 }
 
@@ -995,6 +1077,7 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
     HIPCHK(hipDeviceGetAttribute(&op->num_cus, hipDeviceAttributeMultiprocessorCount, device));
     set_kernel_attrs(lds);
     if (const char* v = std::getenv("PGS_TRSV_VARIANT")) op->trsv_variant = std::atoi(v);
+    if (const char* v = std::getenv("PGS_LR_MFMA")) op->lr_mfma = std::atoi(v) != 0;
     if (const char* v = std::getenv("PGS_TRSV_NT")) {
       const int nt = std::atoi(v);
       if (nt == 64 || nt == 128 || nt == 256) op->trsv_nt = nt;
@@ -1126,11 +1209,12 @@ int pgs_spmv(pgs_dev_op_t dop, pgs_ws_t w, const double* alpha, const double* x,
   return 0;
 }
 
-int pgs_apply(pgs_dev_op_t dop, pgs_ws_t w, const double* v, double* z, int nrhs, int prec, void* stream) {
-  if (!dop || !w || !v || !z) return PGS_ERROR_INVALID_PARAM;
-  pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
-  pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
-  if (ws->op != op || !pow2(nrhs) || nrhs > ws->max_nrhs) return PGS_ERROR_INVALID_PARAM;
+}  // extern "C"
+
+namespace {
+
+// z = M^{-1} v in the precision `prec` selects, with the apply buffers of ws
+int apply_dispatch(pgs_dev_op_s* op, pgs_ws_s* ws, const double* v, double* z, int nrhs, int prec, void* stream) {
   try {
     HIPCHK(hipSetDevice(op->device));
     hipStream_t s = (hipStream_t)stream;
@@ -1162,6 +1246,18 @@ int pgs_apply(pgs_dev_op_t dop, pgs_ws_t w, const double* v, double* z, int nrhs
     return PGS_ERROR_FUNCTION_CALL;
   }
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgs_apply(pgs_dev_op_t dop, pgs_ws_t w, const double* v, double* z, int nrhs, int prec, void* stream) {
+  if (!dop || !w || !v || !z) return PGS_ERROR_INVALID_PARAM;
+  pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
+  pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
+  if (ws->op != op || !pow2(nrhs) || nrhs > ws->max_nrhs) return PGS_ERROR_INVALID_PARAM;
+  return apply_dispatch(op, ws, v, z, nrhs, prec, stream);
 }
 
 }  // extern "C"
@@ -1313,7 +1409,8 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
   const bool dcgs2 = o.orth == PGS_ORTH_DCGS2 && (scalar_traits<S>::is_complex ? m <= 64 : (m <= 64 || nrhs <= 32));
   // ||b||, v0 = b - A x, ||v0||
   colnorm2<S>(ws, n, nrhs, b, ws->nb2.as<double>(), s);
-  launch_spmv<S, S, S, S>(op->A, x, V, b, S(1.0), S(0.0), 1, nrhs, s);
+  timed(ws, s, PGS_PROF_SPMV_A, spmv_bytes<S, S, S>(op->A, nrhs, true),
+        [&] { launch_spmv<S, S, S, S>(op->A, x, V, b, S(1.0), S(0.0), 1, nrhs, s); });
   colnorm2<S>(ws, n, nrhs, V, ws->nr2.as<double>(), s);
   k_fgmres_init<<<1, 64, 0, s>>>(nrhs, ws->nb2.as<double>(), ws->nr2.as<double>(), o.tol, o.absolute_tol, st,
                                  ws->zflag.as<double>());
@@ -1345,7 +1442,8 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
     } else {
       k_copy_conv<S, VZ><<<nb, kNT, 0, ss>>>(n * nrhs, vprev, zc);
     }
-    launch_spmv<S, VZ, S, S>(op->A, zc, w, (const S*)nullptr, S(1.0), S(0.0), 0, nrhs, ss);
+    timed(ws, ss, PGS_PROF_SPMV_A, spmv_bytes<S, VZ, S>(op->A, nrhs, false),
+          [&] { launch_spmv<S, VZ, S, S>(op->A, zc, w, (const S*)nullptr, S(1.0), S(0.0), 0, nrhs, ss); });
     S* h = ws->hbuf.as<S>();
     S* h2 = ws->h2buf.as<S>();
     double* t2 = ws->t2.as<double>();
@@ -1355,11 +1453,15 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
       const int p = i - 1;
       S* d = ws->dc_d.as<S>();
       S* coef = ws->dc_coef.as<S>();
-      dcgs2_dots<S>(ws, n, nrhs, p, V, ldv, w, d, ss);
-      k_dcgs2_fix<S><<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef);
-      dcgs2_update<S>(ws, n, nrhs, p, V, ldv, vprev, w, coef, t2, ss);
-      k_dcgs2_norm<S><<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st);
-      k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w);
+      const double vb = (double)n * nrhs * sizeof(S);
+      timed(ws, ss, PGS_PROF_ORTH_DOTS, (p + 2.0) * vb, [&] { dcgs2_dots<S>(ws, n, nrhs, p, V, ldv, w, d, ss); });
+      timed(ws, ss, PGS_PROF_STATE, 0.0, [&] {
+        k_dcgs2_fix<S><<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef);
+      });
+      timed(ws, ss, PGS_PROF_ORTH_UPDATE, (p + 4.0) * vb,
+            [&] { dcgs2_update<S>(ws, n, nrhs, p, V, ldv, vprev, w, coef, t2, ss); });
+      timed(ws, ss, PGS_PROF_STATE, 0.0, [&] { k_dcgs2_norm<S><<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st); });
+      timed(ws, ss, PGS_PROF_SCALE, 2.0 * vb, [&] { k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w); });
       HIPCHK(hipGetLastError());
       return;
     }
@@ -1370,30 +1472,43 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
       bool fused = false;
       if constexpr (!scalar_traits<S>::is_complex) {
         if (ws->fused_cgs2 && basis_fits(ws, i, nrhs)) {
-          basis_reg(ws, n, nrhs, i, V, ldv, nullptr, w, h, 0, ss);
-          basis_reg(ws, n, nrhs, i, V, ldv, h, w, h2, 1, ss);
-          basis_reg(ws, n, nrhs, i, V, ldv, h2, w, t2, 2, ss);
+          const double vb = (double)n * nrhs * sizeof(S);
+          timed(ws, ss, PGS_PROF_ORTH_DOTS, (i + 1.0) * vb,
+                [&] { basis_reg(ws, n, nrhs, i, V, ldv, nullptr, w, h, 0, ss); });
+          timed(ws, ss, PGS_PROF_ORTH_UPDATE, (i + 2.0) * vb,
+                [&] { basis_reg(ws, n, nrhs, i, V, ldv, h, w, h2, 1, ss); });
+          timed(ws, ss, PGS_PROF_ORTH_UPDATE, (i + 2.0) * vb,
+                [&] { basis_reg(ws, n, nrhs, i, V, ldv, h2, w, t2, 2, ss); });
           fused = true;
         }
       }
       if (!fused) {
-        multidot<S, S, S>(ws, n, nrhs, i, V, ldv, w, h, 0, ss);
-        multi_axpy<S>(ws, n, nrhs, i, V, ldv, h, w, nullptr, ss);
-        multidot<S, S, S>(ws, n, nrhs, i, V, ldv, w, h2, 0, ss);
-        multi_axpy<S>(ws, n, nrhs, i, V, ldv, h2, w, t2, ss);
+        const double vb = (double)n * nrhs * sizeof(S);
+        timed(ws, ss, PGS_PROF_ORTH_DOTS, (i + 1.0) * vb, [&] { multidot<S, S, S>(ws, n, nrhs, i, V, ldv, w, h, 0, ss); });
+        timed(ws, ss, PGS_PROF_ORTH_UPDATE, (i + 2.0) * vb, [&] { multi_axpy<S>(ws, n, nrhs, i, V, ldv, h, w, nullptr, ss); });
+        timed(ws, ss, PGS_PROF_ORTH_DOTS, (i + 1.0) * vb, [&] { multidot<S, S, S>(ws, n, nrhs, i, V, ldv, w, h2, 0, ss); });
+        timed(ws, ss, PGS_PROF_ORTH_UPDATE, (i + 2.0) * vb, [&] { multi_axpy<S>(ws, n, nrhs, i, V, ldv, h2, w, t2, ss); });
       }
       dot_final<S>(1, i, nrhs, h2, h, 1, ss);
     } else {
       // PargemslrMgs (matrixops.cpp:10728-10807): one (dot, axpy) pair per basis vector
+      const double vb = (double)n * nrhs * sizeof(S);
       for (int j = 0; j < i; ++j) {
-        multidot<S, S, S>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, w, h + (int64_t)j * nrhs, 0, ss);
-        multi_axpy<S>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w,
-                      j == i - 1 ? t2 : nullptr, ss);
+        timed(ws, ss, PGS_PROF_ORTH_DOTS, 2.0 * vb, [&] {
+          multidot<S, S, S>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, w, h + (int64_t)j * nrhs, 0, ss);
+        });
+        timed(ws, ss, PGS_PROF_ORTH_UPDATE, 3.0 * vb, [&] {
+          multi_axpy<S>(ws, n, nrhs, 1, V + (int64_t)j * ldv, ldv, h + (int64_t)j * nrhs, w,
+                        j == i - 1 ? t2 : nullptr, ss);
+        });
       }
     }
-    k_givens<S><<<1, 64, 0, ss>>>(nrhs, m, i, maxits, h, ws->t2.as<double>(), H, ws->cs.as<S>(), ws->sn.as<S>(),
-                                  rs, st);
-    k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w);
+    timed(ws, ss, PGS_PROF_STATE, 0.0, [&] {
+      k_givens<S><<<1, 64, 0, ss>>>(nrhs, m, i, maxits, h, ws->t2.as<double>(), H, ws->cs.as<S>(), ws->sn.as<S>(),
+                                    rs, st);
+    });
+    timed(ws, ss, PGS_PROF_SCALE, 2.0 * (double)n * nrhs * sizeof(S),
+          [&] { k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w); });
     HIPCHK(hipGetLastError());
   };
   // graphs of the inner steps, kept per workspace while the settings that
@@ -1455,20 +1570,27 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
       // close the cycle: finalize the last basis vector and Hessenberg
       // column of the columns still running (one more dot sweep)
       S* d = ws->dc_d.as<S>();
-      dcgs2_dots<S>(ws, n, nrhs, i, V, ldv, nullptr, d, s);
-      k_dcgs2_fix<S><<<1, 64, 0, s>>>(nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st,
-                                      ws->dc_coef.as<S>());
+      timed(ws, s, PGS_PROF_ORTH_DOTS, (i + 1.0) * (double)n * nrhs * sizeof(S),
+            [&] { dcgs2_dots<S>(ws, n, nrhs, i, V, ldv, nullptr, d, s); });
+      timed(ws, s, PGS_PROF_STATE, 0.0, [&] {
+        k_dcgs2_fix<S><<<1, 64, 0, s>>>(nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st,
+                                        ws->dc_coef.as<S>());
+      });
       HIPCHK(hipGetLastError());
     }
     // x += Z y
-    k_backsolve<S><<<1, 64, 0, s>>>(nrhs, m, H, rs, st, ws->coef.as<S>(), ws->upd.as<int>());
-    k_update_x<VZ, S><<<nb, kNT, sizeof(S) * std::max(1, i) * nrhs, s>>>(n, nrhs, i, Z, ldv, ws->coef.as<S>(),
-                                                                        ws->upd.as<int>(), x);
+    timed(ws, s, PGS_PROF_STATE, 0.0,
+          [&] { k_backsolve<S><<<1, 64, 0, s>>>(nrhs, m, H, rs, st, ws->coef.as<S>(), ws->upd.as<int>()); });
+    timed(ws, s, PGS_PROF_UPDATE_X, (double)n * nrhs * ((double)i * sizeof(VZ) + 2.0 * sizeof(S)), [&] {
+      k_update_x<VZ, S><<<nb, kNT, sizeof(S) * std::max(1, i) * nrhs, s>>>(n, nrhs, i, Z, ldv, ws->coef.as<S>(),
+                                                                          ws->upd.as<int>(), x);
+    });
     HIPCHK(hipGetLastError());
     if (iters_host >= maxits) break;
     if (read_flag(true) == 0) break;
     // restart: v0 = b - A x (fgmres.hpp:500-514); normr keeps the Givens estimate
-    launch_spmv<S, S, S, S>(op->A, x, V, b, S(1.0), S(0.0), 1, nrhs, s);
+    timed(ws, s, PGS_PROF_SPMV_A, spmv_bytes<S, S, S>(op->A, nrhs, true),
+          [&] { launch_spmv<S, S, S, S>(op->A, x, V, b, S(1.0), S(0.0), 1, nrhs, s); });
   }
   std::vector<KState> hs(nrhs);
   HIPCHK(hipMemcpyAsync(hs.data(), st, sizeof(KState) * nrhs, hipMemcpyDeviceToHost, s));
@@ -1632,6 +1754,7 @@ extern "C" int pgs_vec_axpy(int device, int64_t n, int is_complex, const double*
   if (!alpha || !vec_args_ok(n, x, y)) return PGS_ERROR_INVALID_PARAM;
   try {
     HIPCHK(hipSetDevice(device));
+    (void)hipGetLastError();  // an earlier call's error is not this launch's
     if (n == 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     if (is_complex) {
@@ -1652,6 +1775,7 @@ extern "C" int pgs_vec_scale(int device, int64_t n, int is_complex, const double
   if (!alpha || !vec_args_ok(n, x, x)) return PGS_ERROR_INVALID_PARAM;
   try {
     HIPCHK(hipSetDevice(device));
+    (void)hipGetLastError();  // an earlier call's error is not this launch's
     if (n == 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     if (is_complex) {
@@ -1671,6 +1795,7 @@ extern "C" int pgs_vec_fill(int device, int64_t n, int is_complex, const double*
   if (!value || !vec_args_ok(n, x, x)) return PGS_ERROR_INVALID_PARAM;
   try {
     HIPCHK(hipSetDevice(device));
+    (void)hipGetLastError();  // an earlier call's error is not this launch's
     if (n == 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     if (is_complex) {
@@ -1695,6 +1820,7 @@ extern "C" int pgs_vec_dot(int device, int64_t n, int is_complex, const double* 
   hipStream_t s = (hipStream_t)stream;
   try {
     HIPCHK(hipSetDevice(device));
+    (void)hipGetLastError();  // an earlier call's error is not this launch's
     if (n == 0) return 0;
     const unsigned g = vec_grid(n);
     // partials (2 per workgroup) and the two sums, stream-ordered so that
@@ -1779,6 +1905,7 @@ extern "C" int pgs_dev_csr_spmv(pgs_dev_csr_t m, const double* alpha, const doub
   if (!m || !alpha || !beta || (m->A.nrows > 0 && (!x || !y))) return PGS_ERROR_INVALID_PARAM;
   try {
     HIPCHK(hipSetDevice(m->device));
+    (void)hipGetLastError();  // an earlier call's error is not this launch's
     if (m->is_complex) {
       using Z = cplx<double>;
       launch_spmv<Z, Z, Z, Z>(m->A, reinterpret_cast<const Z*>(x), reinterpret_cast<Z*>(y), (const Z*)nullptr,
@@ -1803,9 +1930,11 @@ extern "C" int pgs_ws_profile(pgs_ws_t w, int enable) {
     return set_err(e);
   }
   ws->prof = enable != 0;
-  ws->prof_ms = 0.0;
-  ws->prof_bytes = 0.0;
-  ws->prof_launches = 0;
+  for (int f = 0; f < PGS_PROFILE_FAMILIES; ++f) {
+    ws->fam_ms[f] = 0.0;
+    ws->fam_bytes[f] = 0.0;
+    ws->fam_launches[f] = 0;
+  }
   return 0;
 }
 
@@ -1818,9 +1947,26 @@ extern "C" int pgs_ws_profile_read(pgs_ws_t w, pgs_kernel_profile_t* out) {
   } catch (const HipError& e) {
     return set_err(e);
   }
-  out->launches = ws->prof_launches;
-  out->total_ms = ws->prof_ms;
-  out->total_bytes = ws->prof_bytes;
+  out->launches = ws->fam_launches[PGS_PROF_TRSV0];
+  out->total_ms = ws->fam_ms[PGS_PROF_TRSV0];
+  out->total_bytes = ws->fam_bytes[PGS_PROF_TRSV0];
+  return 0;
+}
+
+extern "C" int pgs_ws_profile_families(pgs_ws_t w, pgs_kernel_profile_t* out) {
+  if (!w || !out) return PGS_ERROR_INVALID_PARAM;
+  pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
+  try {
+    HIPCHK(hipSetDevice(ws->op->device));
+    prof_flush(ws);
+  } catch (const HipError& e) {
+    return set_err(e);
+  }
+  for (int f = 0; f < PGS_PROFILE_FAMILIES; ++f) {
+    out[f].launches = ws->fam_launches[f];
+    out[f].total_ms = ws->fam_ms[f];
+    out[f].total_bytes = ws->fam_bytes[f];
+  }
   return 0;
 }
 

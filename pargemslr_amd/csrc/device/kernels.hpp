@@ -322,6 +322,121 @@ __global__ void k_lr_update(int m, int k, int nrhs, const V* __restrict__ WH, co
 }
 
 // ---------------------------------------------------------------------------
+// The same low-rank term on the matrix cores for a block of FP32 columns
+// (nrhs = 16 * CT with CT = 1, 2, 4; k <= 16 * KT with KT = 1, 2):
+// v_mfma_f32_16x16x4_f32 tiles.  Lane layout of one 16x16x4 product
+// D = A B + C: A[i][q] comes from lane i + 16 q, B[q][j] from lane j + 16 q,
+// and lane j + 16 r holds D[4 r + e][j] in element e.
+// ---------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Stage 1: each wave multiplies W^T (k x its rows) by z (its rows x nrhs)
+// four rows per product, with the rank index as the product's rows and the
+// columns as its columns; the four waves' tiles are summed in LDS and written
+// as the workgroup's partial part[blk * k * nrhs + j * nrhs + c] (summed over
+// the workgroups by k_lr_final).  rows_per_wave is a multiple of 4.
+template <int KT, int CT>
+__global__ __launch_bounds__(256) void k_lr_dot_mfma(int m, int k, int nrhs, const float* __restrict__ W,
+                                                     const float* __restrict__ z, float* __restrict__ part,
+                                                     int rows_per_wave) {
+  __shared__ float red[4][KT * 16][CT * 16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int jj = lane & 15, q = lane >> 4;
+  const int64_t gw = (int64_t)blockIdx.x * 4 + wv;
+  const int64_t i_begin = gw * rows_per_wave;
+  const int64_t i_end = min((int64_t)m, i_begin + rows_per_wave);
+  f32x4 acc[KT][CT];
+#pragma unroll
+  for (int h = 0; h < KT; ++h)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) acc[h][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nit = i_end > i_begin ? (int)((i_end - i_begin + 3) / 4) : 0;
+  for (int it = 0; it < nit; ++it) {
+    const int64_t i = i_begin + 4 * it + q;
+    const bool ok = i < i_end;
+    float a[KT], b[CT];
+#pragma unroll
+    for (int h = 0; h < KT; ++h) {
+      const int j = 16 * h + jj;
+      a[h] = (ok && j < k) ? W[i * k + j] : 0.f;
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) b[ct] = ok ? z[i * nrhs + 16 * ct + jj] : 0.f;
+#pragma unroll
+    for (int h = 0; h < KT; ++h)
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) acc[h][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[h], b[ct], acc[h][ct], 0, 0, 0);
+  }
+#pragma unroll
+  for (int h = 0; h < KT; ++h)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) red[wv][16 * h + 4 * q + e][16 * ct + jj] = acc[h][ct][e];
+  __syncthreads();
+  for (int t = threadIdx.x; t < k * nrhs; t += 256) {
+    const int j = t / nrhs, c = t % nrhs;
+    part[(int64_t)blockIdx.x * k * nrhs + t] = (red[0][j][c] + red[1][j][c]) + (red[2][j][c] + red[3][j][c]);
+  }
+}
+
+// Stage 3: z[i, c] += sum_j WH[i, j] t[j, c], one 16-row tile per wave.  The
+// rank index of product q and operand row p is j = 16 h + 4 p + q, so a lane
+// loads WH[i, 16 h + 4 p .. + 3] as one 16-byte vector (k % 4 == 0) and
+// feeds its four values to four products; t's operands use the same j.
+template <int KT, int CT>
+__global__ __launch_bounds__(256) void k_lr_update_mfma(int m, int k, int nrhs, const float* __restrict__ WH,
+                                                        const float* __restrict__ t, float* __restrict__ z) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int jj = lane & 15, p = lane >> 4;
+  const int64_t i0 = ((int64_t)blockIdx.x * 4 + wv) * 16;
+  if (i0 >= m) return;  // whole wave
+  float bt[KT][4][CT];
+#pragma unroll
+  for (int h = 0; h < KT; ++h)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = 16 * h + 4 * p + q;
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) bt[h][q][ct] = j < k ? t[j * nrhs + 16 * ct + jj] : 0.f;
+    }
+  const int64_t i = i0 + jj;
+  float a[KT][4];
+#pragma unroll
+  for (int h = 0; h < KT; ++h) {
+    const int j0 = 16 * h + 4 * p;
+    if ((k & 3) == 0 && i < m && j0 + 3 < k) {
+      const float4 v = *reinterpret_cast<const float4*>(WH + i * k + j0);
+      a[h][0] = v.x;
+      a[h][1] = v.y;
+      a[h][2] = v.z;
+      a[h][3] = v.w;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[h][q] = (i < m && j0 + q < k) ? WH[i * k + j0 + q] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct) {
+    f32x4 c;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t r = i0 + 4 * p + e;
+      c[e] = r < m ? z[r * nrhs + 16 * ct + jj] : 0.f;
+    }
+#pragma unroll
+    for (int h = 0; h < KT; ++h)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[h][q], bt[h][q][ct], c, 0, 0, 0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t r = i0 + 4 * p + e;
+      if (r < m) z[r * nrhs + 16 * ct + jj] = c[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Krylov kernels (FP64).  A block of p vectors V_j (each n x nrhs) at
 // V + j*ldv.  State per column in KState.
 // ---------------------------------------------------------------------------

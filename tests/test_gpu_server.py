@@ -86,3 +86,42 @@ def test_server_rejects_bad_requests(operators, gpu):
     X, its, info = S.wait(r)
     assert info["status"] == 0 and not X.any() and its[0] == 0
     S.close()
+
+
+def test_request_joins_a_running_batch_at_a_restart(gpu):
+    """One worker, FGMRES(4) on a 2D Laplacian 256^2 with a one-level ILUT
+    (tens of restart cycles): a request submitted while another batch of the
+    same operator runs joins that batch at a restart (joined_cycle > 0)
+    instead of waiting for it to finish, and both requests' columns equal the
+    same columns solved alone (iterations within two: the server restarts
+    from the true residual's norm, one uninterrupted solve from the Givens
+    estimate)."""
+    A = pg.laplacian(256, 256, 1)
+    G = P.setup(A, 1, 1, 1e-2, 5, 0)
+    D = pg.DevOp(G, prec_mask=pg.PGS_PREC_FP32)
+    kd = 4
+    S = pg.Server(max_batch=8, slots=1, mem_reserve=1 << 30, kdim=kd, maxits=4000, tol=TOL)
+    i = S.add_operator(D)
+    B1 = np.stack([pg.rng_normal(7000 + c, A.n) for c in range(2)], axis=1)
+    B2 = np.stack([pg.rng_normal(7100 + c, A.n) for c in range(3)], axis=1)
+    r1 = S.submit(i, B1)
+    time.sleep(0.01)  # the worker has taken the first request and runs its first cycles
+    r2 = S.submit(i, B2)
+    X2, its2, info2 = S.wait(r2)
+    X1, its1, info1 = S.wait(r1)
+    S.close()
+    print(f"[server] first {info1}, iters {list(its1)}; joiner {info2}, iters {list(its2)}", flush=True)
+    assert info1["status"] == 0 and info2["status"] == 0
+    assert info1["joined_cycle"] == 0
+    assert info2["joined_cycle"] > 0, "the second request did not join the running batch"
+    W = pg.Workspace(D, max_nrhs=1, kdim=kd)
+    for B, X, its in ((B1, X1, its1), (B2, X2, its2)):
+        for c in range(B.shape[1]):
+            b = B[:, c]
+            assert np.linalg.norm(b - A.matvec(X[:, c])) <= TOL * np.linalg.norm(b)
+            xd = pg.DeviceArray(A.n)
+            (st, it, _), = W.fgmres(pg.DeviceArray.from_numpy(b), xd, kdim=kd, maxits=4000, tol=TOL,
+                                    prec=pg.PGS_PREC_FP32)
+            pg.synchronize()
+            assert st == 0 and abs(int(its[c]) - it) <= 2, (its[c], it)
+            assert np.linalg.norm(X[:, c] - xd.numpy()) <= 1e-6 * np.linalg.norm(xd.numpy())

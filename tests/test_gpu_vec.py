@@ -32,7 +32,8 @@ def test_vector_ops(gpu, cplx, n):
     x, y = _vec(rng, n, cplx), _vec(rng, n, cplx)
     xd, yd = pg.DeviceArray.from_numpy(x), pg.DeviceArray.from_numpy(y)
     alpha = (0.7 - 0.2j) if cplx else -1.25
-    assert L.pgs_vec_axpy(0, n, int(cplx), _pair(alpha), xd.ptr, yd.ptr, None) == 0
+    rc = L.pgs_vec_axpy(0, n, int(cplx), _pair(alpha), xd.ptr, yd.ptr, None)
+    assert rc == 0, (rc, L.pgs_last_hip_error())
     pg.synchronize()
     assert np.array_equal(yd.numpy(), y + alpha * x) or np.allclose(yd.numpy(), y + alpha * x, rtol=0, atol=1e-15)
     y = y + alpha * x

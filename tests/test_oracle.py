@@ -338,3 +338,19 @@ def test_setup_deterministic():
     assert np.array_equal(G1.perm(), G2.perm())
     b = np.ones(A.n)
     assert np.array_equal(G1.host_apply(b), G2.host_apply(b))
+
+
+def test_threaded_block_solves_give_the_same_bits():
+    """The CPU baseline's OpenMP-over-blocks variant (set_threads > 1) solves
+    each diagonal block as the serial oracle does: the same bits."""
+    from tests import problems as P
+    A, G = P.build(P.C2_SMALL)
+    O = orc.GemslrOracle(A.n, G.perm(), G.levels())
+    v = pg.rng_normal(1, A.n)
+    z1 = O.apply(v)
+    try:
+        orc.set_threads(4)
+        z4 = O.apply(v)
+    finally:
+        orc.set_threads(1)
+    assert np.array_equal(z1, z4)
