@@ -40,6 +40,7 @@ extern "C" {
 #define PGS_ERR_NOT_CONVERGED 204
 #define PGS_ERR_UNSUPPORTED 205
 #define PGS_ERR_NONFINITE 206 /* a column met NaN or Inf and was retired; the other columns are unaffected */
+#define PGS_ERR_SCRATCH_LIMIT 207 /* the call needs more scratch memory than pgs_apply_noalloc holds */
 
 /* parameter array layout: the reference's PARGEMSLR_IO_* indices
  * (ParGeMSLR/SRC/utils/utils.hpp:77-297); only the entries listed here are read */
@@ -253,6 +254,21 @@ int pgs_spmv(pgs_dev_op_t dop, pgs_ws_t ws, const double* alpha, const double* x
  * v and z are in the operator's precision (double / complex128); `prec`
  * selects the arithmetic of the apply (PGS_PREC_FP64 or PGS_PREC_FP32). */
 int pgs_apply(pgs_dev_op_t dop, pgs_ws_t ws, const double* v, double* z, int nrhs, int prec, void* stream);
+
+/* z = M^{-1} v as pgs_apply, without allocating device memory and without a
+ * workspace.  No counterpart in the reference: for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  The apply's buffers (permuted copies, level
+ * temporaries, low-rank partial sums) live in a file-scope device array of
+ * PGS_APPLY_NOALLOC_SCRATCH bytes that each device the library runs on holds
+ * until the process ends; a call whose buffers do not fit returns
+ * PGS_ERR_SCRATCH_LIMIT (about n * nrhs * 3 values of the operator's type: at
+ * most 2^21 unknowns at one double column).  Operators with the inner Schur
+ * iteration return PGS_ERR_UNSUPPORTED.  Every call on a device shares its
+ * array: calls that overlap on one device (other streams or threads) race;
+ * serialize them.  Results equal pgs_apply's bit for bit. */
+#define PGS_APPLY_NOALLOC_SCRATCH (64 * 1024 * 1024)
+int pgs_apply_noalloc(pgs_dev_op_t dop, const double* v, double* z, int nrhs, int prec, void* stream);
 
 typedef struct {
   int kdim;         /* restart length m */

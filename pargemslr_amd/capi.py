@@ -22,6 +22,8 @@ PGS_ERROR_INVALID_PARAM = 101
 PGS_ERR_BREAKDOWN = 203
 PGS_ERR_NOT_CONVERGED = 204
 PGS_ERR_NONFINITE = 206
+PGS_ERR_SCRATCH_LIMIT = 207
+PGS_APPLY_NOALLOC_SCRATCH = 64 * 1024 * 1024
 
 # parameter indices (reference PARGEMSLR_IO_* numbering)
 IO = dict(
@@ -98,7 +100,7 @@ EXPORTED = [
     "pgs_read_mm_csr", "pgs_read_mm_vector", "pgs_params_read_inputs", "pgs_hypre_gemslr_create",
     "pgs_hypre_gemslr_params", "pgs_hypre_gemslr_set_precision", "pgs_hypre_gemslr_setup", "pgs_hypre_gemslr_solve",
     "pgs_hypre_gemslr_destroy", "pgs_vec_axpy", "pgs_vec_scale", "pgs_vec_fill", "pgs_vec_dot",
-    "pgs_dev_csr_create", "pgs_dev_csr_destroy", "pgs_dev_csr_spmv", "pgs_ws_profile_families",
+    "pgs_dev_csr_create", "pgs_dev_csr_destroy", "pgs_dev_csr_spmv", "pgs_apply_noalloc", "pgs_ws_profile_families",
 ]
 
 # kernel families of pgs_ws_profile_families, by index
@@ -186,6 +188,7 @@ def lib():
     L.pgs_dev_csr_create.argtypes = [C.c_int, C.c_int, ip, ip, dp, C.c_int, C.POINTER(vp)]
     L.pgs_dev_csr_destroy.argtypes = [vp]
     L.pgs_dev_csr_spmv.argtypes = [vp, dp, vp, dp, vp, vp]
+    L.pgs_apply_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
     L.pgs_ws_profile_families.argtypes = [vp, C.POINTER(KernelProfile)]
     _lib = L
     return L
@@ -623,6 +626,11 @@ class DevOp:
         b = C.c_int64()
         _chk(lib().pgs_dev_op_bytes(self._h, C.byref(b)), "pgs_dev_op_bytes")
         return b.value
+
+    def apply_noalloc(self, v, z, nrhs=1, prec=PGS_PREC_FP64, stream=None):
+        """z = M^{-1} v with no device allocation and no workspace
+        (pgs_apply_noalloc; calls on one device must not overlap)."""
+        _chk(lib().pgs_apply_noalloc(self._h, _tptr(v), _tptr(z), nrhs, prec, _sh(stream)), "pgs_apply_noalloc")
 
     def close(self):
         if self._h:

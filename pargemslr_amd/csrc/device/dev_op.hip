@@ -40,14 +40,23 @@ inline unsigned nblocks_for(int64_t work, int nt) { return (unsigned)((work + nt
 struct DevMem {
   void* p = nullptr;
   size_t bytes = 0;
+  bool own = true;  // false: a view of memory that this object does not free
   DevMem() = default;
   DevMem(const DevMem&) = delete;
   DevMem& operator=(const DevMem&) = delete;
   ~DevMem() {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
+  }
+  void view(void* q, size_t b) {
+    if (p && own) (void)hipFree(p);
+    p = q;
+    bytes = b;
+    own = false;
   }
   void alloc(size_t b) {
-    if (p) { (void)hipFree(p); p = nullptr; }
+    if (p && own) (void)hipFree(p);
+    p = nullptr;
+    own = true;
     bytes = b;
     if (b == 0) return;
     hipError_t e = hipMalloc(&p, b);
@@ -1918,6 +1927,87 @@ extern "C" int pgs_dev_csr_spmv(pgs_dev_csr_t m, const double* alpha, const doub
     return set_err(e);
   }
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// z = M^{-1} v without allocating: the apply buffers live in `scratch`, a
+// file-scope device array of PGS_APPLY_NOALLOC_SCRATCH bytes that every device
+// the library runs on holds until the process ends
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[PGS_APPLY_NOALLOC_SCRATCH];
+
+namespace {
+
+std::mutex g_scratch_m;
+std::vector<void*> g_scratch_ptr;  // device address of `scratch`, per device
+
+// The device address of `scratch` on `device` (current), looked up once per
+// device.  In host code the array's name is not a device address; only this
+// pointer is passed to kernels.
+int scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> g(g_scratch_m);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return set_err(HipError{e != hipSuccess ? e : hipErrorInvalidSymbol});
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
+// The apply's buffers (as pgs_ws_create sizes them for one workspace of
+// nr columns), placed one after another at 256-byte boundaries; returns the
+// bytes they take.
+size_t noalloc_layout(const pgs_dev_op_s* op, int nr, pgs_ws_s* ws, unsigned char* base) {
+  const WsSizes z = ws_sizes(op, nr, 1);
+  size_t off = 0;
+  auto place = [&](DevMem& m, size_t b) {
+    if (ws) m.view(base + off, b);
+    off += (b + 255) / 256 * 256;
+  };
+  for (size_t b : z.zlev) {
+    std::unique_ptr<DevMem> m(new DevMem());
+    place(*m, b);
+    if (ws) ws->zlev.push_back(std::move(m));
+  }
+  pgs_ws_s dummy;
+  pgs_ws_s* w = ws ? ws : &dummy;
+  for (const auto& e : z.bufs) {
+    DevMem pgs_ws_s::*f = e.first;
+    if (f == &pgs_ws_s::r0 || f == &pgs_ws_s::x0 || f == &pgs_ws_s::lrpart || f == &pgs_ws_s::lrt)
+      place(w->*f, e.second);
+  }
+  return off;
+}
+
+}  // namespace
+
+extern "C" int pgs_apply_noalloc(pgs_dev_op_t dop, const double* v, double* z, int nrhs, int prec, void* stream) {
+  if (!dop || !v || !z || !pow2(nrhs) || nrhs > 64) return PGS_ERROR_INVALID_PARAM;
+  pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
+  // the inner Schur iteration needs a Krylov workspace: use pgs_apply
+  if (op->inner_maxits > 0) return PGS_ERR_UNSUPPORTED;
+  const size_t need = noalloc_layout(op, nrhs, nullptr, nullptr);
+  if (need > (size_t)PGS_APPLY_NOALLOC_SCRATCH) {
+    g_last_hip_error = "pgs_apply_noalloc: the apply needs " + std::to_string(need) + " bytes of scratch, more than " +
+                       std::to_string((size_t)PGS_APPLY_NOALLOC_SCRATCH);
+    return PGS_ERR_SCRATCH_LIMIT;
+  }
+  int rc = hip_rc(hipSetDevice(op->device));
+  if (rc) return rc;
+  void* base = nullptr;
+  rc = scratch_address(op->device, &base);
+  if (rc) return rc;
+  // a workspace whose apply buffers are views of the scratch array (no
+  // allocation: its destructor frees nothing it does not own)
+  pgs_ws_s ws;
+  ws.op = op;
+  ws.max_nrhs = nrhs;
+  ws.kdim = 1;
+  noalloc_layout(op, nrhs, &ws, static_cast<unsigned char*>(base));
+  return apply_dispatch(op, &ws, v, z, nrhs, prec, stream);
 }
 
 extern "C" int pgs_ws_profile(pgs_ws_t w, int enable) {

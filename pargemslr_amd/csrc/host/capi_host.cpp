@@ -267,6 +267,7 @@ const char* pgs_strerror(int code) {
     case PGS_ERR_NOT_CONVERGED: return "FGMRES did not converge";
     case PGS_ERR_UNSUPPORTED: return "unsupported operation";
     case PGS_ERR_NONFINITE: return "NaN or Inf in the right-hand side, the initial guess or the iteration";
+    case PGS_ERR_SCRATCH_LIMIT: return "the apply needs more scratch memory than pgs_apply_noalloc holds";
     default: return "unknown error";
   }
 }
