@@ -81,8 +81,9 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="counter probe: --nrhs FP32 applies of the 128^3 operator and a 1 GiB copy")
     ap.add_argument("--spmv-probe", action="store_true",
-                    help="one-column SpMV of the 128^3 Laplacian with 8, 4, 2 lanes per row and one thread per row"
-                         " (for a kernel trace; each variant is its own kernel instantiation)")
+                    help="one-column SpMV of the 128^3 Laplacian with 8, 4, 2 lanes per row and one thread per row,"
+                         " and 32-column products with 4 and 8 columns per thread (for a kernel trace; each variant"
+                         " is its own kernel instantiation)")
     ap.add_argument("--restart", default="",
                     help="restart path: save the --config operator to this file, load it, upload it and compare"
                          " the time with the setup")
@@ -470,6 +471,25 @@ def spmv_probe(args):
         log(f"[spmv-probe] lanes per row {lpr}: max |y - y(8 lanes)| {np.abs(out - ref).max():.2e}")
         L.pgs_dev_csr_destroy(h)
     del os.environ["PGS_SPMV_LPR"]
+    # 32-column products with 4 and 8 columns per thread (FP64 x and y)
+    G = pg.Gemslr(A, pg.config_params(1, 1, 1e-1, 2, 0), seed=0)
+    nr = 32
+    X = pg.DeviceArray.from_numpy(np.stack([pg.rng_normal(c, A.n) for c in range(nr)], axis=1))
+    Y = pg.DeviceArray((A.n, nr))
+    ref = None
+    for cols in ("4", "8"):
+        os.environ["PGS_SPMM_COLS"] = cols
+        D = pg.DevOp(G, prec_mask=pg.PGS_PREC_FP64)
+        W = pg.Workspace(D, max_nrhs=nr, kdim=1)
+        for _ in range(20):
+            W.spmv(X, Y, nrhs=nr)
+        pg.synchronize()
+        out = Y.numpy()
+        ref = out if ref is None else ref
+        log(f"[spmv-probe] {nr} columns, {cols} per thread: max |Y - Y(4)| {np.abs(out - ref).max():.2e}")
+        W.close()
+        D.close()
+    del os.environ["PGS_SPMM_COLS"]
 
 
 def probe(args):

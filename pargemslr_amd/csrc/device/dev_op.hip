@@ -105,6 +105,7 @@ struct DevCsr {
   int nrows = 0, ncols = 0;
   int64_t nnz = 0;
   int lpr = 1;
+  int spmm_cols = 4;  // columns per thread of products with 8 or more columns (4 or 8)
   DevMem ia, ja, a;
 };
 
@@ -124,6 +125,8 @@ void upload_csr(DevCsr& d, const Csr<T>& A, int64_t* counter) {
     const int v = std::atoi(e);
     if (v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32 || v == 64) d.lpr = v;
   }
+  // PGS_SPMM_COLS=8: eight columns per thread in products of 8 or more columns
+  if (const char* e = std::getenv("PGS_SPMM_COLS")) d.spmm_cols = std::atoi(e) == 8 ? 8 : 4;
 }
 
 // Pipelined triangular-solve geometry: 512 threads, 4 lanes per row, 5 slots
@@ -344,11 +347,15 @@ void launch_spmv(const DevCsr& M, const VX* x, VY* y, const VY* r, VACC alpha, V
   } else {
     auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
     const bool wide = nrhs % 4 == 0 && al16(x) && al16(y) && (r == nullptr || al16(r));
-    const int64_t threads = (int64_t)M.nrows * nrhs / (wide ? 4 : 1);
+    const int cw = wide ? (M.spmm_cols == 8 && nrhs % 8 == 0 ? 8 : 4) : 1;
+    const int64_t threads = (int64_t)M.nrows * nrhs / cw;
     // a multiple of the XCD count (the SpMV kernels renumber workgroups per
     // XCD; the padding workgroups find no rows)
     const unsigned g = (nblocks_for(threads, kNT) + kXcds - 1) / kXcds * kXcds;
-    if (wide)
+    if (cw == 8)
+      k_spmvN4<VA, VX, VY, VACC, 8><<<g, kNT, 0, s>>>(M.nrows, nrhs, M.ia.as<int>(), M.ja.as<int>(), M.a.as<VA>(), x,
+                                                      y, r, alpha, beta, mode);
+    else if (wide)
       k_spmvN4<VA, VX, VY, VACC><<<g, kNT, 0, s>>>(M.nrows, nrhs, M.ia.as<int>(), M.ja.as<int>(), M.a.as<VA>(), x, y,
                                                    r, alpha, beta, mode);
     else

@@ -174,24 +174,26 @@ __global__ void k_spmvN(int nrows, int nrhs, const int* __restrict__ ia, const i
   else y[t] = cvt<VY>(alpha * acc + beta * y0);
 }
 
-// Four consecutive columns per thread (nrhs a multiple of 4, x, y and r
-// 16-byte aligned): the row's indices and values are loaded once for four
-// columns and every x gather and y access is 16 bytes or wider; each
+// CW (4 or 8) consecutive columns per thread (nrhs a multiple of CW, x, y
+// and r 16-byte aligned): the row's indices and values are loaded once for
+// CW columns and every x gather and y access is 16 bytes or wider; each
 // column's sum runs over the entries in the same order as k_spmvN.
-template <typename VA, typename VX, typename VY, typename VACC>
+template <typename VA, typename VX, typename VY, typename VACC, int CW = 4>
 __global__ void k_spmvN4(int nrows, int nrhs, const int* __restrict__ ia, const int* __restrict__ ja,
                          const VA* __restrict__ a, const VX* __restrict__ x, VY* __restrict__ y,
                          const VY* __restrict__ r, VACC alpha, VACC beta, int mode) {
-  const int64_t e = 4 * (xcd_block() * blockDim.x + threadIdx.x);
+  const int64_t e = CW * (xcd_block() * blockDim.x + threadIdx.x);
   if (e >= (int64_t)nrows * nrhs) return;
   const int row = (int)row_of(e, nrhs);
   const int c = col_of(e, nrhs);
-  using PY = ColPack<VY, 4>;
-  using PX = ColPack<VX, 4>;
+  using PY = ColPack<VY, CW>;
+  using PX = ColPack<VX, CW>;
   PY y0;
   if (mode == 1) y0 = *reinterpret_cast<const PY*>(r + e);
   else if (!(beta == (VACC)0)) y0 = *reinterpret_cast<const PY*>(y + e);
-  VACC acc[4] = {VACC(0), VACC(0), VACC(0), VACC(0)};
+  VACC acc[CW];
+#pragma unroll
+  for (int q = 0; q < CW; ++q) acc[q] = VACC(0);
   const int k1 = ia[row + 1];
   for (int k = ia[row]; k < k1; k += 8) {
     int jj[8];
@@ -210,12 +212,12 @@ __global__ void k_spmvN4(int nrows, int nrhs, const int* __restrict__ ia, const 
     for (int u = 0; u < 8; ++u)
       if (k + u < k1) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] += av[u] * (VACC)xv[u].v[q];
+        for (int q = 0; q < CW; ++q) acc[q] += av[u] * (VACC)xv[u].v[q];
       }
   }
   PY out;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < CW; ++q) {
     if (mode == 1) out.v[q] = cvt<VY>((VACC)y0.v[q] - acc[q]);
     else if (beta == (VACC)0) out.v[q] = cvt<VY>(alpha * acc[q]);
     else out.v[q] = cvt<VY>(alpha * acc[q] + beta * (VACC)y0.v[q]);

@@ -65,7 +65,10 @@ def test_apply_fp32_close(small, gpu):
     zd = _dev(np.zeros(A.n), gpu)
     W.apply(_dev(v, gpu), zd, prec=pg.PGS_PREC_FP32)
     z = _host(zd)
-    assert np.linalg.norm(z - ref) <= 1e-4 * np.linalg.norm(ref)
+    err = np.linalg.norm(z - ref) / np.linalg.norm(ref)
+    print(f"[parity] FP32 apply vs the oracle's FP64 apply: {err:.2e}", flush=True)
+    # the mixed form's bar for the apply: 1e-5 relative to the FP64 apply
+    assert err <= 1e-5
 
 
 def test_apply_multi_rhs_matches_columns(small, gpu):
@@ -173,7 +176,7 @@ def test_apply_trsv_kernels(env, small, gpu, monkeypatch):
     nr = 8
     V = np.stack([pg.rng_normal(60 + c, A.n) for c in range(nr)], axis=1)
     refs = [O.apply(V[:, c]) for c in range(nr)]
-    for prec, tol in ((pg.PGS_PREC_FP64, 1e-12), (pg.PGS_PREC_FP32, 1e-4)):
+    for prec, tol in ((pg.PGS_PREC_FP64, 1e-12), (pg.PGS_PREC_FP32, 1e-5)):
         Zd = _dev(np.zeros((A.n, nr)), gpu)
         W.apply(_dev(V, gpu), Zd, nrhs=nr, prec=prec)
         Z = _host(Zd)
