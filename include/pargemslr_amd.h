@@ -79,6 +79,7 @@ extern "C" {
 #define PGS_IO_SCHUR_ENABLE 130
 #define PGS_IO_SCHUR_ITER_TOL 131
 #define PGS_IO_SCHUR_MAXITS 132
+#define PGS_IO_ADVANCED_GLOBAL_SOLVE 141 /* SolveLevel variant: 0 = LU (kGemslrLUSolve), 1 = U (kGemslrUSolve) */
 /* most inner Schur iterations per apply (PGS_IO_SCHUR_MAXITS) this library takes */
 #define PGS_SCHUR_MAXITS_LIMIT 32
 
@@ -135,6 +136,7 @@ typedef struct {
   double setup_seconds[4]; /* partition+ordering, ILUT, low-rank, total */
   int schur_maxits;        /* inner Schur iteration steps in every apply; 0 when off */
   double schur_tol;        /* its relative tolerance */
+  int global_solve;        /* SolveLevel variant of the apply: 0 = LU, 1 = U (PGS_IO_ADVANCED_GLOBAL_SOLVE) */
 } pgs_gemslr_info_t;
 int pgs_gemslr_info(pgs_gemslr_t op, pgs_gemslr_info_t* info);
 
@@ -417,8 +419,8 @@ void pgs_free(void* p);
  * keywords, line order and GLOBAL-to-LOCAL copies.  Returns PGS_ERROR_IO for a
  * missing file or an unknown keyword (the lines before it stay applied).
  * Options the setup does not implement (B solves other than ILUT, ND
- * partitions, AMD/ND local orderings, the U-only level solve) are read
- * faithfully; pgs_gemslr_setup then refuses them with
+ * partitions, AMD/ND local orderings, global solve options other than the
+ * LU and U variants of the level solve) are read faithfully; pgs_gemslr_setup then refuses them with
  * PGS_ERROR_INVALID_OPTION. */
 int pgs_params_read_inputs(const char* path, int format, double* params);
 

@@ -60,6 +60,7 @@ typedef struct {
   const orc_level* levels;
   int inner_maxits;  /* > 0: inner FGMRES on the level-0 Schur complement */
   double inner_tol;
+  int usolve;        /* 1: the U variant of SolveLevel (no block L solve on the way down) */
 } orc_gemslr;
 
 /* threads over which a level's diagonal blocks are solved (the CPU

@@ -28,7 +28,7 @@ class OrcLevel(C.Structure):
 
 class OrcGemslr(C.Structure):
     _fields_ = [("n", C.c_int), ("nlev", C.c_int), ("perm", C.c_void_p), ("levels", C.POINTER(OrcLevel)),
-                ("inner_maxits", C.c_int), ("inner_tol", C.c_double)]
+                ("inner_maxits", C.c_int), ("inner_tol", C.c_double), ("usolve", C.c_int)]
 
 
 _lib = None
@@ -122,9 +122,9 @@ class GemslrOracle:
     """Oracle view of a factored operator given as the level dicts of
     pargemslr_amd.Gemslr.levels() plus the permutation.  schur = (maxits,
     tol) turns on the inner FGMRES on the level-0 Schur complement (level 0
-    must carry its block "C")."""
+    must carry its block "C"); usolve selects the U variant of SolveLevel."""
 
-    def __init__(self, n, perm, levels, schur=None):
+    def __init__(self, n, perm, levels, schur=None, usolve=False):
         self.n = n
         self._keep = []
         self.is_complex = cz = _cplx(*[d["Dinv"] for d in levels])
@@ -154,7 +154,7 @@ class GemslrOracle:
             if levels[0].get("C") is None:
                 raise ValueError("the inner iteration needs level 0's block C")
             inner = (int(schur[0]), float(schur[1]))
-        self.s = OrcGemslr(n, len(levels), _p(self.perm), lv, inner[0], inner[1])
+        self.s = OrcGemslr(n, len(levels), _p(self.perm), lv, inner[0], inner[1], 1 if usolve else 0)
         self.work = np.zeros(lib().orc_gemslr_work_size(C.byref(self.s)), dtype=self.dtype)
 
     def _k(self, a):

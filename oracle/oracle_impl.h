@@ -233,11 +233,17 @@ static void ORC_FN(solve_level)(const orc_gemslr* op, int l, const ORC_T* rhs, O
   zl = z + nl;
   xlr = z + nl + m;
   t = xlr + m;
-  /* step 1: z_u = B^{-1} rhs_u (block-diagonal ILU, blocks are independent) */
-  ORC_FN(level_ilu_solve)(L, rhs, z);
-  /* steps 2-3: z_l = -E z_u; z_l += rhs_l */
-  ORC_FN(orc_csr_matvec)(m, L->E_ia, L->E_ja, (const ORC_T*)L->E_a, -1.0, z, 0.0, zl);
-  for (i = 0; i < m; ++i) zl[i] += rhs[nl + i];
+  if (op->usolve) {
+    /* the U variant (kGemslrUSolve, gemslr.cpp:3527-3532): step 1 is z = rhs,
+     * steps 2-3 are skipped */
+    for (i = 0; i < m; ++i) zl[i] = rhs[nl + i];
+  } else {
+    /* step 1: z_u = B^{-1} rhs_u (block-diagonal ILU, blocks are independent) */
+    ORC_FN(level_ilu_solve)(L, rhs, z);
+    /* steps 2-3: z_l = -E z_u; z_l += rhs_l */
+    ORC_FN(orc_csr_matvec)(m, L->E_ia, L->E_ja, (const ORC_T*)L->E_a, -1.0, z, 0.0, zl);
+    for (i = 0; i < m; ++i) zl[i] += rhs[nl + i];
+  }
   /* step 4: low-rank correction, then the next level (or the inner iteration) */
   if (l == 0 && op->inner_maxits > 0) {
     ORC_FN(schur_inner)(op, zl, x + nl, t + (L->rank > 0 ? L->rank : 0));

@@ -33,7 +33,7 @@ IO = dict(
     LR_OPTION1=70, LR_OPTION2=71, LR_RANK1=74, LR_RANK2=75, LR_RANK_FACTOR1=78, LR_RANK_FACTOR2=79,
     LR_ARNOLDI_FACTOR1=82, LR_ARNOLDI_FACTOR2=83, LR_TOL_EIG1=86, LR_TOL_EIG2=87,
     LR_MAXITS1=90, LR_MAXITS2=91, LR_TR_FACTOR=94, LR_TOL_ORTH=95, LR_RAND_INIT=97,
-    SCHUR_ENABLE=130, SCHUR_ITER_TOL=131, SCHUR_MAXITS=132,
+    SCHUR_ENABLE=130, SCHUR_ITER_TOL=131, SCHUR_MAXITS=132, GLOBAL_SOLVE=141,
 )
 
 
@@ -46,7 +46,7 @@ class PgsError(RuntimeError):
 class GemslrInfo(C.Structure):
     _fields_ = [("n", C.c_int), ("nlev", C.c_int), ("is_complex", C.c_int), ("nnz_a", C.c_int64),
                 ("nnz_ilu", C.c_int64), ("nnz_lr", C.c_int64), ("setup_seconds", C.c_double * 4),
-                ("schur_maxits", C.c_int), ("schur_tol", C.c_double)]
+                ("schur_maxits", C.c_int), ("schur_tol", C.c_double), ("global_solve", C.c_int)]
 
 
 class LevelView(C.Structure):
@@ -432,6 +432,11 @@ class Gemslr:
             return None
         return int(inf.schur_maxits), float(inf.schur_tol)
 
+    def usolve(self):
+        """True when the apply uses the U variant of SolveLevel (no block L
+        solve on the way down; PARGEMSLR_IO_ADVANCED_GLOBAL_SOLVE = 1)."""
+        return self.info().global_solve == 1
+
     def close(self):
         if self._h:
             lib().pgs_gemslr_destroy(self._h)
@@ -755,7 +760,7 @@ class Server:
 
 
 def config_params(nlev, ncomp, droptol, rownnz, rank, kdim=50, maxits=1000, tol=1e-8, lr_tol_eig=1e-3,
-                  lr_maxits=10, droptol_c=None, rownnz_c=None, perm=1, schur=None):
+                  lr_maxits=10, droptol_c=None, rownnz_c=None, perm=1, schur=None, usolve=False):
     """Reference parameter array for a GeMSLR configuration.  schur: None
     (inner Schur iteration off) or (maxits, tol) of the inner FGMRES on the
     level-0 Schur complement (the reference's defaults are (5, 1e-2))."""
@@ -775,6 +780,7 @@ def config_params(nlev, ncomp, droptol, rownnz, rank, kdim=50, maxits=1000, tol=
     p[IO["LR_MAXITS1"]] = lr_maxits
     p[IO["LR_MAXITS2"]] = lr_maxits
     p[IO["SCHUR_ENABLE"]] = 0 if schur is None else 1
+    p[IO["GLOBAL_SOLVE"]] = 1 if usolve else 0
     if schur is not None:
         p[IO["SCHUR_MAXITS"]] = schur[0]
         p[IO["SCHUR_ITER_TOL"]] = schur[1]

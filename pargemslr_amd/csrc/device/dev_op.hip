@@ -246,6 +246,7 @@ struct pgs_dev_op_s {
   int ring_nt = 0;
   int ring_lpr = 4;
   int lr_mfma = 1;          // low-rank term of FP32 blocks of 16, 32 or 64 columns on the matrix cores
+  int usolve = 0;           // 1: the U variant of SolveLevel (no block L solve on the way down)
   int inner_maxits = 0;     // > 0: inner FGMRES on the level-0 Schur complement in every apply
   double inner_tol = 0.0;
   int64_t bytes = 0;
@@ -866,12 +867,19 @@ void solve_level(pgs_dev_op_s* op, pgs_ws_s* ws, DevPrec<V>& P, int l, const V* 
   const int nl = L.nloc;
   V* z = ws->zlev[l]->as<V>();
   V* zl = z + (int64_t)nl * nrhs;
-  // z_u = B^{-1} rhs_u
-  launch_trsv<V>(op, ws, l, L.B, rhs, z, nrhs, s);
-  // z_l = rhs_l - E z_u
-  timed(ws, s, PGS_PROF_SPMV_EF, spmv_bytes<V, V, V>(L.E, nrhs, true), [&] {
-    launch_spmv<V, V, V, V>(L.E, z, zl, rhs + (int64_t)nl * nrhs, (V)1, (V)0, 1, nrhs, s);
-  });
+  if (op->usolve) {
+    // the U variant (kGemslrUSolve, gemslr.cpp:3527-3532): z_l = rhs_l
+    const int64_t cnt = (int64_t)L.nrem * nrhs;
+    k_copy_conv<V, V><<<nblocks_for(cnt, kNT), kNT, 0, s>>>(cnt, rhs + (int64_t)nl * nrhs, zl);
+    HIPCHK(hipGetLastError());
+  } else {
+    // z_u = B^{-1} rhs_u
+    launch_trsv<V>(op, ws, l, L.B, rhs, z, nrhs, s);
+    // z_l = rhs_l - E z_u
+    timed(ws, s, PGS_PROF_SPMV_EF, spmv_bytes<V, V, V>(L.E, nrhs, true), [&] {
+      launch_spmv<V, V, V, V>(L.E, z, zl, rhs + (int64_t)nl * nrhs, (V)1, (V)0, 1, nrhs, s);
+    });
+  }
   if (l == 0 && op->inner_maxits > 0) {
     // x_l = inner FGMRES on the Schur complement (gemslr.cpp:3556-3563)
     inner_schur<V>(op, ws, P, zl, x + (int64_t)nl * nrhs, nrhs, s);
@@ -927,6 +935,7 @@ void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask, bool all_c =
     op->inner_maxits = H.prm.schur_maxits;
     op->inner_tol = H.prm.schur_tol;
   }
+  op->usolve = H.prm.global_solve == 1 ? 1 : 0;
   upload_csr<V64>(op->A, H.A, &op->bytes);
   upload(op->perm, H.perm, &op->bytes);
   // ring packing (ring kernel) unless another kernel geometry was asked for;
@@ -2071,8 +2080,8 @@ extern "C" int pgs_ws_profile_families(pgs_ws_t w, pgs_kernel_profile_t* out) {
 }
 
 // ---------------------------------------------------------------------------
-// Low-rank build of the setup on the device (f2 of the solve path's scope:
-// GemslrClass::SetupLowRank*, gemslr.cpp:2782-3257).  The thick-restart
+// Low-rank build of the setup on the device (GemslrClass::SetupLowRank*,
+// gemslr.cpp:2782-3257).  The thick-restart
 // Arnoldi driver (Hessenberg matrix, Schur forms, restart choices) runs on
 // the host as in the host setup; the basis and every m-length operation live
 // on the device: G x = (I - C Ct^{-1}) x + E B^{-1} F Ct^{-1} x with Ct^{-1}
@@ -2210,6 +2219,9 @@ int lowrank_build_device(HostOp<T>& H, int device, std::string* msg) {
     set_kernel_attrs(lds);
     op->is_complex = scalar_traits<V64>::is_complex ? 1 : 0;
     upload_op<V64, V32>(op.get(), H, PGS_PREC_FP64, true);
+    // the setup phase solves with the LU variant of SolveLevel whatever the
+    // solve phase will use (gemslr.cpp:3527: the U variant is for kGemslrPhaseSolve)
+    op->usolve = 0;
     for (int l = H.nlev - 2; l >= 0; --l) {
       {
         DevLrBackend<T, V64> be(op.get());

@@ -113,6 +113,7 @@ void fill_info(const HostOp<T>& H, pgs_gemslr_info_t* info) {
   const bool inner = H.prm.schur_enable && H.nlev > 1;
   info->schur_maxits = inner ? H.prm.schur_maxits : 0;
   info->schur_tol = inner ? H.prm.schur_tol : 0.0;
+  info->global_solve = H.prm.global_solve;
 }
 
 template <typename T>

@@ -23,7 +23,7 @@ namespace pgs {
 struct GemslrParams {
   int bsolve_top = 0;    // B solve of the top levels: 0 = ILUT (the only option implemented)
   int bsolve_lower = 0;  // B solve of the other levels: 0 = ILUT
-  int global_solve = 0;  // SolveLevel variant: 0 = LU (the only option implemented)
+  int global_solve = 0;  // SolveLevel variant of the solve phase: 0 = LU, 1 = U (skip the block L solve)
   int nlev = 5;
   int ncomp = 4;
   int partition = 1;     // 1 = RKway (the only option implemented)

@@ -259,12 +259,18 @@ void host_solve_level(const HostOp<T>& op, int l, const T* rhs, T* x, HostApplyW
   const int nl = op.nloc(l), m = op.nrem(l);
   T* z = w.z[l].data();
   T* zl = z + nl;
-  solve_b(op, l, rhs, z);
-  // z_l = rhs_l - E z_u
-  for (int i = 0; i < m; ++i) {
-    T s = T(0);
-    for (int k = L.E.ia[i]; k < L.E.ia[i + 1]; ++k) s += L.E.a[k] * z[L.E.ja[k]];
-    zl[i] = rhs[nl + i] - s;
+  if (op.solve_phase && op.prm.global_solve == 1) {
+    // the U variant (kGemslrUSolve, gemslr.cpp:3527-3532) skips the block L
+    // solve: z_l = rhs_l
+    for (int i = 0; i < m; ++i) zl[i] = rhs[nl + i];
+  } else {
+    solve_b(op, l, rhs, z);
+    // z_l = rhs_l - E z_u
+    for (int i = 0; i < m; ++i) {
+      T s = T(0);
+      for (int k = L.E.ia[i]; k < L.E.ia[i + 1]; ++k) s += L.E.a[k] * z[L.E.ja[k]];
+      zl[i] = rhs[nl + i] - s;
+    }
   }
   if (l == 0 && op.solve_phase && op.prm.schur_enable) {
     // x_l = inner FGMRES on the Schur complement (gemslr.cpp:3556-3563)
@@ -717,8 +723,8 @@ int gemslr_setup(const Csr<T>& A, const GemslrParams& prm, HostOp<T>& op, std::s
     if (msg) *msg = "only ILUT block solves (B solve option 0) are implemented";
     return 100;
   }
-  if (prm.global_solve != 0) {
-    if (msg) *msg = "only the LU variant of the level solve (global solve option 0) is implemented";
+  if (prm.global_solve != 0 && prm.global_solve != 1) {
+    if (msg) *msg = "only the LU and U variants of the level solve (global solve options 0, 1) are implemented";
     return 100;
   }
   if (prm.perm_option != 0 && prm.perm_option != 1) {

@@ -31,16 +31,17 @@ def dense(A):
     return A.to_scipy().toarray()
 
 
-def setup(A, nlev, ncomp, droptol, rownnz, rank, seed=0, lr_maxits=10, lr_tol_eig=1e-3, perm=1, schur=None):
+def setup(A, nlev, ncomp, droptol, rownnz, rank, seed=0, lr_maxits=10, lr_tol_eig=1e-3, perm=1, schur=None,
+          usolve=False):
     p = pg.config_params(nlev, ncomp, droptol, rownnz, rank, lr_maxits=lr_maxits, lr_tol_eig=lr_tol_eig, perm=perm,
-                         schur=schur)
+                         schur=schur, usolve=usolve)
     return pg.Gemslr(A, p, seed=seed)
 
 
 def oracle_of(G):
     """The oracle on G's factors, with G's inner Schur iteration settings."""
     from oracle import oracle as orc
-    return orc.GemslrOracle(G.A.n, G.perm(), G.levels(), schur=G.schur())
+    return orc.GemslrOracle(G.A.n, G.perm(), G.levels(), schur=G.schur(), usolve=G.usolve())
 
 
 # configurations (sizes scaled so the oracle finishes in seconds)

@@ -108,7 +108,7 @@ def test_setup_refuses_options_it_does_not_implement():
             pg.Gemslr(A, p)
         assert e.value.code == 100
     base = pg.config_params(2, 4, 1e-2, 10, 4)
-    for idx, val in ((1, 1), (3, 2), (141, 1), (58, 2), (70, 0)):
+    for idx, val in ((1, 1), (3, 2), (141, 2), (58, 2), (70, 0)):
         p = base.copy()
         p[idx] = val
         with pytest.raises(pg.PgsError) as e:
