@@ -125,3 +125,31 @@ def test_request_joins_a_running_batch_at_a_restart(gpu):
             pg.synchronize()
             assert st == 0 and abs(int(its[c]) - it) <= 2, (its[c], it)
             assert np.linalg.norm(X[:, c] - xd.numpy()) <= 1e-6 * np.linalg.norm(xd.numpy())
+
+
+def test_nonfinite_request_fails_alone(operators, gpu):
+    """A request with a NaN in its right-hand side, batched with finite
+    requests for the same operator: it returns PGS_ERR_NONFINITE after its
+    first cycle, and the other requests' columns converge with the iterations
+    and solutions they have without it (to rounding)."""
+    A, G, D = operators[1]
+    S = pg.Server(max_batch=8, slots=1, mem_reserve=1 << 30, kdim=20, tol=TOL)
+    i = S.add_operator(D)
+    B1 = np.stack([pg.rng_normal(8000 + c, A.n) for c in range(2)], axis=1)
+    B2 = np.stack([pg.rng_normal(8100 + c, A.n) for c in range(2)], axis=1)
+    Bbad = np.stack([pg.rng_normal(8200 + c, A.n) for c in range(2)], axis=1)
+    Bbad[A.n // 2, 1] = np.nan
+    outs = []
+    for with_bad in (False, True):
+        reqs = [S.submit(i, B1)] + ([S.submit(i, Bbad)] if with_bad else []) + [S.submit(i, B2)]
+        outs.append([S.wait(r) for r in reqs])
+    S.close()
+    (g1, g2), (b1, bad, b2) = outs
+    print(f"[server] bad request {bad[2]}, iterations {list(bad[1])}", flush=True)
+    assert bad[2]["status"] == pg.PGS_ERR_NONFINITE
+    assert int(bad[1][1]) == 0  # the NaN column stops before its first step
+    for (Xa, ia, fa), (Xb, ib, fb) in ((g1, b1), (g2, b2)):
+        assert fa["status"] == 0 and fb["status"] == 0
+        # the batch widths differ (the reductions' partial sums with them): rounding only
+        assert max(abs(int(a) - int(b)) for a, b in zip(ia, ib)) <= 1
+        assert np.linalg.norm(Xa - Xb) <= 1e-6 * np.linalg.norm(Xa)
