@@ -275,6 +275,7 @@ struct pgs_ws_s {
   int* flag_host = nullptr;
   int part_blocks = 0;
   bool fused_cgs2 = true;  // CGS2 as three k_basis_reg sweeps instead of four
+  int64_t basis_pad = 0;    // values between consecutive Krylov vectors (set at creation)
   int64_t v_bytes = 0;
   // kernel timing by family (pgs_ws_profile): HIP events on the launch
   // stream around every launch, and each launch's algorithmic bytes
@@ -514,13 +515,23 @@ void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
   if (op->trsv_cg == 0)
     while (cg > 1 && (int64_t)T.nblocks * (nrhs / cg) * 2 <= (int64_t)op->num_cus) cg >>= 1;
   // Eight columns per group halve the per-column loads, address arithmetic
-  // and reductions but lengthen each step: they pay only when four-column
-  // groups would already keep every CU four workgroups deep (measured at
-  // 128^3 level 0, FP32: 64 RHS 2.36 ms against 2.86 ms; 32 RHS 1.75 ms
-  // against 1.32 ms).
-  if (op->trsv_cg == 0 && sizeof(V) == 4 && nrhs >= 8 && (int64_t)T.nblocks * (nrhs / 4) >= 4 * (int64_t)op->num_cus)
-    cg = 8;
-  if (op->ring_lpr == 2) cg = std::min(cg, 2);  // a lane pair finalizes at most two columns
+  // and reductions but lengthen each step.  They pay when four-column groups
+  // would put four or more one-wave workgroups on every CU, and at two per
+  // CU when the half steps make a block's chain half again as long as its
+  // whole steps (level sets of more than 16 rows), so that 8-column groups
+  // of 128 threads on whole steps shorten it.  Measured at 128^3, FP32, ms
+  // per launch: level 0 (6,300 half against 3,350 whole steps) at 32 RHS
+  // 1.19 with 8-column groups of 128 threads against 1.31 with 4-column
+  // one-wave groups, at 64 RHS 1.59 with 8-column one-wave groups against
+  // 3.14 with 4 (2.06 with 8 columns and 128 threads), at 16 RHS 0.85 with
+  // 4 columns against 1.18 with 8; level 1 (800 half against 700 whole
+  // steps) at 32 RHS 0.16 with 4-column one-wave groups against 0.24.
+  if (op->trsv_cg == 0 && sizeof(V) == 4 && nrhs >= 8) {
+    const int64_t g4 = (int64_t)T.nblocks * (nrhs / 4);
+    const bool long_halves = T.desc_h.p && 2 * (int64_t)T.max_block_steps_h >= 3 * (int64_t)T.max_block_steps;
+    if (g4 >= 4 * (int64_t)op->num_cus || (g4 >= 2 * (int64_t)op->num_cus && long_halves)) cg = 8;
+  }
+  if (op->ring_lpr == 2) cg = std::min(cg, 4);  // a lane pair finalizes at most two columns each
   while (cg > 1 && (size_t)(T.window + 1) * cg * sizeof(V) > (size_t)op->lds_max) cg >>= 1;
   // Per launch: at two or more workgroups per CU the CUs' issue and memory
   // pipes bound the solve, and one-wave workgroups on half steps (no barrier
@@ -531,7 +542,8 @@ void launch_trsv_ring(pgs_dev_op_s* op, const DevTrsv& T, const V* rhs, V* x, in
     nt = (T.desc_h.p && (int64_t)T.nblocks * (nrhs / cg) >= 2 * (int64_t)op->num_cus) ? 64 : 128;
   if (nt == 64 && op->ring_lpr == 4 && !T.desc_h.p) nt = 128;
   if (nt == 64 && op->ring_lpr == 2) {
-    if (cg >= 2) launch_trsv_ring_cg<V, 64, 2, 2>(T, rhs, x, nrhs, s);
+    if (cg >= 4) launch_trsv_ring_cg<V, 64, 2, 4>(T, rhs, x, nrhs, s);
+    else if (cg >= 2) launch_trsv_ring_cg<V, 64, 2, 2>(T, rhs, x, nrhs, s);
     else launch_trsv_ring_cg<V, 64, 2, 1>(T, rhs, x, nrhs, s);
   } else if (nt == 64) {
     if constexpr (sizeof(V) == 4) {
@@ -615,6 +627,7 @@ void set_trsv_attrs(int lds_max) {
   set_ring_attrs(k_trsv_ring<V, 64, 4, kRingWidth / 4, D4, 4, 1>, lds_max);
   set_ring_attrs(k_trsv_ring<V, 64, 2, kRingWidth / 2, (ring_depth<V, 1, 2>()), 1, 1>, lds_max);
   set_ring_attrs(k_trsv_ring<V, 64, 2, kRingWidth / 2, (ring_depth<V, 2, 2>()), 2, 1>, lds_max);
+  set_ring_attrs(k_trsv_ring<V, 64, 2, kRingWidth / 2, (ring_depth<V, 4, 2>()), 4, 1>, lds_max);
   if constexpr (sizeof(V) == 4) {
     constexpr int D8 = ring_depth<V, 8>();
     set_ring_attrs(k_trsv_ring<V, 128, 4, kRingWidth / 4, D8, 8, 1>, lds_max);
@@ -982,6 +995,14 @@ void upload_op(pgs_dev_op_s* op, const HostOp<T>& H, int prec_mask, bool all_c =
   if (prec_mask & PGS_PREC_FP32) build(prec_of<V32>(op), V32(0));
 }
 
+// Krylov vectors (and their preconditioned images) lie kBasisPad values
+// apart beyond their length: with a stride of exactly n * nrhs values (a
+// power of two times n) the dozens of vectors one sweep streams start on the
+// same memory channels.  At 2^21 rows x 32 columns the delayed-CGS2 sweeps
+// ran 2-7 % faster with the padded stride (the update over 40 vectors 4.50
+// against 4.20 TB/s, the dots over 24 vectors 6.06 against 5.95 TB/s).
+constexpr int64_t kBasisPad = 320;
+
 // Byte sizes of every buffer of a workspace (nr columns, a power of two; Krylov
 // dimension kdim): one list serves the allocation and pgs_ws_estimate.
 struct WsSizes {
@@ -1020,8 +1041,8 @@ WsSizes ws_sizes(const pgs_dev_op_s* op, int nr, int kdim) {
   add(&pgs_ws_s::x0, vb);
   add(&pgs_ws_s::lrpart, (size_t)std::max(1, (kLrBlocks + 1) * maxrank * nr) * es);
   add(&pgs_ws_s::lrt, (size_t)std::max(1, maxrank * nr) * es);
-  add(&pgs_ws_s::V, (ik + 1) * vb);
-  add(&pgs_ws_s::Z, ik * vb);
+  add(&pgs_ws_s::V, (ik + 1) * (vb + kBasisPad * es));
+  add(&pgs_ws_s::Z, ik * (vb + kBasisPad * es));
   // partial rows per workgroup: kdim + 1 dots, or 2 * 32 + 2 of a delayed-CGS2 dot chunk
   add(&pgs_ws_s::part, (size_t)z.part_blocks * std::max(kdim + 1, kPartRowsDcgs2) * nr * es);
   add(&pgs_ws_s::hbuf, (ik + 1) * nr * es);
@@ -1192,6 +1213,8 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
     ws->part_blocks = z.part_blocks;
     ws->v_bytes = (int64_t)z.vb;
     if (const char* v = std::getenv("PGS_FUSED_CGS2")) ws->fused_cgs2 = std::atoi(v) != 0;
+    ws->basis_pad = kBasisPad;
+    if (const char* v = std::getenv("PGS_BASIS_PAD")) ws->basis_pad = std::atoi(v) == 0 ? 0 : kBasisPad;
     ws->zlev.clear();
     for (size_t b : z.zlev) {
       std::unique_ptr<DevMem> m(new DevMem());
@@ -1420,7 +1443,7 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
                 pgs_fgmres_result_t* res, hipStream_t s) {
   const int64_t n = op->n;
   const int m = o.kdim;
-  const int64_t ldv = n * nrhs;
+  const int64_t ldv = n * nrhs + ws->basis_pad;
   S* V = ws->V.as<S>();
   VZ* Z = ws->Z.as<VZ>();
   KState* st = ws->st.as<KState>();

@@ -428,6 +428,29 @@ __device__ __forceinline__ void bstore_val(__amdgpu_buffer_rsrc_t r, int off, cp
   __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, off, 0, 0);
 }
 
+// Two consecutive values in one buffer access (8-byte and smaller values).
+template <typename V>
+__device__ __forceinline__ RingRow<V, 2> bload_val2(__amdgpu_buffer_rsrc_t r, int off) {
+  static_assert(sizeof(V) <= 8, "two values in one access");
+  RingRow<V, 2> o;
+  if constexpr (sizeof(V) == 4) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    o = __builtin_bit_cast(RingRow<V, 2>, v);
+  } else {
+    auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    o = __builtin_bit_cast(RingRow<V, 2>, v);
+  }
+  return o;
+}
+template <typename V>
+__device__ __forceinline__ void bstore_val2(__amdgpu_buffer_rsrc_t r, int off, const RingRow<V, 2>& o) {
+  static_assert(sizeof(V) <= 8, "two values in one access");
+  if constexpr (sizeof(V) == 4)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, o), r, off, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, o), r, off, 0, 0);
+}
+
 // An empty asm that takes a value as read and rewritten: the value must have
 // arrived, and it counts as produced here rather than by its load.
 __device__ __forceinline__ void pin_reg(int& v) { __asm__ __volatile__("" : "+v"(v)); }
@@ -459,14 +482,19 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   constexpr int VS = (int)sizeof(V);
   constexpr int RW = LPR * EPL;  // slab width of every ring-packed step
   static_assert(D % 2 == 0, "stage depth must be even");
-  static_assert((LPR == 4 && (CG == 1 || CG == 2 || CG == 4 || CG == 8)) || (LPR == 2 && (CG == 1 || CG == 2)),
-                "quad lanes with 1, 2, 4 or 8 columns, or lane pairs with 1 or 2 columns");
-  constexpr int NCL = CG == 8 ? 2 : 1;  // columns finalized per lane
+  static_assert((LPR == 4 && (CG == 1 || CG == 2 || CG == 4 || CG == 8)) ||
+                    (LPR == 2 && (CG == 1 || CG == 2 || CG == 4)),
+                "quad lanes with 1, 2, 4 or 8 columns, or lane pairs with 1, 2 or 4 columns");
+  constexpr int NCL = CG == 2 * LPR ? 2 : 1;  // columns finalized per lane
+  // a lane's two columns are adjacent: one access for both (8-byte and
+  // smaller values)
+  constexpr bool kPairIO = NCL == 2 && sizeof(V) <= 8;
   const int tid = threadIdx.x % NT;  // thread within this column group
   const int q = tid % LPR;
   const int t0 = tid / LPR;
   // after the reduction lane q holds column q % CG (lanes q < CG write it), or
-  // with 8 columns columns 2q and 2q + 1
+  // with two columns per lane (8 columns on quads, 4 on lane pairs) columns
+  // 2q and 2q + 1
   int cl[NCL];
   if constexpr (NCL == 2) {
     cl[0] = 2 * q;
@@ -518,8 +546,14 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
   } while (0)
 #define PGS_RING_RHS(K)                                                           \
   do {                                                                            \
-    _Pragma("unroll") for (int j_ = 0; j_ < NCL; ++j_) xr[K][j_] =                \
-        bload_val(rs_src, ((pr[K].col << csh) + c0 + cl[j_]) * VS, V(0));         \
+    if constexpr (kPairIO) {                                                      \
+      const RingRow<V, 2> t_ = bload_val2<V>(rs_src, ((pr[K].col << csh) + c0 + cl[0]) * VS); \
+      xr[K][0] = t_.v[0];                                                         \
+      xr[K][1] = t_.v[1];                                                         \
+    } else {                                                                      \
+      _Pragma("unroll") for (int j_ = 0; j_ < NCL; ++j_) xr[K][j_] =              \
+          bload_val(rs_src, ((pr[K].col << csh) + c0 + cl[j_]) * VS, V(0));       \
+    }                                                                             \
   } while (0)
 
   const int rel = s_begin - sbeg;
@@ -593,6 +627,11 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
         // xor 1: keep columns 0-1 of the half (q bit 0 clear) or 2-3
 #pragma unroll
         for (int c = 0; c < 2; ++c) a[c] = (b0 ? h[c + 2] : h[c]) + dpp_xor1(b0 ? h[c] : h[c + 2]);
+      } else if constexpr (LPR == 2 && CG == 4) {
+        // xor 1: keep columns 0-1 (q = 0) or 2-3 (q = 1), send the other pair
+        const bool b0 = q & 1;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) a[c] = (b0 ? acc[c + 2] : acc[c]) + dpp_xor1(b0 ? acc[c] : acc[c + 2]);
       } else if constexpr (LPR == 2) {
         if constexpr (CG == 2) {
           const bool b0 = q & 1;
@@ -616,12 +655,23 @@ __device__ __forceinline__ void ring_sweep(int s_begin, int s_end, int sbeg, int
       }
       // lanes with nothing to write store to the dummy LDS slot and out of
       // the buffer's range (dropped): no branch
+      if constexpr (kPairIO) {
+        RingRow<V, 2> o;
 #pragma unroll
-      for (int j = 0; j < NCL; ++j) {
-        V v = xin[j] - a[j];
-        if (UPPER) v *= dv;
-        xs[wr ? slot * CG + cl[j] : dummy] = v;
-        bstore_val(rs_dst, wr ? ((row << csh) + c0 + cl[j]) * VS : 0x7ffffff0, v);
+        for (int j = 0; j < 2; ++j) {
+          o.v[j] = xin[j] - a[j];
+          if (UPPER) o.v[j] *= dv;
+        }
+        *reinterpret_cast<RingRow<V, 2>*>(xs + (wr ? slot * CG + cl[0] : dummy)) = o;
+        bstore_val2<V>(rs_dst, wr ? ((row << csh) + c0 + cl[0]) * VS : 0x7ffffff0, o);
+      } else {
+#pragma unroll
+        for (int j = 0; j < NCL; ++j) {
+          V v = xin[j] - a[j];
+          if (UPPER) v *= dv;
+          xs[wr ? slot * CG + cl[j] : dummy] = v;
+          bstore_val(rs_dst, wr ? ((row << csh) + c0 + cl[j]) * VS : 0x7ffffff0, v);
+        }
       }
       // refill this stage with step s + D and load the right-hand side of
       // step s + D/2, whose row pair arrived D/2 steps ago.  The refill

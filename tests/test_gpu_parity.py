@@ -152,19 +152,20 @@ def test_fgmres_unpreconditioned_matches_oracle(c1, gpu):
 
 @pytest.mark.parametrize("env", [{}, {"PGS_TRSV_CG": "1"}, {"PGS_TRSV_CG": "2"},
                                  {"PGS_RING_GEOM": "64x4"}, {"PGS_RING_GEOM": "64x4", "PGS_TRSV_CG": "1"},
-                                 {"PGS_RING_GEOM": "64x2"}, {"PGS_RING_GEOM": "64x2", "PGS_TRSV_CG": "1"},
+                                 {"PGS_RING_GEOM": "64x2", "PGS_TRSV_CG": "2"}, {"PGS_RING_GEOM": "64x2", "PGS_TRSV_CG": "1"},
+                                 {"PGS_RING_GEOM": "64x2", "PGS_TRSV_CG": "4"},
                                  {"PGS_TRSV_NT": "256"}, {"PGS_TRSV_VARIANT": "1"}, {"PGS_TRSV_GROUPS": "1"},
                                  {"PGS_TRSV_GROUPS": "2"}, {"PGS_TRSV_CG": "8", "PGS_RING_GEOM": "128x4"},
                                  {"PGS_TRSV_CG": "8", "PGS_RING_GEOM": "64x4"}],
-                         ids=["ring4", "ring1", "ring2", "wave4x4", "wave4x1", "wave2x2", "wave2x1", "pipelined",
+                         ids=["ring4", "ring1", "ring2", "wave4x4", "wave4x1", "wave2x2", "wave2x1", "wave2x4", "pipelined",
                               "direct", "ring4-one-group", "ring4-paired", "ring8", "wave4x8"])
 def test_apply_trsv_kernels(env, small, gpu, monkeypatch):
     """Every triangular-solve kernel (ring with 128 threads and 1, 2 or 4
     columns per group, two 4-column groups per workgroup for eight columns
     when the paired grid fills the device, else one; 8 columns per group for
     FP32 values, with 128 threads or one wave; single-wave ring on half
-    steps with 4 lanes per row and 1 or 4 columns, or 2 lanes per row and 1
-    or 2 columns;
+    steps with 4 lanes per row and 1 or 4 columns, or whole steps with 2
+    lanes per row and 1, 2 or 4 columns;
     the pipelined kernel; the direct kernel) gives the oracle's apply, for one
     and for eight right-hand sides."""
     A, G, _, _ = small

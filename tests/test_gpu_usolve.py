@@ -59,7 +59,10 @@ def test_apply_parity(cplx, schur, gpu):
 
 
 def test_fgmres_parity(gpu):
-    A, G = _op(False)
+    """On the convection-diffusion operator (the shifted Laplacian above is
+    indefinite, and the U variant leaves FGMRES(50) stagnating on it)."""
+    A = pg.convdiff(24, 24, 24, 0.3)
+    G = P.setup(A, 3, 8, 1e-3, 20, 16, usolve=True)
     D = pg.DevOp(G)
     W = pg.Workspace(D, max_nrhs=1, kdim=50)
     b = _rhs(A, 21)
