@@ -43,8 +43,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes read + written per level-0 ring-solve launch of the 128^3 operator
 # (FP32 apply), by right-hand sides per launch, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --probe --nrhs N`, each
-# scaled by what the counter reads for a 1 GiB device copy
-TRAFFIC_LEVEL0 = {32: 1994.3e6}
+# scaled by what the counter reads for a 1 GiB device copy (8-column groups of
+# 128 threads at 32 columns)
+TRAFFIC_LEVEL0 = {32: 1667.6e6}
 
 
 def log(*a):
@@ -80,6 +81,9 @@ def main():
                     help="inner Schur iteration 'maxits,tol' (e.g. 5,1e-2, the reference's default); off if empty")
     ap.add_argument("--probe", action="store_true",
                     help="counter probe: --nrhs FP32 applies of the 128^3 operator and a 1 GiB copy")
+    ap.add_argument("--trsv-sweep", default="",
+                    help="comma-separated column counts: level-set solve times per launch (FP32 and FP64 applies"
+                         " of the 128^3 operator, the launch's own geometry or the one PGS_* variables force)")
     ap.add_argument("--spmv-probe", action="store_true",
                     help="one-column SpMV of the 128^3 Laplacian with 8, 4, 2 lanes per row and one thread per row,"
                          " and 32-column products with 4 and 8 columns per thread (for a kernel trace; each variant"
@@ -100,6 +104,9 @@ def main():
         return
     if args.spmv_probe:
         spmv_probe(args)
+        return
+    if args.trsv_sweep:
+        trsv_sweep(args)
         return
     if args.restart:
         restart(args)
@@ -441,6 +448,46 @@ def restart(args):
                       "apply_bit_identical": same}), flush=True)
     if not same:
         sys.exit(1)
+
+
+def trsv_sweep(args):
+    """Level-set solve times of the 128^3 operator (C2 setup) per column
+    count: one warm-up apply, then five applies with the workspace profiler on;
+    prints ms per launch of level 0 and of levels 1+ (the profiler's trsv
+    families) and checks every width's columns against the one-column apply."""
+    import pargemslr_amd as pg
+
+    cfg = CONFIGS["c2"]
+    A = pg.laplacian(*cfg["grid"])
+    G = pg.Gemslr(A, pg.config_params(cfg["nlev"], cfg["ncomp"], cfg["droptol"], cfg["rownnz"], cfg["rank"]), seed=0)
+    nrs = [int(x) for x in args.trsv_sweep.split(",")]
+    D = pg.DevOp(G)
+    W = pg.Workspace(D, max_nrhs=max(nrs), kdim=2)
+    V = np.stack([pg.rng_normal(c, A.n) for c in range(max(nrs))], axis=1)
+    for name, prec in (("fp32", pg.PGS_PREC_FP32), ("fp64", pg.PGS_PREC_FP64)):
+        z1 = pg.DeviceArray(A.n)
+        W.apply(pg.DeviceArray.from_numpy(np.ascontiguousarray(V[:, 0])), z1, prec=prec)
+        pg.synchronize()
+        ref = z1.numpy()
+        for nr in nrs:
+            Vd = pg.DeviceArray.from_numpy(np.ascontiguousarray(V[:, :nr]))
+            Z = pg.DeviceArray((A.n, nr))
+            W.apply(Vd, Z, nrhs=nr, prec=prec)
+            pg.synchronize()
+            W.profile(True)
+            for _ in range(5):
+                W.apply(Vd, Z, nrhs=nr, prec=prec)
+            pg.synchronize()
+            f = W.profile_families()
+            W.profile(False)
+            col0 = Z.numpy().reshape(A.n, nr)[:, 0]
+            diff = np.linalg.norm(col0 - ref) / np.linalg.norm(ref)
+            l0, l1 = f["trsv level 0"], f["trsv levels 1+"]
+            log(f"[trsv-sweep] {name} nrhs {nr:3d}: level 0 {l0['total_ms'] / l0['launches']:.3f} ms/launch,"
+                f" levels 1+ {l1['total_ms'] / max(l1['launches'], 1):.3f} ms/launch,"
+                f" column 0 vs the one-column apply {diff:.1e}")
+    W.close()
+    D.close()
 
 
 def spmv_probe(args):
