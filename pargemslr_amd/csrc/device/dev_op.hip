@@ -1018,7 +1018,11 @@ struct WsSizes {
   }
 };
 
-WsSizes ws_sizes(const pgs_dev_op_s* op, int nr, int kdim) {
+// lowrank_build: only what the device low-rank build's level solves,
+// multi-dots and norms use (level temporaries, partial sums, the low-rank
+// term's partials); no Krylov bases, Hessenberg state or inner-iteration
+// buffers.
+WsSizes ws_sizes(const pgs_dev_op_s* op, int nr, int kdim, bool lowrank_build = false) {
   WsSizes z;
   const int64_t n = op->n;
   const size_t es = vec_es(op);  // bytes of one value of the operator (double or complex<double>)
@@ -1037,6 +1041,13 @@ WsSizes ws_sizes(const pgs_dev_op_s* op, int nr, int kdim) {
   z.part_blocks = (int)std::max<int64_t>(1, (n + rows_per_blk - 1) / rows_per_blk);
   const size_t ik = (size_t)kdim;
   auto add = [&](DevMem pgs_ws_s::*m, size_t b) { z.bufs.push_back({m, b}); };
+  if (lowrank_build) {
+    add(&pgs_ws_s::lrpart, (size_t)std::max(1, (kLrBlocks + 1) * maxrank * nr) * es);
+    add(&pgs_ws_s::lrt, (size_t)std::max(1, maxrank * nr) * es);
+    add(&pgs_ws_s::part, (size_t)z.part_blocks * std::max(kdim + 1, kPartRowsDcgs2) * nr * es);
+    add(&pgs_ws_s::flag, 16);
+    return z;
+  }
   add(&pgs_ws_s::r0, vb);
   add(&pgs_ws_s::x0, vb);
   add(&pgs_ws_s::lrpart, (size_t)std::max(1, (kLrBlocks + 1) * maxrank * nr) * es);
@@ -1197,10 +1208,9 @@ int pgs_ws_estimate(pgs_dev_op_t dop, int max_nrhs, int kdim, int64_t* bytes) {
   return 0;
 }
 
-int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
-  if (!dop || !out || max_nrhs < 1 || max_nrhs > 64 || kdim < 1 || kdim + 1 > kPMAX) return PGS_ERROR_INVALID_PARAM;
+namespace {
+int ws_create(pgs_dev_op_s* op, int max_nrhs, int kdim, bool lowrank_build, pgs_ws_t* out) {
   *out = nullptr;
-  pgs_dev_op_s* op = reinterpret_cast<pgs_dev_op_s*>(dop);
   int nr = 1;
   while (nr < max_nrhs) nr <<= 1;
   std::unique_ptr<pgs_ws_s> ws(new pgs_ws_s());
@@ -1209,7 +1219,7 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
   ws->kdim = kdim;
   try {
     HIPCHK(hipSetDevice(op->device));
-    const WsSizes z = ws_sizes(op, nr, kdim);
+    const WsSizes z = ws_sizes(op, nr, kdim, lowrank_build);
     ws->part_blocks = z.part_blocks;
     ws->v_bytes = (int64_t)z.vb;
     if (const char* v = std::getenv("PGS_FUSED_CGS2")) ws->fused_cgs2 = std::atoi(v) != 0;
@@ -1228,6 +1238,12 @@ int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
   }
   *out = reinterpret_cast<pgs_ws_t>(ws.release());
   return 0;
+}
+}  // namespace
+
+int pgs_ws_create(pgs_dev_op_t dop, int max_nrhs, int kdim, pgs_ws_t* out) {
+  if (!dop || !out || max_nrhs < 1 || max_nrhs > 64 || kdim < 1 || kdim + 1 > kPMAX) return PGS_ERROR_INVALID_PARAM;
+  return ws_create(reinterpret_cast<pgs_dev_op_s*>(dop), max_nrhs, kdim, false, out);
 }
 
 int pgs_ws_destroy(pgs_ws_t w) {
@@ -2155,7 +2171,8 @@ struct DevLrBackend final : LrBackend<T> {
     l = lev;
     m = mm;
     ncols = nc;
-    const int e = pgs_ws_create(reinterpret_cast<pgs_dev_op_t>(op), 1, 1, &wsh);
+    // a workspace of the level temporaries and partial sums only
+    const int e = ws_create(op, 1, 1, true, &wsh);
     if (e) throw HipError{e == PGS_ERR_OOM ? hipErrorOutOfMemory : hipErrorUnknown};
     ws = reinterpret_cast<pgs_ws_s*>(wsh);
     Vb.alloc(sizeof(V) * (size_t)m * nc);
