@@ -19,6 +19,7 @@ PGS_ORTH_MGS = 0
 PGS_ORTH_CGS2 = 1
 PGS_ORTH_DCGS2 = 2
 PGS_ERROR_INVALID_PARAM = 101
+PGS_ERR_HIP = 200
 PGS_ERR_BREAKDOWN = 203
 PGS_ERR_NOT_CONVERGED = 204
 PGS_ERR_NONFINITE = 206
@@ -203,6 +204,8 @@ def strerror(code):
 
 def _chk(code, what):
     if code != 0:
+        if code == PGS_ERR_HIP:
+            raise PgsError(code, f"{what} [{lib().pgs_last_hip_error().decode(errors='replace')}]")
         raise PgsError(code, what)
 
 

@@ -1130,6 +1130,7 @@ int pgs_dev_op_create(pgs_gemslr_t hop, int device, int prec_mask, pgs_dev_op_t*
   std::unique_ptr<pgs_dev_op_s> op(new pgs_dev_op_s());
   try {
     HIPCHK(hipSetDevice(device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     op->device = device;
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -1219,6 +1220,7 @@ int ws_create(pgs_dev_op_s* op, int max_nrhs, int kdim, bool lowrank_build, pgs_
   ws->kdim = kdim;
   try {
     HIPCHK(hipSetDevice(op->device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     const WsSizes z = ws_sizes(op, nr, kdim, lowrank_build);
     ws->part_blocks = z.part_blocks;
     ws->v_bytes = (int64_t)z.vb;
@@ -1262,6 +1264,7 @@ int pgs_spmv(pgs_dev_op_t dop, pgs_ws_t w, const double* alpha, const double* x,
   (void)w;
   try {
     HIPCHK(hipSetDevice(op->device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     if (op->is_complex) {
       using Z = cplx<double>;
       launch_spmv<Z, Z, Z, Z>(op->A, reinterpret_cast<const Z*>(x), reinterpret_cast<Z*>(y), (const Z*)nullptr,
@@ -1284,6 +1287,7 @@ namespace {
 int apply_dispatch(pgs_dev_op_s* op, pgs_ws_s* ws, const double* v, double* z, int nrhs, int prec, void* stream) {
   try {
     HIPCHK(hipSetDevice(op->device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     hipStream_t s = (hipStream_t)stream;
     using Z = cplx<double>;
     const Z* vz = reinterpret_cast<const Z*>(v);
@@ -1692,6 +1696,7 @@ extern "C" int pgs_fgmres(pgs_dev_op_t dop, pgs_ws_t w, const double* b, double*
   if (opts->use_precond && opts->prec == PGS_PREC_FP32 && !has32) return PGS_ERROR_INVALID_OPTION;
   try {
     HIPCHK(hipSetDevice(op->device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     hipStream_t s = (hipStream_t)stream;
     const bool low = opts->use_precond && opts->prec == PGS_PREC_FP32;
     if (op->is_complex) {
@@ -1935,6 +1940,7 @@ extern "C" int pgs_dev_csr_create(int device, int n, const int* ia, const int* j
     m->device = device;
     m->is_complex = is_complex ? 1 : 0;
     HIPCHK(hipSetDevice(device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     if (is_complex) {
       Csr<zdouble> H;
       H.nrows = H.ncols = n;
@@ -2073,6 +2079,7 @@ extern "C" int pgs_ws_profile(pgs_ws_t w, int enable) {
   pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
   try {
     HIPCHK(hipSetDevice(ws->op->device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     prof_flush(ws);
   } catch (const HipError& e) {
     return set_err(e);
@@ -2091,6 +2098,7 @@ extern "C" int pgs_ws_profile_read(pgs_ws_t w, pgs_kernel_profile_t* out) {
   pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
   try {
     HIPCHK(hipSetDevice(ws->op->device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     prof_flush(ws);
   } catch (const HipError& e) {
     return set_err(e);
@@ -2106,6 +2114,7 @@ extern "C" int pgs_ws_profile_families(pgs_ws_t w, pgs_kernel_profile_t* out) {
   pgs_ws_s* ws = reinterpret_cast<pgs_ws_s*>(w);
   try {
     HIPCHK(hipSetDevice(ws->op->device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     prof_flush(ws);
   } catch (const HipError& e) {
     return set_err(e);
@@ -2251,6 +2260,7 @@ int lowrank_build_device(HostOp<T>& H, int device, std::string* msg) {
   try {
     std::unique_ptr<pgs_dev_op_s> op(new pgs_dev_op_s());
     HIPCHK(hipSetDevice(device));
+    (void)hipGetLastError();  // an earlier call's error is not this call's
     op->device = device;
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
