@@ -559,17 +559,17 @@ class DeviceArray:
     def upload(self, a, stream=None):
         a = np.ascontiguousarray(a, dtype=self.dtype)
         assert a.nbytes == self.nbytes
-        _chk(lib().pgs_memcpy_htod(self.device, self._p, C.c_void_p(a.ctypes.data), self.nbytes, stream),
+        _chk(lib().pgs_memcpy_htod(self.device, self._p, C.c_void_p(a.ctypes.data), self.nbytes, _sh(stream)),
              "pgs_memcpy_htod")
 
     def numpy(self, stream=None):
         out = np.empty(self.shape, dtype=self.dtype)
-        _chk(lib().pgs_memcpy_dtoh(self.device, C.c_void_p(out.ctypes.data), self._p, self.nbytes, stream),
+        _chk(lib().pgs_memcpy_dtoh(self.device, C.c_void_p(out.ctypes.data), self._p, self.nbytes, _sh(stream)),
              "pgs_memcpy_dtoh")
         return out
 
     def zero(self, stream=None):
-        _chk(lib().pgs_memset_zero(self.device, self._p, self.nbytes, stream), "pgs_memset_zero")
+        _chk(lib().pgs_memset_zero(self.device, self._p, self.nbytes, _sh(stream)), "pgs_memset_zero")
 
     def close(self):
         if self._p:

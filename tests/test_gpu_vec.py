@@ -85,3 +85,20 @@ def test_device_csr_matvec(gpu, cplx):
     assert L.pgs_dev_csr_create(0, A.n, pg.capi._iptr(A.ia), pg.capi._iptr(bad),
                                 a.ctypes.data_as(C.POINTER(C.c_double)), int(cplx), C.byref(h2)) == \
         pg.capi.PGS_ERROR_INVALID_PARAM
+
+
+def test_device_array_copies_on_a_stream(gpu):
+    """DeviceArray's upload, zero and numpy take a Stream object as well as a
+    raw stream handle."""
+    s = pg.Stream()
+    try:
+        x = np.arange(1000, dtype=np.float64)
+        d = pg.DeviceArray((1000,))
+        d.upload(x, stream=s)
+        s.synchronize()
+        assert np.array_equal(d.numpy(stream=s), x)
+        d.zero(stream=s.handle)
+        s.synchronize()
+        assert not d.numpy().any()
+    finally:
+        s.close()
