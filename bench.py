@@ -43,9 +43,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes read + written per level-0 ring-solve launch of the 128^3 operator
 # (FP32 apply), by right-hand sides per launch, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes over `bench.py --probe --nrhs N`, each
-# scaled by what the counter reads for a 1 GiB device copy (8-column groups of
-# 128 threads at 32 columns)
-TRAFFIC_LEVEL0 = {32: 1667.6e6}
+# scaled by what the counter reads for a 1 GiB device copy (8-column groups: of
+# 128 threads on whole steps at 32 columns, one wave on half steps at 64)
+TRAFFIC_LEVEL0 = {32: 1667.6e6, 64: 2742.2e6}
 
 
 def log(*a):
@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--nrhs", type=int, default=32)
+    ap.add_argument("--nrhs", type=int, default=64, help="right-hand sides per batched solve")
     ap.add_argument("--single-solves", type=int, default=5, help="timed single-RHS solves per form")
     ap.add_argument("--fp64-batch-steps", type=int, default=3, help="timed batched solves in the plain FP64 form")
     ap.add_argument("--profile-steps", type=int, default=2,

@@ -91,9 +91,9 @@ extern "C" {
 #define PGS_ORTH_MGS 0  /* modified Gram-Schmidt, as FlexGmresClass (fgmres.hpp:424-426) */
 #define PGS_ORTH_CGS2 1 /* classical Gram-Schmidt with one re-orthogonalization pass */
 #define PGS_ORTH_DCGS2 2 /* CGS2 with the re-orthogonalization delayed into the next step
-                            (two basis sweeps per step instead of three); real bases up to
-                            64 vectors or with at most 32 columns, complex bases up to 64
-                            vectors; larger ones use PGS_ORTH_CGS2 */
+                            (two basis sweeps per step instead of three); real bases of any
+                            Krylov dimension, complex bases up to 64 vectors (larger ones use
+                            PGS_ORTH_CGS2) */
 
 typedef struct pgs_gemslr_s* pgs_gemslr_t; /* host-resident factored operator (setup output) */
 typedef struct pgs_dev_op_s* pgs_dev_op_t; /* operator resident in device memory */

@@ -666,7 +666,6 @@ void set_kernel_attrs(int lds_max) {
     set_lds_limit(k_dcgs2_update<double, kBasisNT, 8, 1>, lds_max);
     set_lds_limit(k_dcgs2_update<double, kBasisNT, 16, 1>, lds_max);
     set_lds_limit(k_dcgs2_update<double, kBasisNT, 32, 1>, lds_max);
-    set_lds_limit(k_dcgs2_update<double, kBasisNT, 64, 1>, lds_max);
     set_lds_limit(k_dcgs2_update<double, kBasisNT, 32, 2>, lds_max);
     set_lds_limit(k_dcgs2_update<double, kBasisNT, 64, 2>, lds_max);
     using Z = cplx<double>;
@@ -1406,6 +1405,9 @@ void dcgs2_dots(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ld
                 hipStream_t s) {
   constexpr int PM = scalar_traits<S>::is_complex ? 8 : 16;
   const S* q = V + (int64_t)p * ldv;
+  // two lanes per element up to 32 columns; at 64 columns one lane per
+  // element runs faster although its chunks of 16 vectors read q and w twice
+  // as often (128^3, 64 columns: 478 against 500 ms of dots per two solves)
   const int lpe = nrhs <= 32 ? 2 : 1;
   const int chunk = PM * lpe;
   const int64_t nthr = (n * nrhs * lpe + kBasisNT - 1) / kBasisNT;
@@ -1427,14 +1429,14 @@ void dcgs2_dots(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ld
 }
 
 // Delayed CGS2, update of step p (k_dcgs2_update), then t2[c] = ||w(:, c)||^2.
-// Real values: one lane per element up to 32 vectors (64 with 64 columns),
-// then two; complex values (twice the registers per vector): one lane up to
-// 16 vectors, then two, at most 64 vectors.
+// Real values: one lane per element up to 32 vectors, then two (the one-lane
+// 64-vector kernel needs more than 256 VGPRs); complex values (twice the
+// registers per vector): one lane up to 16 vectors, then two, at most 64.
 template <typename S>
 void dcgs2_update(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ldv, S* q, S* w, const S* coef,
                   double* t2, hipStream_t s) {
   constexpr bool cx = scalar_traits<S>::is_complex;
-  const bool two = cx ? p > 16 : (p > 32 && nrhs <= 32);
+  const bool two = cx ? p > 16 : p > 32;
   const int64_t nthr = (n * nrhs * (two ? 2 : 1) + kBasisNT - 1) / kBasisNT;
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(ws->part_blocks, nthr));
   const size_t lds = sizeof(S) * (size_t)(2 * p + 2) * nrhs + sizeof(double) * (size_t)(kBasisNT / kWave) * nrhs;
@@ -1449,10 +1451,8 @@ void dcgs2_update(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t 
     if (p <= 8) k_dcgs2_update<S, kBasisNT, 8, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
     else if (p <= 16) k_dcgs2_update<S, kBasisNT, 16, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
     else if (p <= 32) k_dcgs2_update<S, kBasisNT, 32, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-    else if (two && p <= 64) k_dcgs2_update<S, kBasisNT, 32, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-    else if (two) k_dcgs2_update<S, kBasisNT, 64, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-    else if (p <= 64) k_dcgs2_update<S, kBasisNT, 64, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
-    else throw std::runtime_error("delayed CGS2: more than 64 basis vectors need at most 32 columns");
+    else if (p <= 64) k_dcgs2_update<S, kBasisNT, 32, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
+    else k_dcgs2_update<S, kBasisNT, 64, 2><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
   }
   k_dot_final_wave<<<(nrhs + 3) / 4, 256, 0, s>>>(nblk, 1, nrhs, part, t2, 0);
   HIPCHK(hipGetLastError());
@@ -1472,12 +1472,10 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
   const int nb = nblocks_for(n * nrhs, kNT);
   const int check = o.check_every > 0 ? o.check_every : 1;
   const int maxits = o.maxits;
-  // delayed CGS2 on real bases (more than 64 basis vectors need two lanes per
-  // element, i.e. at most 32 columns); complex bases use CGS2
   // delayed CGS2 keeps a step's coefficients in LDS and its basis values in
-  // registers: real bases up to 64 vectors, or more with at most 32 columns;
-  // complex bases up to 64 vectors
-  const bool dcgs2 = o.orth == PGS_ORTH_DCGS2 && (scalar_traits<S>::is_complex ? m <= 64 : (m <= 64 || nrhs <= 32));
+  // registers: real bases up to the Krylov limit (two lanes per element past
+  // 32 vectors), complex bases up to 64 vectors (CGS2 beyond)
+  const bool dcgs2 = o.orth == PGS_ORTH_DCGS2 && (!scalar_traits<S>::is_complex || m <= 64);
   // ||b||, v0 = b - A x, ||v0||
   colnorm2<S>(ws, n, nrhs, b, ws->nb2.as<double>(), s);
   timed(ws, s, PGS_PROF_SPMV_A, spmv_bytes<S, S, S>(op->A, nrhs, true),

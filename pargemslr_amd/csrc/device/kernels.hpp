@@ -670,6 +670,8 @@ __global__ __launch_bounds__(NT) void k_basis_reg(int64_t n, int nrhs, int p, co
 // V_{j0 + u LPE + sub}, u < ceil(cnt / LPE) <= PM; every lane walks the
 // elements with a grid stride; reductions over lanes, then over waves in a
 // fixed order (one LDS table of nout * nrhs doubles, the waves add in turn).
+// With nrhs * LPE > 64 a wave covers only part of a row's columns, and each
+// of its lanes adds its own sums (no lane reduction).
 template <typename S, int NT, int PM, int LPE>
 __global__ __launch_bounds__(NT) void k_dcgs2_dots(int64_t n, int nrhs, int j0, int cnt, const S* __restrict__ V,
                                                    int64_t ldv, const S* __restrict__ q, const S* __restrict__ w,
@@ -680,6 +682,7 @@ __global__ __launch_bounds__(NT) void k_dcgs2_dots(int64_t n, int nrhs, int j0, 
   const int nout = (hw ? 2 * cnt : cnt) + (first ? (hw ? 2 : 1) : 0);
   const int tid = threadIdx.x;
   const int sub = tid % LPE;
+  const int cth = (tid / LPE) % nrhs;  // the lane's column (the grid stride keeps it)
   const int pu = (cnt + LPE - 1) / LPE;
   const S* Vl = V + (int64_t)(j0 + sub) * ldv;
   S aq[PM], aw[PM];
@@ -736,14 +739,14 @@ __global__ __launch_bounds__(NT) void k_dcgs2_dots(int64_t n, int nrhs, int j0, 
       for (int u = 0; u < PM; ++u) {
         const int j = u * LPE + sub;
         if (u < pu && j < cnt) {
-          dd_sm[j * nrhs + lane / LPE] += aq[u];
-          if (hw) dd_sm[(cnt + j) * nrhs + lane / LPE] += aw[u];
+          dd_sm[j * nrhs + cth] += aq[u];
+          if (hw) dd_sm[(cnt + j) * nrhs + cth] += aw[u];
         }
       }
       if (first && lane % LPE == 0) {
         const int o = hw ? 2 * cnt : cnt;
-        dd_sm[o * nrhs + lane / LPE] += aa;
-        if (hw) dd_sm[(o + 1) * nrhs + lane / LPE] += ab;
+        dd_sm[o * nrhs + cth] += aa;
+        if (hw) dd_sm[(o + 1) * nrhs + cth] += ab;
       }
     }
     __syncthreads();
@@ -767,6 +770,9 @@ __global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p,
   double* red = reinterpret_cast<double*>(cs + nc);
   const int tid = threadIdx.x;
   for (int t = tid; t < nc; t += NT) cs[t] = coef[t];
+  // a wave covers 64 / LPE elements: all columns when nrhs * LPE <= 64, else
+  // half of them, and the other half of its row of the table stays zero
+  for (int t = tid; t < (NT / kWave) * nrhs; t += NT) red[t] = 0.0;
   __syncthreads();
   const S* sc = cs;
   const S* gc = cs + p * nrhs;
@@ -807,7 +813,7 @@ __global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p,
   }
   const int wid = tid / kWave, lane = tid % kWave;
   for (int mm = kWave / 2; mm >= nrhs * LPE; mm >>= 1) acc += __shfl_xor(acc, mm);
-  if (lane < nrhs * LPE && lane % LPE == 0) red[wid * nrhs + lane / LPE] = acc;
+  if (lane < nrhs * LPE && lane % LPE == 0) red[wid * nrhs + c] = acc;
   __syncthreads();
   for (int t = tid; t < nrhs; t += NT) {
     double s2 = 0.0;

@@ -208,13 +208,13 @@ def test_apply_wide_rows(gpu):
 
 @pytest.mark.parametrize("nr,orth,kdim", [(4, pg.PGS_ORTH_CGS2, 50), (64, pg.PGS_ORTH_CGS2, 50),
                                           (4, pg.PGS_ORTH_DCGS2, 50), (64, pg.PGS_ORTH_DCGS2, 50),
-                                          (8, pg.PGS_ORTH_DCGS2, 80)])
+                                          (8, pg.PGS_ORTH_DCGS2, 80), (64, pg.PGS_ORTH_DCGS2, 80)])
 def test_fgmres_cgs2_long_basis(nr, orth, kdim, c1, gpu):
     """Unpreconditioned FGMRES(kdim) with CGS2 or delayed CGS2: the basis
     grows past 32 and 64 vectors' worth of register sweeps (two lanes per
-    element, 64 columns with one lane); every column's Givens residual after
-    60 (or kdim + 10) iterations equals the oracle's (GMRES residuals are
-    unique)."""
+    element; at 64 columns a wave then covers half a row's columns); every
+    column's Givens residual after 60 (or kdim + 10) iterations equals the
+    oracle's (GMRES residuals are unique)."""
     A = c1[0]
     D = pg.DevOp(c1[1])
     W = pg.Workspace(D, max_nrhs=nr, kdim=kdim)
