@@ -268,6 +268,7 @@ struct pgs_ws_s {
   // Krylov
   DevMem V, Z, part, hbuf, h2buf, t2, H, cs, sn, rs, coef, upd, st, zflag, nb2, nr2, flag;
   DevMem dc_d, dc_coef;  // delayed CGS2: the step's dot table and update coefficients
+  DevMem dc_vscale;      // delayed CGS2: pending normalization of the newest basis vector, per column
   // inner Schur iteration: basis, preconditioned basis, temporaries (vectors
   // in the apply's precision) and its Hessenberg / Givens state per column
   DevMem in_V, in_Z, in_T, in_U, in_Y, in_h, in_t2, in_H, in_cs, in_sn, in_rs, in_coef, in_upd, in_st, in_flag,
@@ -912,9 +913,10 @@ void solve_level(pgs_dev_op_s* op, pgs_ws_s* ws, DevPrec<V>& P, int l, const V* 
 }
 
 // z = M^{-1} v with v in VI (the operator's double or complex<double>), z in
-// VZ, apply arithmetic V.
+// VZ, apply arithmetic V; cscale (may be null) scales v's columns first.
 template <typename V, typename VI, typename VZ>
-void apply_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const VI* v, VZ* z, int nrhs, hipStream_t s) {
+void apply_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const VI* v, VZ* z, int nrhs, hipStream_t s,
+                const double* cscale = nullptr) {
   DevPrec<V>& P = prec_of<V>(op);
   if (!P.present) throw std::runtime_error("operator was uploaded without this precision");
   const int64_t n = op->n;
@@ -923,8 +925,8 @@ void apply_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const VI* v, VZ* z, int nrhs, hi
   auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
   const bool wide = nrhs % 4 == 0 && al16(v) && al16(z);  // workspace buffers are allocation-aligned
   timed(ws, s, PGS_PROF_PERM, (double)n * nrhs * (sizeof(VI) + sizeof(V)) + 4.0 * n, [&] {
-    if (wide) k_gather4<VI, V><<<nblocks_for(n * nrhs / 4, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
-    else k_gather<VI, V><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0);
+    if (wide) k_gather4<VI, V><<<nblocks_for(n * nrhs / 4, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0, cscale);
+    else k_gather<VI, V><<<nblocks_for(n * nrhs, kNT), kNT, 0, s>>>(n, nrhs, op->perm.as<int>(), v, r0, cscale);
   });
   HIPCHK(hipGetLastError());
   solve_level<V>(op, ws, P, 0, r0, x0, nrhs, s);
@@ -1066,6 +1068,7 @@ WsSizes ws_sizes(const pgs_dev_op_s* op, int nr, int kdim, bool lowrank_build = 
   add(&pgs_ws_s::upd, (size_t)nr * sizeof(int));
   add(&pgs_ws_s::dc_d, (2 * ik + 4) * nr * es);
   add(&pgs_ws_s::dc_coef, (2 * ik + 4) * nr * es);
+  add(&pgs_ws_s::dc_vscale, (size_t)nr * sizeof(double));
   add(&pgs_ws_s::st, (size_t)nr * sizeof(KState));
   add(&pgs_ws_s::zflag, (size_t)nr * sizeof(double));
   add(&pgs_ws_s::nb2, (size_t)nr * sizeof(double));
@@ -1503,11 +1506,16 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
     S* vprev = V + (int64_t)(i - 1) * ldv;
     S* w = V + (int64_t)i * ldv;
     VZ* zc = Z + (int64_t)(i - 1) * ldv;
+    // delayed CGS2 leaves each new basis vector unnormalized: its readers
+    // (this apply's gather, the next sweep's dot table) apply the scale
+    const double* vsc = dcgs2 ? ws->dc_vscale.as<double>() : nullptr;
     if (o.use_precond) {
       if (o.prec == PGS_PREC_FP64)
-        apply_impl<typename PrecTypes<S>::P64, S, VZ>(op, ws, vprev, zc, nrhs, ss);
+        apply_impl<typename PrecTypes<S>::P64, S, VZ>(op, ws, vprev, zc, nrhs, ss, vsc);
       else
-        apply_impl<typename PrecTypes<S>::P32, S, VZ>(op, ws, vprev, zc, nrhs, ss);
+        apply_impl<typename PrecTypes<S>::P32, S, VZ>(op, ws, vprev, zc, nrhs, ss, vsc);
+    } else if (vsc) {
+      k_copy_scale_cols<S, VZ><<<nb, kNT, 0, ss>>>(n, nrhs, vsc, vprev, zc);
     } else {
       k_copy_conv<S, VZ><<<nb, kNT, 0, ss>>>(n * nrhs, vprev, zc);
     }
@@ -1525,12 +1533,13 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
       const double vb = (double)n * nrhs * sizeof(S);
       timed(ws, ss, PGS_PROF_ORTH_DOTS, (p + 2.0) * vb, [&] { dcgs2_dots<S>(ws, n, nrhs, p, V, ldv, w, d, ss); });
       timed(ws, ss, PGS_PROF_STATE, 0.0, [&] {
-        k_dcgs2_fix<S><<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef);
+        k_dcgs2_fix<S><<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef,
+                                         vsc);
       });
       timed(ws, ss, PGS_PROF_ORTH_UPDATE, (p + 4.0) * vb,
             [&] { dcgs2_update<S>(ws, n, nrhs, p, V, ldv, vprev, w, coef, t2, ss); });
-      timed(ws, ss, PGS_PROF_STATE, 0.0, [&] { k_dcgs2_norm<S><<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st); });
-      timed(ws, ss, PGS_PROF_SCALE, 2.0 * vb, [&] { k_scale_cols<S><<<nb, kNT, 0, ss>>>(n, nrhs, st, w); });
+      timed(ws, ss, PGS_PROF_STATE, 0.0,
+            [&] { k_dcgs2_norm<S><<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st, ws->dc_vscale.as<double>()); });
       HIPCHK(hipGetLastError());
       return;
     }
@@ -1596,7 +1605,7 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
   int iters_host = 0;
   while (true) {
     if (read_flag(true) == 0) break;
-    k_cycle_start<S><<<1, 64, 0, s>>>(nrhs, m, st, rs);
+    k_cycle_start<S><<<1, 64, 0, s>>>(nrhs, m, st, rs, dcgs2 ? ws->dc_vscale.as<double>() : nullptr);
     k_scale_cols<S><<<nb, kNT, 0, s>>>(n, nrhs, st, V);
     int i = 0;
     bool none_active = false;
@@ -1643,7 +1652,7 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
             [&] { dcgs2_dots<S>(ws, n, nrhs, i, V, ldv, nullptr, d, s); });
       timed(ws, s, PGS_PROF_STATE, 0.0, [&] {
         k_dcgs2_fix<S><<<1, 64, 0, s>>>(nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st,
-                                        ws->dc_coef.as<S>());
+                                        ws->dc_coef.as<S>(), ws->dc_vscale.as<double>());
       });
       HIPCHK(hipGetLastError());
     }

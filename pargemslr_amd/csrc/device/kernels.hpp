@@ -53,13 +53,18 @@ __device__ __forceinline__ int col_of(int64_t t, int nrhs) {
 // permutations (IntVectorClass::GatherPerm / ScatterRperm,
 // ParGeMSLR/SRC/vectors/int_vector.cpp:629-696) fused with a precision change
 // ---------------------------------------------------------------------------
+// cscale (may be null): a per-column factor applied to the input first (the
+// pending normalization of a Krylov vector)
 template <typename VI, typename VO>
-__global__ void k_gather(int64_t n, int nrhs, const int* __restrict__ perm, const VI* __restrict__ in, VO* __restrict__ out) {
+__global__ void k_gather(int64_t n, int nrhs, const int* __restrict__ perm, const VI* __restrict__ in, VO* __restrict__ out,
+                         const double* __restrict__ cscale) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * nrhs) return;
   const int64_t i = row_of(t, nrhs);
   const int c = col_of(t, nrhs);
-  out[t] = cvt<VO>(in[(int64_t)perm[i] * nrhs + c]);
+  VI a = in[(int64_t)perm[i] * nrhs + c];
+  if (cscale) a *= VI(cscale[c]);
+  out[t] = cvt<VO>(a);
 }
 
 // Four consecutive columns of a row per thread (nrhs a multiple of 4, both
@@ -72,13 +77,17 @@ struct alignas(W * sizeof(V) < 16 ? W * sizeof(V) : 16) ColPack {
 
 template <typename VI, typename VO>
 __global__ void k_gather4(int64_t n, int nrhs, const int* __restrict__ perm, const VI* __restrict__ in,
-                          VO* __restrict__ out) {
+                          VO* __restrict__ out, const double* __restrict__ cscale) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t e = 4 * t;
   if (e >= n * nrhs) return;
   const int64_t i = row_of(e, nrhs);
   const int c = col_of(e, nrhs);
-  const ColPack<VI, 4> a = *reinterpret_cast<const ColPack<VI, 4>*>(in + (int64_t)perm[i] * nrhs + c);
+  ColPack<VI, 4> a = *reinterpret_cast<const ColPack<VI, 4>*>(in + (int64_t)perm[i] * nrhs + c);
+  if (cscale) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a.v[u] *= VI(cscale[c + u]);
+  }
   ColPack<VO, 4> b;
 #pragma unroll
   for (int u = 0; u < 4; ++u) b.v[u] = cvt<VO>(a.v[u]);
@@ -844,10 +853,14 @@ __global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p,
 template <typename S>
 __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, const S* __restrict__ d,
                             S* __restrict__ H, S* __restrict__ cs, S* __restrict__ sn, S* __restrict__ rs,
-                            KState* __restrict__ st, S* __restrict__ coef) {
+                            KState* __restrict__ st, S* __restrict__ coef, const double* __restrict__ vscale) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
   KState s = st[c];
+  // q = basis slot p may still wait for its normalization by sg (vscale):
+  // the dots of this sweep saw sg^-1 q, so s = sg s_raw, q.q = sg^2 (q.q)_raw,
+  // q.w = sg (q.w)_raw; the update takes s_raw, sg / nu and sg gamma
+  const double sg = vscale ? vscale[c] : 1.0;
   S* sco = coef;
   S* gco = coef + (int64_t)p * nrhs;
   S* inu = coef + (int64_t)2 * p * nrhs;
@@ -867,8 +880,8 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
   }
   const double eps = 2.220446049250313e-16;
   // d rows: s_j (j < p), then a_j (j < p), then q.q, q.w (final: s_j, q.q)
-  const double qq = (double)sreal(final_pass ? d[p * nrhs + c] : d[2 * p * nrhs + c]);
-  const S qw = final_pass ? S(0.0) : d[(2 * p + 1) * nrhs + c];
+  const double qq = sg * sg * (double)sreal(final_pass ? d[p * nrhs + c] : d[2 * p * nrhs + c]);
+  const S qw = final_pass ? S(0.0) : S(sg) * d[(2 * p + 1) * nrhs + c];
   double nu = 1.0;
   S* Hc = H + (int64_t)c * (m + 1) * m;
   auto retire = [&](int status) {
@@ -891,7 +904,7 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
   bool dependent = false;
   if (p > 0) {
     double ss = 0.0;
-    for (int j = 0; j < p; ++j) ss += (double)sabs2(d[j * nrhs + c]);
+    for (int j = 0; j < p; ++j) ss += (double)sabs2(S(sg) * d[j * nrhs + c]);
     if (!isfinite(ss)) {
       retire(5);
       return;
@@ -900,7 +913,7 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
     nu = sqrt(fmax(qq - ss, 0.0));
     const int col = p - 1;
     const S hq = Hc[p + col * (m + 1)];
-    for (int j = 0; j < p; ++j) Hc[j + col * (m + 1)] += hq * d[j * nrhs + c];
+    for (int j = 0; j < p; ++j) Hc[j + col * (m + 1)] += hq * (S(sg) * d[j * nrhs + c]);
     Hc[p + col * (m + 1)] = hq * S(nu);
     // Givens update of column col, as k_givens (fgmres.hpp:428-461)
     S* csc = cs + (int64_t)c * m;
@@ -947,18 +960,19 @@ __global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, 
   const double in = nu > 0.0 ? 1.0 / nu : 0.0;
   // coefficient of w on the final v_p: (q - V s)^H w / nu = (q.w - s^H a) / nu
   S sa = S(0.0);
-  for (int j = 0; j < p; ++j) sa += sconj(d[j * nrhs + c]) * d[(p + j) * nrhs + c];
+  for (int j = 0; j < p; ++j) sa += sconj(S(sg) * d[j * nrhs + c]) * d[(p + j) * nrhs + c];
   const S cp = (qw - sa) * S(in);
   const S gm = cp * S(in);
   for (int j = 0; j < p; ++j) {
-    const S sj = d[j * nrhs + c], aj = d[(p + j) * nrhs + c];
+    const S sr = d[j * nrhs + c], aj = d[(p + j) * nrhs + c];
+    const S sj = S(sg) * sr;
     Hc[j + p * (m + 1)] = aj;
-    sco[j * nrhs + c] = sj;
+    sco[j * nrhs + c] = sr;
     gco[j * nrhs + c] = aj - gm * sj;
   }
   Hc[p + p * (m + 1)] = cp;
-  inu[c] = S(in);
-  gam[c] = gm;
+  inu[c] = S(sg * in);
+  gam[c] = gm * S(sg);
 }
 
 // Sums the nblk partials of one k_dcgs2_dots launch into the rows of the
@@ -988,7 +1002,7 @@ __global__ __launch_bounds__(256) void k_dcgs2_final(int nblk, int nout, int cnt
 // scale that normalizes w (zero for columns that are not active).
 template <typename S>
 __global__ void k_dcgs2_norm(int nrhs, int m, int p, const double* __restrict__ t2, S* __restrict__ H,
-                             KState* __restrict__ st) {
+                             KState* __restrict__ st, double* __restrict__ vscale) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
   KState s = st[c];
@@ -1004,6 +1018,7 @@ __global__ void k_dcgs2_norm(int nrhs, int m, int p, const double* __restrict__ 
     s.scale = t > 0.0 ? 1.0 / t : 0.0;
   }
   st[c] = s;
+  if (vscale) vscale[c] = s.scale;  // w stays as it is; its readers apply the scale
 }
 
 // v[r, c] *= s[c]   (s from KState.scale)
@@ -1053,9 +1068,11 @@ __global__ void k_zero_cols(int64_t n, int nrhs, const double* __restrict__ flag
 
 // Start of a restart cycle (fgmres.hpp:382-386): rs[0] = normr, v0 /= normr.
 template <typename S>
-__global__ void k_cycle_start(int nrhs, int m, KState* __restrict__ st, S* __restrict__ rs) {
+__global__ void k_cycle_start(int nrhs, int m, KState* __restrict__ st, S* __restrict__ rs,
+                              double* __restrict__ vscale = nullptr) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nrhs) return;
+  if (vscale) vscale[c] = 1.0;  // v0 is normalized in place
   KState s = st[c];
   if (s.status == 0) {
     s.active = 1;
@@ -1204,6 +1221,15 @@ template <typename S, typename VZ>
 __global__ void k_copy_conv(int64_t m, const S* __restrict__ in, VZ* __restrict__ out) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m) out[t] = (VZ)in[t];
+}
+
+// z[r, c] = v[r, c] * cscale[c] : identity preconditioner on a Krylov vector
+// whose normalization is pending
+template <typename S, typename VZ>
+__global__ void k_copy_scale_cols(int64_t n, int nrhs, const double* __restrict__ cscale, const S* __restrict__ in,
+                                  VZ* __restrict__ out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n * nrhs) out[t] = (VZ)(in[t] * S(cscale[col_of(t, nrhs)]));
 }
 
 // ---------------------------------------------------------------------------
