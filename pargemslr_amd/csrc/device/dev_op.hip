@@ -268,6 +268,7 @@ struct pgs_ws_s {
   // Krylov
   DevMem V, Z, part, hbuf, h2buf, t2, H, cs, sn, rs, coef, upd, st, zflag, nb2, nr2, flag;
   DevMem dc_d, dc_coef;  // delayed CGS2: the step's dot table and update coefficients
+  DevMem dc_R;           // delayed CGS2: final basis in terms of the raw slots, (kdim+1)^2 per column
   DevMem dc_vscale;      // delayed CGS2: pending normalization of the newest basis vector, per column
   // inner Schur iteration: basis, preconditioned basis, temporaries (vectors
   // in the apply's precision) and its Hessenberg / Givens state per column
@@ -1068,6 +1069,7 @@ WsSizes ws_sizes(const pgs_dev_op_s* op, int nr, int kdim, bool lowrank_build = 
   add(&pgs_ws_s::upd, (size_t)nr * sizeof(int));
   add(&pgs_ws_s::dc_d, (2 * ik + 4) * nr * es);
   add(&pgs_ws_s::dc_coef, (2 * ik + 4) * nr * es);
+  add(&pgs_ws_s::dc_R, (ik + 1) * (ik + 1) * nr * es);
   add(&pgs_ws_s::dc_vscale, (size_t)nr * sizeof(double));
   add(&pgs_ws_s::st, (size_t)nr * sizeof(KState));
   add(&pgs_ws_s::zflag, (size_t)nr * sizeof(double));
@@ -1431,18 +1433,20 @@ void dcgs2_dots(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ld
   HIPCHK(hipGetLastError());
 }
 
-// Delayed CGS2, update of step p (k_dcgs2_update), then t2[c] = ||w(:, c)||^2.
+// Delayed CGS2, update of step p (k_dcgs2_update: w minus its projection on
+// the final basis v_0..v_p, in terms of the raw slots 0..p), then
+// t2[c] = ||w(:, c)||^2.
 // Real values: one lane per element up to 32 vectors, then two (the one-lane
 // 64-vector kernel needs more than 256 VGPRs); complex values (twice the
 // registers per vector): one lane up to 16 vectors, then two, at most 64.
 template <typename S>
-void dcgs2_update(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ldv, S* q, S* w, const S* coef,
-                  double* t2, hipStream_t s) {
+void dcgs2_update(pgs_ws_s* ws, int64_t n, int nrhs, int p, const S* V, int64_t ldv, const S* q, S* w,
+                  const S* coef, double* t2, hipStream_t s) {
   constexpr bool cx = scalar_traits<S>::is_complex;
   const bool two = cx ? p > 16 : p > 32;
   const int64_t nthr = (n * nrhs * (two ? 2 : 1) + kBasisNT - 1) / kBasisNT;
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(ws->part_blocks, nthr));
-  const size_t lds = sizeof(S) * (size_t)(2 * p + 2) * nrhs + sizeof(double) * (size_t)(kBasisNT / kWave) * nrhs;
+  const size_t lds = sizeof(S) * (size_t)(p + 1) * nrhs + sizeof(double) * (size_t)(kBasisNT / kWave) * nrhs;
   double* part = ws->part.as<double>();
   if constexpr (cx) {
     if (p <= 8) k_dcgs2_update<S, kBasisNT, 8, 1><<<nblk, kBasisNT, lds, s>>>(n, nrhs, p, V, ldv, q, w, coef, part);
@@ -1526,17 +1530,19 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
     double* t2 = ws->t2.as<double>();
     if (dcgs2) {
       // delayed CGS2: finalize v_{i-1} (and Hessenberg column i-2, with its
-      // Givens update) from the dots of this sweep, then project w once
+      // Givens update) from the dots of this sweep, then project w once.
+      // The slots keep the raw vectors; the final basis is carried by R
+      // (k_dcgs2_fix), so the update writes w alone
       const int p = i - 1;
       S* d = ws->dc_d.as<S>();
       S* coef = ws->dc_coef.as<S>();
       const double vb = (double)n * nrhs * sizeof(S);
       timed(ws, ss, PGS_PROF_ORTH_DOTS, (p + 2.0) * vb, [&] { dcgs2_dots<S>(ws, n, nrhs, p, V, ldv, w, d, ss); });
       timed(ws, ss, PGS_PROF_STATE, 0.0, [&] {
-        k_dcgs2_fix<S><<<1, 64, 0, ss>>>(nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef,
-                                         vsc);
+        k_dcgs2_fix<S><<<nrhs, kWave, sizeof(S) * 2 * std::max(p, 1), ss>>>(
+            nrhs, m, p, maxits, 0, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, coef, vsc, ws->dc_R.as<S>());
       });
-      timed(ws, ss, PGS_PROF_ORTH_UPDATE, (p + 4.0) * vb,
+      timed(ws, ss, PGS_PROF_ORTH_UPDATE, (p + 3.0) * vb,
             [&] { dcgs2_update<S>(ws, n, nrhs, p, V, ldv, vprev, w, coef, t2, ss); });
       timed(ws, ss, PGS_PROF_STATE, 0.0,
             [&] { k_dcgs2_norm<S><<<1, 64, 0, ss>>>(nrhs, m, p, t2, H, st, ws->dc_vscale.as<double>()); });
@@ -1651,8 +1657,9 @@ int fgmres_impl(pgs_dev_op_s* op, pgs_ws_s* ws, const S* b, S* x, int nrhs, cons
       timed(ws, s, PGS_PROF_ORTH_DOTS, (i + 1.0) * (double)n * nrhs * sizeof(S),
             [&] { dcgs2_dots<S>(ws, n, nrhs, i, V, ldv, nullptr, d, s); });
       timed(ws, s, PGS_PROF_STATE, 0.0, [&] {
-        k_dcgs2_fix<S><<<1, 64, 0, s>>>(nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st,
-                                        ws->dc_coef.as<S>(), ws->dc_vscale.as<double>());
+        k_dcgs2_fix<S><<<nrhs, kWave, sizeof(S) * 2 * std::max(i, 1), s>>>(
+            nrhs, m, i, maxits, 1, d, H, ws->cs.as<S>(), ws->sn.as<S>(), rs, st, ws->dc_coef.as<S>(),
+            ws->dc_vscale.as<double>(), ws->dc_R.as<S>());
       });
       HIPCHK(hipGetLastError());
     }

@@ -774,18 +774,18 @@ __global__ __launch_bounds__(NT) void k_dcgs2_dots(int64_t n, int nrhs, int j0, 
   for (int t = tid; t < nout * nrhs; t += NT) part[(int64_t)blockIdx.x * nout * nrhs + t] = dd_sm[t];
 }
 
-// q <- (q - sum_{j<p} V_j s_j) inv_nu (the delayed second projection and
-// normalization of q) and w <- w - sum_{j<p} V_j g_j - gamma q_old (the first
-// projection of w on the final basis and q), with per-workgroup partials of
-// |w|^2 in part[blk * nrhs + c].  coef = [s (p x nrhs) | g (p x nrhs) |
-// inv_nu (nrhs) | gamma (nrhs)], copied to LDS.  Element layout as
-// k_basis_reg (LPE lanes per element, lane sub holds V_{u LPE + sub}).
+// w <- w - sum_{j<p} V_j e_j - e_p q: the projection of w on the final
+// basis v_0..v_p, written in terms of the raw basis slots 0..p (q = slot p,
+// which stays as it is), with per-workgroup partials of |w|^2 in
+// part[blk * nrhs + c].  coef = [e_0..e_{p-1} (p x nrhs) | e_p (nrhs)],
+// copied to LDS.  Element layout as k_basis_reg (LPE lanes per element, lane
+// sub holds V_{u LPE + sub}).
 template <typename S, int NT, int PM, int LPE>
 __global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p, const S* __restrict__ V,
-                                                     int64_t ldv, S* __restrict__ q, S* __restrict__ w,
+                                                     int64_t ldv, const S* __restrict__ q, S* __restrict__ w,
                                                      const S* __restrict__ coef, double* __restrict__ part) {
   extern __shared__ unsigned char du_raw[];
-  const int nc = (2 * p + 2) * nrhs;
+  const int nc = (p + 1) * nrhs;
   S* cs = reinterpret_cast<S*>(du_raw);
   double* red = reinterpret_cast<double*>(cs + nc);
   const int tid = threadIdx.x;
@@ -794,10 +794,8 @@ __global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p,
   // half of them, and the other half of its row of the table stays zero
   for (int t = tid; t < (NT / kWave) * nrhs; t += NT) red[t] = 0.0;
   __syncthreads();
-  const S* sc = cs;
-  const S* gc = cs + p * nrhs;
-  const S* inu = cs + 2 * p * nrhs;
-  const S* gam = inu + nrhs;
+  const S* gc = cs;
+  const S* gam = cs + p * nrhs;
   const int sub = tid % LPE;
   const int c = (tid / LPE) % nrhs;
   const int pu = (p + LPE - 1) / LPE;
@@ -810,23 +808,16 @@ __global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p,
 #pragma unroll
     for (int u = 0; u < PM; ++u)
       if (u < pu) v[u] = (u * LPE + sub < p) ? ld_stream(Vl + (int64_t)(u * LPE) * ldv + g) : S(0.0);
-    S sv = S(0.0), gv = S(0.0);
+    S gv = S(0.0);
 #pragma unroll
     for (int u = 0; u < PM; ++u)
       if (u < pu) {
         const int j = u * LPE + sub;
-        const bool in = j < p;
-        sv += v[u] * (in ? sc[j * nrhs + c] : S(0.0));
-        gv += v[u] * (in ? gc[j * nrhs + c] : S(0.0));
+        gv += v[u] * (j < p ? gc[j * nrhs + c] : S(0.0));
       }
-    if constexpr (LPE == 2) {
-      sv += shfl_xor_v(sv, 1);
-      gv += shfl_xor_v(gv, 1);
-    }
+    if constexpr (LPE == 2) gv += shfl_xor_v(gv, 1);
     if (sub == 0) {
-      const S qo = q[g];
-      const S un = w[g] - gv - gam[c] * qo;
-      q[g] = (qo - sv) * inu[c];
+      const S un = w[g] - gv - gam[c] * q[g];
       w[g] = un;
       acc += (double)sabs2(un);
     }
@@ -842,137 +833,173 @@ __global__ __launch_bounds__(NT) void k_dcgs2_update(int64_t n, int nrhs, int p,
   }
 }
 
-// Per column, after the dots of step p (q = basis slot p): finalize q's norm
-// nu = sqrt(q.q - |s|^2), correct the tentative Hessenberg column p-1
-// (H[0..p-1, p-1] += H[p, p-1] s; H[p, p-1] *= nu), then the Givens update of
-// column p-1 as k_givens (convergence test, cycle end); then (not final) the
-// coefficients of w on the final basis: H[0..p-1, p] = a, H[p, p] = c_p =
-// (q.w - s.a) / nu, and coef = [s | g = a - gamma s | 1/nu | gamma = c_p / nu]
-// for k_dcgs2_update.  p = 0: q is the exact v0 (nu = 1, no column to finalize).
-// Columns that are not active get zero coefficients.
+// Per column (one workgroup of 64 lanes, blockIdx.x = column), after the
+// dots of step p.  The basis slots hold raw vectors: slot j < p is step j's
+// first projection as k_dcgs2_update left it, and the final (orthonormal)
+// basis is v_j = sum_{i<=j} slot_i R[i, j], R upper triangular (row-major,
+// (m+1) x (m+1) per column); slot p = q still waits for its scale sg
+// (vscale).  From the raw dots d = [slot_j^H q (p) | slot_j^H w (p) | q.q |
+// q.w] (final pass: [slot_j^H q | q.q]):
+//   s_j = v_j^H (sg q) = sg sum_{i<=j} conj(R[i, j]) d_i,  a_j = v_j^H w alike;
+// then q's norm nu = sqrt(sg^2 q.q - |s|^2), the correction of the tentative
+// Hessenberg column p-1 (H[0..p-1, p-1] += H[p, p-1] s; H[p, p-1] *= nu) and
+// its Givens update as k_givens (convergence test, cycle end); then (not
+// final) H[0..p-1, p] = a, H[p, p] = c_p = (sg q.w - s^H a) / nu.  The final
+// v_p = (sg q - sum_j v_j s_j) / nu is R's column p: R[i, p] = -(R s)_i / nu,
+// R[p, p] = sg / nu; and w - sum_{j<p} v_j a_j - c_p v_p = w - sum_{i<=p}
+// slot_i e_i with e_i = (R (a - gm s))_i for i < p, e_p = sg gm, gm = c_p / nu:
+// coef = [e_0..e_p] (rows of nrhs) for k_dcgs2_update, zero for columns that
+// are not active.  p = 0: q is the exact v0 (nu = 1, no column to finalize).
+// Dynamic LDS: 2 p values of S.
 template <typename S>
-__global__ void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass, const S* __restrict__ d,
-                            S* __restrict__ H, S* __restrict__ cs, S* __restrict__ sn, S* __restrict__ rs,
-                            KState* __restrict__ st, S* __restrict__ coef, const double* __restrict__ vscale) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nrhs) return;
-  KState s = st[c];
-  // q = basis slot p may still wait for its normalization by sg (vscale):
-  // the dots of this sweep saw sg^-1 q, so s = sg s_raw, q.q = sg^2 (q.q)_raw,
-  // q.w = sg (q.w)_raw; the update takes s_raw, sg / nu and sg gamma
+__global__ __launch_bounds__(64) void k_dcgs2_fix(int nrhs, int m, int p, int maxits, int final_pass,
+                                                  const S* __restrict__ d, S* __restrict__ H, S* __restrict__ cs,
+                                                  S* __restrict__ sn, S* __restrict__ rs, KState* __restrict__ st,
+                                                  S* __restrict__ coef, const double* __restrict__ vscale,
+                                                  S* __restrict__ R) {
+  extern __shared__ unsigned char fx_raw[];
+  S* sv = reinterpret_cast<S*>(fx_raw);  // s_j
+  S* av = sv + p;                        // a_j (not final)
+  __shared__ int go_sh;
+  __shared__ double in_sh;
+  __shared__ S gm_sh;
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t ldr = m + 1;
+  S* Rc = R + (int64_t)c * ldr * ldr;
   const double sg = vscale ? vscale[c] : 1.0;
-  S* sco = coef;
-  S* gco = coef + (int64_t)p * nrhs;
-  S* inu = coef + (int64_t)2 * p * nrhs;
-  S* gam = inu + nrhs;
-  auto zero = [&]() {
-    if (final_pass) return;
-    for (int j = 0; j < p; ++j) {
-      sco[j * nrhs + c] = S(0.0);
-      gco[j * nrhs + c] = S(0.0);
-    }
-    inu[c] = S(0.0);
-    gam[c] = S(0.0);
-  };
-  if (!s.active) {
-    zero();
-    return;
-  }
-  const double eps = 2.220446049250313e-16;
-  // d rows: s_j (j < p), then a_j (j < p), then q.q, q.w (final: s_j, q.q)
-  const double qq = sg * sg * (double)sreal(final_pass ? d[p * nrhs + c] : d[2 * p * nrhs + c]);
-  const S qw = final_pass ? S(0.0) : S(sg) * d[(2 * p + 1) * nrhs + c];
-  double nu = 1.0;
-  S* Hc = H + (int64_t)c * (m + 1) * m;
-  auto retire = [&](int status) {
-    s.status = status;
-    s.active = 0;
-    s.scale = 0.0;
-    st[c] = s;
-    zero();
-  };
-  if (!isfinite(qq) || !isfinite(sabs_d(qw))) {
-    retire(5);
-    return;
-  }
-  // the tentative v_p lies in the span of v_0..v_{p-1} to working precision
-  // when q.q - |s|^2 cancels (|s| = 1 - 5e-11 and closer): nu and the final
-  // v_p = (q - V s) / nu are then rounding noise, so the column's cycle ends
-  // with Hessenberg column p - 1 (whose subdiagonal nu * h is negligible
-  // there, a lucky breakdown) and restarts from its true residual instead of
-  // continuing on a basis vector that is not orthogonal
-  bool dependent = false;
-  if (p > 0) {
-    double ss = 0.0;
-    for (int j = 0; j < p; ++j) ss += (double)sabs2(S(sg) * d[j * nrhs + c]);
-    if (!isfinite(ss)) {
-      retire(5);
-      return;
-    }
-    dependent = !(qq - ss > kDcgs2DependentTol * qq);
-    nu = sqrt(fmax(qq - ss, 0.0));
-    const int col = p - 1;
-    const S hq = Hc[p + col * (m + 1)];
-    for (int j = 0; j < p; ++j) Hc[j + col * (m + 1)] += hq * (S(sg) * d[j * nrhs + c]);
-    Hc[p + col * (m + 1)] = hq * S(nu);
-    // Givens update of column col, as k_givens (fgmres.hpp:428-461)
-    S* csc = cs + (int64_t)c * m;
-    S* snc = sn + (int64_t)c * m;
-    S* rsc = rs + (int64_t)c * (m + 1);
-    s.iters += 1;
-    s.icur = p;
-    for (int j = 1; j < p; ++j) {
-      const S hii = Hc[(j - 1) + col * (m + 1)];
-      const S hj = Hc[j + col * (m + 1)];
-      Hc[(j - 1) + col * (m + 1)] = sconj(csc[j - 1]) * hii + snc[j - 1] * hj;
-      Hc[j + col * (m + 1)] = -snc[j - 1] * hii + csc[j - 1] * hj;
-    }
-    const S hii = Hc[(p - 1) + col * (m + 1)];
-    const S hii1 = Hc[p + col * (m + 1)];
-    const double g = sqrt((double)sabs2(hii) + (double)sabs2(hii1));
-    if (!isfinite(g)) {
-      retire(5);
-      return;
-    }
-    if (fabs(g) < eps) {  // breakdown: the reference leaves x untouched (fgmres.hpp:440-443)
-      retire(2);
-      return;
-    }
-    csc[p - 1] = hii / S(g);
-    snc[p - 1] = hii1 / S(g);
-    rsc[p] = -snc[p - 1] * rsc[p - 1];
-    rsc[p - 1] = sconj(csc[p - 1]) * rsc[p - 1];
-    Hc[(p - 1) + col * (m + 1)] = sconj(csc[p - 1]) * hii + snc[p - 1] * hii1;
-    s.normr = sabs_d(rsc[p]);
-    if (s.normr <= s.tolr) {
-      s.status = 1;
-      s.active = 0;
-    } else if (p >= m || s.iters >= maxits || dependent) {
-      s.active = 0;  // cycle ends for this column
-    }
-    st[c] = s;
-    if (!s.active) {
-      zero();
-      return;
+  const bool was_active = st[c].active != 0;
+  // s and a on the final basis (lanes over j, rows of R read along j)
+  if (was_active) {
+    for (int j = lane; j < p; j += kWave) {
+      S ts = S(0.0), ta = S(0.0);
+      for (int i = 0; i <= j; ++i) {
+        const S r = sconj(Rc[i * ldr + j]);
+        ts += r * d[i * nrhs + c];
+        if (!final_pass) ta += r * d[(p + i) * nrhs + c];
+      }
+      sv[j] = S(sg) * ts;
+      if (!final_pass) av[j] = ta;
     }
   }
+  __syncthreads();
+  if (lane == 0) {
+    // the column's scalar part; returns whether w gets coefficients
+    auto scalar_part = [&]() -> bool {
+      KState s = st[c];
+      if (!s.active) return false;
+      const double eps = 2.220446049250313e-16;
+      // d rows: s_j (j < p), then a_j (j < p), then q.q, q.w (final: s_j, q.q)
+      const double qq = sg * sg * (double)sreal(final_pass ? d[p * nrhs + c] : d[2 * p * nrhs + c]);
+      const S qw = final_pass ? S(0.0) : S(sg) * d[(2 * p + 1) * nrhs + c];
+      double nu = 1.0;
+      S* Hc = H + (int64_t)c * (m + 1) * m;
+      auto retire = [&](int status) {
+        s.status = status;
+        s.active = 0;
+        s.scale = 0.0;
+        st[c] = s;
+      };
+      if (!isfinite(qq) || !isfinite(sabs_d(qw))) {
+        retire(5);
+        return false;
+      }
+      // the tentative v_p lies in the span of v_0..v_{p-1} to working
+      // precision when q.q - |s|^2 cancels (|s| = 1 - 5e-11 and closer): nu
+      // and the final v_p = (q - V s) / nu are then rounding noise, so the
+      // column's cycle ends with Hessenberg column p - 1 (whose subdiagonal
+      // nu * h is negligible there, a lucky breakdown) and restarts from its
+      // true residual instead of continuing on a basis vector that is not
+      // orthogonal
+      bool dependent = false;
+      if (p > 0) {
+        double ss = 0.0;
+        for (int j = 0; j < p; ++j) ss += (double)sabs2(sv[j]);
+        if (!isfinite(ss)) {
+          retire(5);
+          return false;
+        }
+        dependent = !(qq - ss > kDcgs2DependentTol * qq);
+        nu = sqrt(fmax(qq - ss, 0.0));
+        const int col = p - 1;
+        const S hq = Hc[p + col * (m + 1)];
+        for (int j = 0; j < p; ++j) Hc[j + col * (m + 1)] += hq * sv[j];
+        Hc[p + col * (m + 1)] = hq * S(nu);
+        // Givens update of column col, as k_givens (fgmres.hpp:428-461)
+        S* csc = cs + (int64_t)c * m;
+        S* snc = sn + (int64_t)c * m;
+        S* rsc = rs + (int64_t)c * (m + 1);
+        s.iters += 1;
+        s.icur = p;
+        for (int j = 1; j < p; ++j) {
+          const S hii = Hc[(j - 1) + col * (m + 1)];
+          const S hj = Hc[j + col * (m + 1)];
+          Hc[(j - 1) + col * (m + 1)] = sconj(csc[j - 1]) * hii + snc[j - 1] * hj;
+          Hc[j + col * (m + 1)] = -snc[j - 1] * hii + csc[j - 1] * hj;
+        }
+        const S hii = Hc[(p - 1) + col * (m + 1)];
+        const S hii1 = Hc[p + col * (m + 1)];
+        const double g = sqrt((double)sabs2(hii) + (double)sabs2(hii1));
+        if (!isfinite(g)) {
+          retire(5);
+          return false;
+        }
+        if (fabs(g) < eps) {  // breakdown: the reference leaves x untouched (fgmres.hpp:440-443)
+          retire(2);
+          return false;
+        }
+        csc[p - 1] = hii / S(g);
+        snc[p - 1] = hii1 / S(g);
+        rsc[p] = -snc[p - 1] * rsc[p - 1];
+        rsc[p - 1] = sconj(csc[p - 1]) * rsc[p - 1];
+        Hc[(p - 1) + col * (m + 1)] = sconj(csc[p - 1]) * hii + snc[p - 1] * hii1;
+        s.normr = sabs_d(rsc[p]);
+        if (s.normr <= s.tolr) {
+          s.status = 1;
+          s.active = 0;
+        } else if (p >= m || s.iters >= maxits || dependent) {
+          s.active = 0;  // cycle ends for this column
+        }
+        st[c] = s;
+        if (!s.active) return false;
+      }
+      if (final_pass) return false;
+      const double in = nu > 0.0 ? 1.0 / nu : 0.0;
+      // coefficient of w on the final v_p: (q - V s)^H w / nu = (q.w - s^H a) / nu
+      S sa = S(0.0);
+      for (int j = 0; j < p; ++j) sa += sconj(sv[j]) * av[j];
+      const S cp = (qw - sa) * S(in);
+      for (int j = 0; j < p; ++j) Hc[j + p * (m + 1)] = av[j];
+      Hc[p + p * (m + 1)] = cp;
+      in_sh = in;
+      gm_sh = cp * S(in);
+      return true;
+    };
+    go_sh = scalar_part() ? 1 : 0;
+  }
+  __syncthreads();
   if (final_pass) return;
-  const double in = nu > 0.0 ? 1.0 / nu : 0.0;
-  // coefficient of w on the final v_p: (q - V s)^H w / nu = (q.w - s^H a) / nu
-  S sa = S(0.0);
-  for (int j = 0; j < p; ++j) sa += sconj(S(sg) * d[j * nrhs + c]) * d[(p + j) * nrhs + c];
-  const S cp = (qw - sa) * S(in);
-  const S gm = cp * S(in);
-  for (int j = 0; j < p; ++j) {
-    const S sr = d[j * nrhs + c], aj = d[(p + j) * nrhs + c];
-    const S sj = S(sg) * sr;
-    Hc[j + p * (m + 1)] = aj;
-    sco[j * nrhs + c] = sr;
-    gco[j * nrhs + c] = aj - gm * sj;
+  if (!go_sh) {
+    for (int j = lane; j <= p; j += kWave) coef[(int64_t)j * nrhs + c] = S(0.0);
+    return;
   }
-  Hc[p + p * (m + 1)] = cp;
-  inu[c] = S(sg * in);
-  gam[c] = gm * S(sg);
+  const double in = in_sh;
+  const S gm = gm_sh;
+  // R's column p and the update's coefficients (lanes over rows of R)
+  for (int i = lane; i < p; i += kWave) {
+    S rsv = S(0.0), rg = S(0.0);
+    for (int j = i; j < p; ++j) {
+      const S r = Rc[i * ldr + j];
+      rsv += r * sv[j];
+      rg += r * (av[j] - gm * sv[j]);
+    }
+    Rc[i * ldr + p] = -rsv * S(in);
+    coef[(int64_t)i * nrhs + c] = rg;
+  }
+  if (lane == 0) {
+    Rc[p * ldr + p] = S(sg * in);
+    coef[(int64_t)p * nrhs + c] = S(sg) * gm;
+  }
 }
 
 // Sums the nblk partials of one k_dcgs2_dots launch into the rows of the
