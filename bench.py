@@ -83,6 +83,8 @@ def main():
                     help="inner Schur iteration 'maxits,tol' (e.g. 5,1e-2, the reference's default); off if empty")
     ap.add_argument("--probe", action="store_true",
                     help="counter probe: --nrhs FP32 applies of the 128^3 operator and a 1 GiB copy")
+    ap.add_argument("--probe-fgmres", action="store_true",
+                    help="counter workload: the calibration copy, then one batched mixed FGMRES solve")
     ap.add_argument("--trsv-sweep", default="",
                     help="comma-separated column counts: level-set solve times per launch (FP32 and FP64 applies"
                          " of the 128^3 operator, the launch's own geometry or the one PGS_* variables force)")
@@ -101,7 +103,7 @@ def main():
     if args.serve:
         serve(args)
         return
-    if args.probe:
+    if args.probe or args.probe_fgmres:
         probe(args)
         return
     if args.spmv_probe:
@@ -574,14 +576,31 @@ def probe(args):
     """Workload for hardware-counter passes (rocprofv3 --pmc): one warm-up
     apply of the 128^3 operator (FP32, --nrhs right-hand sides), a 1 GiB
     device-to-device copy whose known size calibrates FETCH_SIZE and
-    WRITE_SIZE, then three applies."""
+    WRITE_SIZE, then three applies; with --probe-fgmres the copy, then one
+    batched mixed FGMRES solve of the benchmark (delayed CGS2) instead."""
     import pargemslr_amd as pg
 
     cfg = CONFIGS["c2"]
     A = pg.laplacian(*cfg["grid"])
     G = pg.Gemslr(A, pg.config_params(cfg["nlev"], cfg["ncomp"], cfg["droptol"], cfg["rownnz"], cfg["rank"]), seed=0)
-    D = pg.DevOp(G, prec_mask=pg.PGS_PREC_FP32)
     nr = args.nrhs
+    if args.probe_fgmres:
+        D = pg.DevOp(G, prec_mask=pg.PGS_PREC_FP64 | pg.PGS_PREC_FP32)
+        W = pg.Workspace(D, max_nrhs=nr, kdim=cfg["kdim"])
+        Xs = np.stack([pg.rng_uniform(c, A.n) for c in range(nr)], axis=1)
+        B = pg.DeviceArray.from_numpy(A.matvec(Xs))
+        X = pg.DeviceArray((A.n, nr))
+        nb = 1 << 30
+        src, dst = pg.DeviceArray(nb // 8), pg.DeviceArray(nb // 8)
+        pg.capi._chk(pg.lib().pgs_memcpy_dtod(0, dst.ptr, src.ptr, nb, None), "pgs_memcpy_dtod")
+        pg.synchronize()
+        log(f"[probe] copied {nb} bytes")
+        res = W.fgmres(B, X, nrhs=nr, kdim=cfg["kdim"], maxits=1000, tol=cfg["tol"], prec=pg.PGS_PREC_FP32,
+                       orth=pg.PGS_ORTH_DCGS2)
+        pg.synchronize()
+        log(f"[probe] fgmres done, iterations {min(r[1] for r in res)}-{max(r[1] for r in res)}")
+        return
+    D = pg.DevOp(G, prec_mask=pg.PGS_PREC_FP32)
     W = pg.Workspace(D, max_nrhs=nr, kdim=2)
     V = pg.DeviceArray.from_numpy(np.stack([pg.rng_normal(c, A.n) for c in range(nr)], axis=1))
     Z = pg.DeviceArray(V.shape)
