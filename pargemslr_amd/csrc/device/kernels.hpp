@@ -889,7 +889,8 @@ __global__ __launch_bounds__(64) void k_dcgs2_fix(int nrhs, int m, int p, int ma
       KState s = st[c];
       if (!s.active) return false;
       const double eps = 2.220446049250313e-16;
-      // d rows: s_j (j < p), then a_j (j < p), then q.q, q.w (final: s_j, q.q)
+      // d rows: slot_j^H q (j < p), then slot_j^H w (j < p), then q.q, q.w
+      // (final: slot_j^H q, q.q)
       const double qq = sg * sg * (double)sreal(final_pass ? d[p * nrhs + c] : d[2 * p * nrhs + c]);
       const S qw = final_pass ? S(0.0) : S(sg) * d[(2 * p + 1) * nrhs + c];
       double nu = 1.0;
